@@ -1,0 +1,316 @@
+"""Pins for the FP64 oracle: every check compares the oracle with something
+that is not the oracle's own formula -- a brute-force DFT, a closed form, an
+integration-by-parts identity, or finite differences of another oracle
+function (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.lq_oracle import (INNER_H, INNER_HL, INNER_L2, Grid, Oracle,
+                              OracleStop, Params, fftbwd, fftfwd)
+from oracle.seeded import seeded_field
+from tests.fields import abc_field, bandlimit, grid_xyz, mode_index, rel
+
+TP = 2.0 * np.pi
+
+
+def _dft3(u, sign):
+    n = u.shape[-1]
+    idx = np.arange(n)
+    W = np.exp(sign * 2j * np.pi * np.outer(idx, idx) / n)
+    out = np.einsum("az,by,cx,zyx->abc", W, W, W, u.astype(np.complex128))
+    return out
+
+
+def test_fft_bruteforce_dft():
+    rng = np.random.default_rng(0)
+    for n in (4, 6, 8):
+        u = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+        F = fftfwd(u)
+        assert rel(F, _dft3(u, -1.0)) < 1e-13
+        assert rel(fftbwd(F), _dft3(F, +1.0) / n ** 3) < 1e-13
+
+
+def test_fft_nan_stops():
+    u = np.zeros((4, 4, 4))
+    u[1, 2, 3] = np.nan
+    with pytest.raises(OracleStop):
+        fftfwd(u)
+
+
+def test_wavenumbers():
+    g = Grid(8)
+    expect = TP * np.array([0, 1, 2, 3, 4, -3, -2, -1], dtype=float)
+    assert np.array_equal(g.k1d, expect)          # Nyquist is +pi*n
+    assert g.k1d[4] == np.pi * 8
+    assert g.dV == 1.0 / 512
+
+
+@pytest.mark.parametrize("p,mcut", [(2.0, 2), (1.0, 4), (1.5, 3), (0.0, 4), (3.0, 2)])
+def test_dealias_bruteforce(p, mcut):
+    n = 8
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((n, n, n))
+    orc = Oracle(Params(n=n))
+    F = _dft3(u, -1.0)
+    m = np.abs(mode_index(n))
+    keep = np.maximum(np.maximum(m[:, None, None], m[None, :, None]), m[None, None, :]) <= mcut
+    expect = (_dft3(F * keep, +1.0) / n ** 3).real
+    assert rel(orc.dealias_scalar(u, p), expect) < 1e-13
+
+
+def test_gradient_closed_form_and_nyquist():
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    orc = Oracle(Params(n=n))
+    a, b, c = 1, -2, 3
+    ph = TP * (a * X + b * Y + c * Z)
+    gu = orc.gradient(np.sin(ph))
+    for j, kj in enumerate((a, b, c)):
+        assert rel(gu[j], TP * kj * np.cos(ph)) < 1e-12
+    # Nyquist rule: REAL(ifft(iK fft)) kills the own-dimension Nyquist mode
+    nyq = np.cos(np.pi * n * X)                      # (-1)^i
+    g2 = orc.gradient(nyq * np.sin(TP * Y))
+    assert np.max(np.abs(g2[0])) < 1e-12
+    assert rel(g2[1], TP * nyq * np.cos(TP * Y)) < 1e-12
+    assert np.max(np.abs(g2[2])) < 1e-12
+
+
+def test_divergence_closed_form():
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    orc = Oracle(Params(n=n))
+    u = np.stack([np.sin(TP * X), np.cos(TP * 2 * Z) * np.sin(TP * Y), np.cos(TP * Z)])
+    expect = TP * np.cos(TP * X) + TP * np.cos(TP * 2 * Z) * np.cos(TP * Y) - TP * np.sin(TP * Z)
+    assert rel(orc.divergence(u), expect) < 1e-12
+
+
+def test_laplacian_abc():
+    n = 16
+    u = abc_field(n, 1.0, 0.7, 0.3)
+    orc = Oracle(Params(n=n))
+    assert rel(orc.laplacian(u), -4 * np.pi ** 2 * u) < 1e-12
+
+
+def test_poisson_closed_form():
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    orc = Oracle(Params(n=n))
+    f = np.cos(TP * X) + 2.0 * np.sin(TP * (Y + 2 * Z)) + 3.0
+    expect = -np.cos(TP * X) / (4 * np.pi ** 2) - 2.0 * np.sin(TP * (Y + 2 * Z)) / (20 * np.pi ** 2)
+    assert rel(orc.solve_poisson(f), expect) < 1e-12
+
+
+def test_leray_invariants():
+    n = 16
+    orc = Oracle(Params(n=n))
+    u = bandlimit(np.random.default_rng(2).standard_normal((3, n, n, n)), 5)
+    Pu = orc.divAvg_free(u)
+    assert np.max(np.abs(orc.divergence(Pu))) < 1e-10 * np.max(np.abs(u)) * TP * n
+    assert rel(orc.divAvg_free(Pu), Pu) < 1e-12              # idempotent
+    assert np.max(np.abs(Pu.mean(axis=(1, 2, 3)))) < 1e-14   # mean removed
+    psi = bandlimit(np.random.default_rng(3).standard_normal((n, n, n)), 5)
+    assert np.max(np.abs(orc.divAvg_free(orc.gradient(psi)))) < 1e-11 * np.max(np.abs(orc.gradient(psi)))
+    w = abc_field(n)                                          # div-free, mean 0
+    assert rel(orc.divAvg_free(w), w) < 1e-12
+    # u = w + grad psi + const  ->  P u = w
+    uu = w + orc.gradient(psi) + 0.5
+    assert rel(orc.divAvg_free(uu), w) < 1e-11
+
+
+def test_hilbert_single_mode():
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    q = 4.0
+    orc = Oracle(Params(n=n, q=q))
+    s = 1.5 - 1.0 / q
+    ph = TP * (X + 2 * Y)
+    u = np.stack([np.sin(ph), 0 * ph, np.cos(ph)])
+    k = TP * math.sqrt(5.0)
+    fac = 1.0 / (1.0 + (0.1 * k) ** (2 * s))
+    assert rel(orc.HilbertGradient(u, s), fac * u) < 1e-12
+
+
+def test_inner_products_closed_form():
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    q = 6.0
+    orc = Oracle(Params(n=n, q=q))
+    s = 1.5 - 1.0 / q
+    u = np.stack([np.sin(TP * X), np.cos(TP * Y), 0 * X])
+    v = np.stack([np.sin(TP * X), 2.0 * np.cos(TP * Y), np.ones_like(X)])
+    ip = orc.global_summed_field_inner_product
+    assert abs(ip(u, v, INNER_L2) - (0.5 + 1.0)) < 1e-14
+    wl = (1.0 + (0.1 * TP) ** s) ** 2
+    assert abs(ip(u, v, INNER_HL) - wl * 1.5) < 1e-13
+    wh = (1.0 + TP) ** (2 * s)
+    assert abs(ip(u, v, INNER_H) - wh * 1.5) < 1e-12
+    # constant (k = 0) parts: weight 1 in both Sobolev products
+    c = np.stack([np.ones_like(X)] * 3)
+    assert abs(ip(c, c, INNER_HL) - 3.0) < 1e-14
+    with pytest.raises(OracleStop):
+        orc.inner_product(u[0], u[0], "H_l^((3q-1)/(2q))")
+
+
+@pytest.mark.parametrize("k", [0.0, 0.5, 1.0, 2.0, 3.0, 5.0, 7.0])
+def test_gk_order2_polynomial(k):
+    n = 32
+    X, Y, Z = grid_xyz(n)
+    g = 2.0 + np.cos(TP * X) + 0.5 * np.sin(TP * Y)          # positive, modes <= 1
+    orc = Oracle(Params(n=n))
+    assert rel(orc.calc_gk_order2(g, k), g ** k) < 1e-12
+
+
+@pytest.mark.parametrize("k", [0.0, 1.0, 2.0, 2.5, 4.5, 7.0])
+def test_calc_uk_constant_modulus(k):
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    c = 1.7
+    u = c * np.stack([np.sin(TP * Z), np.cos(TP * Z), 0 * Z])   # |u| = c
+    orc = Oracle(Params(n=n))
+    assert rel(orc.calc_uk(u, k), np.full_like(X, c ** k)) < 1e-12
+
+
+@pytest.mark.parametrize("q", [4.0, 6.0, 9.0])
+def test_lq_norm_closed_form(q):
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    c = 0.8
+    u = c * np.stack([np.sin(TP * Z), np.cos(TP * Z), 0 * Z])
+    orc = Oracle(Params(n=n, q=q))
+    assert abs(orc.calc_global_Lq_norm(u) - c) < 1e-13
+    B = 2.0
+    assert abs(orc.calc_global_Lq_norm(orc.rescaleLqNorm(u, B)) - B) < 1e-13
+
+
+def test_lq_norm_abc_q4():
+    n = 32
+    u = abc_field(n)
+    orc = Oracle(Params(n=n, q=4.0))
+    s = (u * u).sum(axis=0)                     # (u.u)^2 has modes <= 4 < n/3
+    assert abs(orc.calc_global_Lq_norm(u) - np.mean(s * s) ** 0.25) < 1e-13
+
+
+def _spectral_grad(f):
+    n = f.shape[-1]
+    K = TP * mode_index(n).astype(float)
+    K[n // 2] = 0.0  # band-limited fields carry no Nyquist content
+    fh = np.fft.fftn(f)
+    return np.stack([np.fft.ifftn(1j * K[None, None, :] * fh).real,
+                     np.fft.ifftn(1j * K[None, :, None] * fh).real,
+                     np.fft.ifftn(1j * K[:, None, None] * fh).real])
+
+
+def test_R_q2_closed_form():
+    n = 16
+    u = seeded_field(n, 7, "gauss", 16.0, q=2.0, B=1.0)
+    orc = Oracle(Params(n=n, q=2.0))
+    g = Grid(n)
+    uh = np.fft.fftn(u, axes=(-3, -2, -1))
+    Keff = g.k1d.copy()
+    Keff[n // 2] = 0.0
+    k2 = (Keff[None, None, :] ** 2 + Keff[None, :, None] ** 2 + Keff[:, None, None] ** 2)
+    expect = -np.sum(k2 * np.abs(uh) ** 2) / n ** 6
+    R = orc.calc_local_dLqdt_inclParts(u)
+    assert abs(R[0] - expect) < 1e-12 * abs(expect)
+    assert R[2] == 0.0
+
+
+@pytest.mark.parametrize("q", [4.0, 6.0])
+def test_R_abc_closed_form(q):
+    n = 32
+    u = abc_field(n)
+    orc = Oracle(Params(n=n, q=q))
+    s = (u * u).sum(axis=0)
+    Lqq = np.mean(s ** (q / 2.0))
+    R = orc.calc_local_dLqdt_inclParts(u)
+    assert abs(R[1] - (-4 * np.pi ** 2 * Lqq)) < 1e-11 * 4 * np.pi ** 2 * Lqq
+    assert abs(R[2]) < 1e-10 * abs(R[1])      # Beltrami: pressure part vanishes
+
+
+@pytest.mark.parametrize("q,M", [(4.0, 2), (6.0, 1)])
+def test_R_integration_by_parts_identity(q, M):
+    """R1+R2 = nu int |u|^(q-2) u.Lap u and R3 = -int |u|^(q-2) u.grad p for a
+    band-limited divergence-free field (different formulas from the
+    reference's function_ops.f90:1304-1358)."""
+    n = 32
+    orc = Oracle(Params(n=n, q=q))
+    u = orc.divAvg_free(bandlimit(seeded_field(n, 11, "gauss", 16.0, q=q), M))
+    s = (u * u).sum(axis=0)
+    A = s ** ((q - 2.0) / 2.0)
+    gu = np.stack([_spectral_grad(u[i]) for i in range(3)])      # gu[i, j] = d_j u_i
+    lap = np.stack([sum(_spectral_grad(gu[i, j])[j] for j in range(3)) for i in range(3)])
+    src = sum(gu[i, j] * gu[j, i] for i in range(3) for j in range(3))
+    K = TP * mode_index(n).astype(float)
+    ksq = K[None, None, :] ** 2 + K[None, :, None] ** 2 + K[:, None, None] ** 2
+    ksq[0, 0, 0] = 1.0
+    ph = np.fft.fftn(src) / ksq
+    ph[0, 0, 0] = 0.0
+    p = np.fft.ifftn(ph).real                                    # p = -Lap^-1 src
+    gp = _spectral_grad(p)
+    R12 = np.mean(A * (u * lap).sum(axis=0))
+    R3 = -np.mean(A * (u * gp).sum(axis=0))
+    R = orc.calc_local_dLqdt_inclParts(u)
+    assert abs(R[1] - R12) < 1e-11 * abs(R12)
+    assert abs(R[2] - R3) < 1e-9 * abs(R12)
+    assert abs(R3) > 1e-6 * abs(R12)           # the pressure part is exercised
+
+
+@pytest.mark.parametrize("q,M,tol", [(4.0, 2, 1e-7), (6.0, 1, 1e-6), (9.0, 1, 2e-3)])
+def test_gradient_fd_bandlimited(q, M, tol):
+    n = 32
+    orc = Oracle(Params(n=n, q=q))
+    u = orc.divAvg_free(bandlimit(seeded_field(n, 21, "gauss", 16.0, q=q), M))
+    g = orc.GradL2ForLq(u)
+    h = 1e-4
+    for seed in (1001, 1002):
+        phi = orc.divAvg_free(bandlimit(seeded_field(n, seed, "gauss", 16.0, q=q), M))
+        fd = (orc.eval_J(u + h * phi, "maxdLqdt") - orc.eval_J(u - h * phi, "maxdLqdt")) / (2 * h)
+        an = orc.global_summed_field_inner_product(g, phi, INNER_L2)
+        assert abs(fd - an) < tol * abs(an), (fd, an)
+
+
+def test_gradient_q2_closed_form():
+    """q = 2: P(GradL2ForLq(u)) = 2 nu Lap u for band-limited div-free u."""
+    n = 16
+    orc = Oracle(Params(n=n, q=2.0))
+    u = orc.divAvg_free(bandlimit(seeded_field(n, 5, "gauss", 16.0, q=2.0), 3))
+    Pg = orc.divAvg_free(orc.GradL2ForLq(u))
+    assert rel(Pg, 2.0 * orc.laplacian(u)) < 1e-11
+
+
+def test_constraint_derivative_fd():
+    n = 32
+    q = 4.0
+    orc = Oracle(Params(n=n, q=q))
+    u = orc.divAvg_free(bandlimit(seeded_field(n, 31, "gauss", 16.0, q=q), 2))
+    phi = orc.divAvg_free(bandlimit(seeded_field(n, 32, "gauss", 16.0, q=q), 2))
+    nrm = orc.calcConstraintDerivativeL2(u)
+    h = 1e-5
+    fd = (orc.calc_global_Lq_norm(u + h * phi) - orc.calc_global_Lq_norm(u - h * phi)) / (2 * h)
+    an = orc.global_summed_field_inner_product(nrm, phi, INNER_L2)
+    assert abs(fd - an) < 1e-8 * abs(an)
+
+
+def test_vector_transport_tangency():
+    n = 32
+    q, B = 4.0, 1.5
+    orc = Oracle(Params(n=n, q=q, B=B))
+    u = orc.rescaleLqNorm(orc.divAvg_free(bandlimit(seeded_field(n, 41, "gauss", 16.0, q=q), 2)), B)
+    eta = 0.3 * orc.divAvg_free(bandlimit(seeded_field(n, 42, "gauss", 16.0, q=q), 2))
+    xi1 = orc.divAvg_free(bandlimit(seeded_field(n, 43, "gauss", 16.0, q=q), 2))
+    xi2 = orc.divAvg_free(bandlimit(seeded_field(n, 44, "gauss", 16.0, q=q), 2))
+    T1 = orc.vectorTransport(u, q, B, eta, xi1)
+    T2 = orc.vectorTransport(u, q, B, eta, xi2)
+    T12 = orc.vectorTransport(u, q, B, eta, 2.0 * xi1 - 3.0 * xi2)
+    assert rel(T12, 2.0 * T1 - 3.0 * T2) < 1e-12                  # linear in xi
+    upe = u + eta
+    nrm = (upe * upe).sum(axis=0) * upe                            # |u+eta|^2 (u+eta)
+    ip = orc.global_summed_field_inner_product
+    assert abs(ip(nrm, T1)) < 1e-11 * math.sqrt(ip(nrm, nrm) * ip(T1, T1))
+    # with eta = 0 on the sphere the transport is the tangent projection
+    T0 = orc.vectorTransport(u, q, B, 0 * u, xi1)
+    n0 = (u * u).sum(axis=0) * u
+    c = ip(n0, xi1) / np.mean(((u * u).sum(axis=0)) ** 2)
+    assert rel(T0, xi1 - c * u) < 1e-11
