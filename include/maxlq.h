@@ -1,0 +1,196 @@
+/*
+ * maxlq.h -- C ABI of the MI355X-native maxLq RCG hot path (libmaxlq.so).
+ *
+ * Every host array on this boundary uses the reference layout: FP64,
+ * Fortran column-major with x fastest; a scalar field is n1*n2*n3 doubles
+ * (reference dims (1:n(1),1:n(2),1:local_N), local_N = n3 on one GPU), a
+ * vector field is 3 such blocks (dims (...,1:3)).  Complex arrays are
+ * interleaved (re, im) doubles.  Fields named by an ``mlq_field`` handle live
+ * in the device's HBM for the lifetime of the context; host arrays are only
+ * read or written inside the call that receives them.
+ *
+ * Threading: one host thread per context, one HIP stream per context, not
+ * re-entrant.  Scalars are returned synchronously.  All functions return an
+ * MLQ_* status; algorithm outcomes (mnbrak flags, stop reasons) are carried
+ * in result structs.  Each declaration cites the reference routine it
+ * replaces (file:line in uniFabiB/maxLq).
+ */
+#ifndef MAXLQ_H
+#define MAXLQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MLQ_OK 0
+#define MLQ_ENAN 1       /* NaN in a transform / unit vector: the reference does `stop 1`
+                            (fftwfunction.f90:78-89, function_ops.f90:1843-1854) */
+#define MLQ_EBADKIND 2   /* unknown/old inner-product kind: reference `call exit`
+                            (function_ops.f90:3485-3495) */
+#define MLQ_EHIP 3       /* HIP runtime error */
+#define MLQ_EOOM 4       /* device allocation failed */
+#define MLQ_EINVAL 5     /* bad argument (size not supported, bad handle, ...) */
+#define MLQ_EUNSUP 6     /* configuration outside the implemented path */
+
+/* precision modes */
+#define MLQ_FP64 0       /* all field arithmetic in FP64 (parity path) */
+#define MLQ_MIXED 1      /* FP32 transforms and pointwise work, FP64 reductions/scalars */
+
+/* inner-product kinds (the reference's `mytype` strings, function_ops.f90:3426-3496) */
+#define MLQ_IP_L2 0      /* "L2" */
+#define MLQ_IP_HL 1      /* "H_l^(3/2-1/q)" */
+#define MLQ_IP_H 2       /* "H^(3/2-1/q)" */
+#define MLQ_IP_OLD_H 3   /* "H^((3q-1)/(2q))"   -> MLQ_EBADKIND */
+#define MLQ_IP_OLD_HL 4  /* "H_l^((3q-1)/(2q))" -> MLQ_EBADKIND */
+
+/* eval_J systems (optimization.f90:533-545) */
+#define MLQ_J_MAXDLQDT 0 /* "maxdLqdt": R(u) */
+#define MLQ_J_LINEMIN 1  /* "LineMin":  -R(B u/||u||_q) */
+#define MLQ_J_LQNORM 2   /* "||u||_q" */
+
+/* The hot-path subset of global_variables.f90 mirrored into one struct. */
+typedef struct mlq_params {
+  double q;              /* lebesgueQ */
+  double B;              /* constraintB (global_variables.f90:115) */
+  double visc;           /* visc = 1 (:8) */
+  double lambda1;        /* lambda1 = 0.1 (:31) */
+  double viscCoefficient;      /* 1 (:113) */
+  double pressureCoefficient;  /* 1 (:114) */
+  int toDealias;         /* .true. (:68) */
+  int use_e_u;           /* -1: automatic as set_e_u_or_just_u (initialize.f90:75-86) */
+  int dealiase_if_mult_by_e_u; /* .true. (:40) */
+  int precision;         /* MLQ_FP64 or MLQ_MIXED */
+} mlq_params;
+
+typedef struct mlq_ctx mlq_ctx;
+typedef int mlq_field;   /* handle of a device-resident vector (3 comp) or scalar field */
+
+/* Fill *p with the reference defaults for exponent q (global_variables.f90). */
+void mlq_default_params(mlq_params* p, double q);
+
+/* ---- context (replaces init_fft / fft_deallocate, fftwfunction.f90:169-212,
+ *      and the grid set-up of initialize, initialize.f90:131-203) ---- */
+int mlq_init(mlq_ctx** ctx, int n1, int n2, int n3, const mlq_params* p);
+int mlq_finalize(mlq_ctx* ctx);
+int mlq_set_params(mlq_ctx* ctx, const mlq_params* p);
+const char* mlq_last_error(const mlq_ctx* ctx);
+/* single-rank slab: local_N = n3, offset 0 (replaces fftw_mpi_local_size_3d,
+ * initialize.f90:149) */
+int mlq_local_size_3d(const mlq_ctx* ctx, int* local_n, int* local_k_offset);
+int mlq_sync(mlq_ctx* ctx);
+
+/* ---- fftwfunction (fftwfunction.f90:62-165) ---- */
+/* fftfwd(u, fu): unnormalised complex forward transform, :62-92 */
+int mlq_fftfwd(mlq_ctx* ctx, const double* u_cplx, double* fu_cplx);
+/* fftbwd(fu, u): complex backward transform divided by n1*n2*n3, :97-113 */
+int mlq_fftbwd(mlq_ctx* ctx, const double* fu_cplx, double* u_cplx);
+/* fftfwdr(u, fu): real -> (n1/2+1, n2, n3) half spectrum, :118-134 */
+int mlq_fftfwdr(mlq_ctx* ctx, const double* u_real, double* fu_half);
+/* fftbwdr(fu, u): half spectrum -> real, divided by n1*n2*n3, :139-165 */
+int mlq_fftbwdr(mlq_ctx* ctx, const double* fu_half, double* u_real);
+/* fftfwdV / fftbwdV: the vector forms, :24-57 */
+int mlq_fftfwdV(mlq_ctx* ctx, const double* vec_cplx, double* fvec_cplx);
+int mlq_fftbwdV(mlq_ctx* ctx, const double* fvec_cplx, double* vec_cplx);
+
+/* ---- function_ops field routines, host arrays (function_ops.f90) ---- */
+int mlq_gradient(mlq_ctx* ctx, const double* u, double* grad_u);          /* :3147-3195 */
+int mlq_divergence(mlq_ctx* ctx, const double* u, double* divU);          /* :3200-3243 */
+int mlq_laplacian(mlq_ctx* ctx, double* f);                               /* :3248-3282 */
+int mlq_solve_poisson(mlq_ctx* ctx, const double* f, double l, double* phi); /* :1951-1996 */
+int mlq_dealias_scalar(mlq_ctx* ctx, double* g, double nonlinOrder);      /* :3332-3350 */
+int mlq_divAvg_free(mlq_ctx* ctx, double* myfield);                       /* :2843-2918 */
+int mlq_HilbertGradient(mlq_ctx* ctx, double* grad, double order);        /* :3583-3627 */
+int mlq_calc0thFourierModes(mlq_ctx* ctx, const double* vec, double out[3]); /* :3048-3094 */
+int mlq_calc_uk(mlq_ctx* ctx, const double* u, double k, double* uk);     /* :2324-2343 */
+int mlq_calc_gk_order2(mlq_ctx* ctx, const double* g, double k, double* gk); /* :2281-2319 */
+int mlq_calc_global_Lq_norm(mlq_ctx* ctx, const double* u, double* norm); /* :1675-1692 */
+int mlq_rescaleLqNorm(mlq_ctx* ctx, double* u, double B);                 /* :1699-1709 */
+int mlq_calcConstraintDerivativeL2(mlq_ctx* ctx, const double* u, double* out); /* :984-1006 */
+int mlq_vectorTransport(mlq_ctx* ctx, const double* u, double q, double B,
+                        const double* eta, const double* xi, double* out); /* :1716-1768 */
+int mlq_inner_product(mlq_ctx* ctx, const double* f, const double* g, int kind,
+                      double* out);                                       /* :3404-3503 */
+int mlq_global_inner_product(mlq_ctx* ctx, const double* f, const double* g, int kind,
+                             double* out);                                /* :3508-3521 */
+int mlq_global_summed_field_inner_product(mlq_ctx* ctx, const double* f, const double* g,
+                                          int kind, double* out);         /* :3557-3577 */
+int mlq_calc_local_dLqdt_inclParts(mlq_ctx* ctx, const double* u, double q,
+                                   double out[3]);                        /* :1304-1358 */
+int mlq_calc_global_dLqdt_inclParts(mlq_ctx* ctx, const double* u, double q,
+                                    double out[3]);                       /* :1392-1405 */
+int mlq_GradL2ForLq(mlq_ctx* ctx, const double* U, double* V);            /* :1066-1211 */
+
+/* ---- optimization (optimization.f90) ---- */
+/* eval_J(myfield, mysystem), :520-547 */
+int mlq_eval_J(mlq_ctx* ctx, const double* field, int mysystem, double* J);
+
+/* ---- device-resident fields ---- */
+int mlq_field_alloc(mlq_ctx* ctx, int ncomp, mlq_field* f);
+int mlq_field_free(mlq_ctx* ctx, mlq_field f);
+int mlq_upload(mlq_ctx* ctx, mlq_field f, const double* host);
+int mlq_download(mlq_ctx* ctx, mlq_field f, double* host);
+/* the project's seeded divergence-free field, written into f and rescaled to
+ * ||u||_q = B (kind 0: exp(-|m|^2/a), kind 1: |m|^2 exp(-|m|^2/a)) */
+int mlq_seeded_field(mlq_ctx* ctx, mlq_field f, unsigned long long seed, int kind, double a);
+
+/* J_LineMin(tau_i) = eval_J(u + tau_i d, "LineMin") for ntau step sizes in one
+ * batched evaluation (the J evaluations of mnbrak/brent, optimization.f90:666-752,
+ * 833-908) */
+int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
+                     double* J);
+
+/* ---- the whole RCG solve for one B: maxdLqdt, optimization.f90:7-514 ---- */
+typedef struct mlq_rcg_cfg {
+  int max_iter;          /* MAX_ITER */
+  double optim_tol;      /* OPTIM_TOL (initialize.f90:95-100) */
+  int start_iter;        /* optimizationIterOffset when resuming (:81-86) */
+  int batch_linesearch;  /* evaluate speculative mnbrak/brent step sizes in batches */
+  int reset_every;       /* resetMomentumTermEveryXiterations (25) */
+  int rolling;           /* toleranceRollingAverageSize (10) */
+  int flags;             /* reserved, 0 */
+} mlq_rcg_cfg;
+
+typedef struct mlq_rcg_result {
+  double J1;             /* final R(u) */
+  int iters;             /* final iteration counter */
+  int successful;        /* optimizationSuccessful */
+  int while_flag;
+  int stop_reason;       /* 0 converged, 1 max iter, 2 mnbrak flag 1, 3 mnbrak flag 2,
+                            4 tau > TAU_MAX, 5 cost not increasing */
+  int n_J;               /* J evaluations requested by mnbrak+brent (all iterations) */
+  int n_J_computed;      /* J evaluations computed (>= n_J with speculation) */
+  double seconds;
+} mlq_rcg_result;
+
+/* log record kinds delivered to the callback (data_ops.f90 writers) */
+#define MLQ_LOG_ITER 1     /* save_diagnostics_optim: i0=iter, d[0..2]=tau,beta,J (:764-799) */
+#define MLQ_LOG_LINEMIN 2  /* save_linemin_data: i0=brakIter, i1=0 replace/1 append,
+                              d[0..5]=tA,tB,tC,FA,FB,FC (:1403-1442) */
+#define MLQ_LOG_BRENT 3    /* brent-info: i0=j, d[0..1]=X,FX (optimization.f90:836-946) */
+#define MLQ_LOG_MSG 4      /* optim_msg_handle: i0=message code (data_ops.f90:1571-1632) */
+#define MLQ_LOG_ERR 5      /* optim_error_handle: i0=error code (:1527-1566) */
+typedef void (*mlq_log_cb)(void* user, int kind, int iter, int i0, int i1, const double* d);
+
+int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_result* res,
+                  mlq_log_cb cb, void* user);
+
+/* ---- measurement helpers ---- */
+/* time `reps` launches of the 3-D real transform pair of a batch of `batch`
+ * fields on the context stream with HIP events; per-pass average kernel
+ * durations (ms) are returned in t_ms[0..5] = {x fwd, y fwd, z fwd, z inv, y inv, x inv} */
+int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms);
+/* bytes of one real field and of one half spectrum in the active precision */
+int mlq_sizes(const mlq_ctx* ctx, double* real_bytes, double* spec_bytes);
+/* per-kernel-class timing accumulated while profiling is on (HIP events on the
+ * context stream); class ids are listed by mlq_prof_name */
+int mlq_prof_enable(mlq_ctx* ctx, int on);
+int mlq_prof_get(mlq_ctx* ctx, int cls, long long* count, double* total_ms, double* bytes);
+const char* mlq_prof_name(int cls);
+int mlq_mem_info(mlq_ctx* ctx, double* used_bytes, double* peak_bytes, double* device_free,
+                 double* device_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAXLQ_H */

@@ -1,0 +1,412 @@
+// Engine<T> definitions: memory, transforms, primitives.
+#pragma once
+#include "mlq_engine_impl.hpp"
+
+namespace mlq {
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
+  n_ = n;
+  px_ = n / 2 + 1;
+  nr_ = (long long)n * n * n;
+  ns_ = (long long)n * n * px_;
+  resolve_params();
+  MLQ_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  // twiddles exp(-2 pi i k/N), computed in FP64 on the host
+  std::vector<C> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    tw[k].x = (T)std::cos(a);
+    tw[k].y = (T)std::sin(a);
+  }
+  tw_ = (C*)dmalloc(sizeof(C) * n);
+  MLQ_HIP(hipMemcpy(tw_, tw.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+  scratch_ = (C*)dmalloc(sizeof(C) * ns_ * MAXB);
+  part_ = (double*)dmalloc(sizeof(double) * RED_BLOCKS * RED_MAXK);
+  red_dev_ = (double*)dmalloc(sizeof(double) * 64);
+  MLQ_HIP(hipHostMalloc((void**)&red_host_, sizeof(double) * 64, hipHostMallocDefault));
+}
+
+template <typename T> Engine<T>::~Engine() {
+  hipStreamSynchronize(st_);
+  for (auto& f : fields_) {
+    if (f.phys) hipFree(f.phys);
+    if (f.spec) hipFree(f.spec);
+  }
+  for (auto& kv : pool_)
+    for (auto p : kv.second) hipFree(p);
+  for (auto& r : prof_recs_) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto e : ev_pool_) hipEventDestroy(e);
+  hipFree(tw_);
+  hipFree(scratch_);
+  hipFree(part_);
+  hipFree(red_dev_);
+  hipHostFree(red_host_);
+  hipStreamDestroy(st_);
+}
+
+template <typename T> void Engine<T>::resolve_params() {
+  if (p_.use_e_u < 0) use_e_u_ = p_.q < 4.0 - 2.0e-16;   // initialize.f90:79-85
+  else use_e_u_ = p_.use_e_u != 0;
+}
+
+template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(MLQ_EOOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  used_ += bytes;
+  peak_ = std::max(peak_, used_);
+  return p;
+}
+template <typename T> void Engine<T>::dfree(void* p, size_t bytes) {
+  if (!p) return;
+  MLQ_HIP(hipStreamSynchronize(st_));
+  MLQ_HIP(hipFree(p));
+  used_ -= bytes;
+}
+
+template <typename T> typename Engine<T>::RT Engine<T>::rtmp(int k) {
+  const size_t bytes = sizeof(T) * nr_ * k;
+  auto& v = pool_[bytes];
+  void* p;
+  if (!v.empty()) { p = v.back(); v.pop_back(); }
+  else p = dmalloc(bytes);
+  return RT(this, (T*)p, bytes);
+}
+template <typename T> typename Engine<T>::ST Engine<T>::stmp(int k) {
+  const size_t bytes = sizeof(C) * ns_ * k + 64;  // distinct key from real blocks
+  auto& v = pool_[bytes];
+  void* p;
+  if (!v.empty()) { p = v.back(); v.pop_back(); }
+  else p = dmalloc(bytes);
+  return ST(this, (C*)p, bytes);
+}
+
+template <typename T> DevField& Engine<T>::fld(int h) {
+  if (h < 0 || h >= (int)fields_.size() || fields_[h].ncomp == 0)
+    throw Error(MLQ_EINVAL, "bad field handle " + std::to_string(h));
+  return fields_[h];
+}
+template <typename T> const DevField& Engine<T>::fld(int h) const {
+  if (h < 0 || h >= (int)fields_.size() || fields_[h].ncomp == 0)
+    throw Error(MLQ_EINVAL, "bad field handle " + std::to_string(h));
+  return fields_[h];
+}
+
+template <typename T> int Engine<T>::field_alloc(int ncomp) {
+  if (ncomp != 1 && ncomp != 3) throw Error(MLQ_EINVAL, "ncomp must be 1 or 3");
+  DevField f;
+  f.ncomp = ncomp;
+  f.phys = dmalloc(sizeof(T) * nr_ * ncomp);
+  MLQ_HIP(hipMemsetAsync(f.phys, 0, sizeof(T) * nr_ * ncomp, st_));
+  for (size_t i = 0; i < fields_.size(); ++i)
+    if (fields_[i].ncomp == 0) { fields_[i] = f; return (int)i; }
+  fields_.push_back(f);
+  return (int)fields_.size() - 1;
+}
+template <typename T> void Engine<T>::field_free(int h) {
+  DevField& f = fld(h);
+  dfree(f.phys, sizeof(T) * nr_ * f.ncomp);
+  if (f.spec) dfree(f.spec, sizeof(C) * ns_ * f.ncomp);
+  f = DevField();
+}
+
+template <typename T> void Engine<T>::upload(int h, const double* host) {
+  DevField& f = fld(h);
+  const long long cnt = nr_ * f.ncomp;
+  if (sizeof(T) == sizeof(double)) {
+    MLQ_HIP(hipMemcpyAsync(f.phys, host, sizeof(double) * cnt, hipMemcpyHostToDevice, st_));
+  } else {
+    std::vector<T> tmp(cnt);
+    for (long long i = 0; i < cnt; ++i) tmp[i] = (T)host[i];
+    MLQ_HIP(hipMemcpyAsync(f.phys, tmp.data(), sizeof(T) * cnt, hipMemcpyHostToDevice, st_));
+    sync();
+  }
+  f.spec_valid = false;
+  sync();
+}
+template <typename T> void Engine<T>::download(int h, double* host) {
+  DevField& f = fld(h);
+  const long long cnt = nr_ * f.ncomp;
+  if (sizeof(T) == sizeof(double)) {
+    MLQ_HIP(hipMemcpyAsync(host, f.phys, sizeof(double) * cnt, hipMemcpyDeviceToHost, st_));
+    sync();
+  } else {
+    std::vector<T> tmp(cnt);
+    MLQ_HIP(hipMemcpyAsync(tmp.data(), f.phys, sizeof(T) * cnt, hipMemcpyDeviceToHost, st_));
+    sync();
+    for (long long i = 0; i < cnt; ++i) host[i] = (double)tmp[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// profiling (HIP events on the engine stream)
+// ---------------------------------------------------------------------------
+template <typename T> void Engine<T>::prof_begin(int cls) {
+  if (!prof_) return;
+  ProfRec r;
+  r.cls = cls;
+  r.bytes = 0;
+  for (hipEvent_t* e : {&r.a, &r.b}) {
+    if (!ev_pool_.empty()) { *e = ev_pool_.back(); ev_pool_.pop_back(); }
+    else MLQ_HIP(hipEventCreate(e));
+  }
+  MLQ_HIP(hipEventRecord(r.a, st_));
+  prof_recs_.push_back(r);
+}
+template <typename T> void Engine<T>::prof_end(int cls, double bytes) {
+  if (!prof_) return;
+  ProfRec& r = prof_recs_.back();
+  r.bytes = bytes;
+  MLQ_HIP(hipEventRecord(r.b, st_));
+  if (prof_recs_.size() > 4096) prof_flush();
+}
+template <typename T> void Engine<T>::prof_flush() {
+  if (prof_recs_.empty()) return;
+  MLQ_HIP(hipStreamSynchronize(st_));
+  for (auto& r : prof_recs_) {
+    float ms = 0.f;
+    MLQ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    prof_count_[r.cls] += 1;
+    prof_ms_[r.cls] += ms;
+    prof_bytes_[r.cls] += r.bytes;
+    ev_pool_.push_back(r.a);
+    ev_pool_.push_back(r.b);
+  }
+  prof_recs_.clear();
+}
+template <typename T> void Engine<T>::prof_enable(bool on) {
+  prof_flush();
+  prof_ = on;
+  if (on) {
+    for (int i = 0; i < PC_COUNT; ++i) { prof_count_[i] = 0; prof_ms_[i] = 0; prof_bytes_[i] = 0; }
+  }
+}
+template <typename T> void Engine<T>::prof_get(int cls, long long* count, double* ms, double* bytes) {
+  prof_flush();
+  if (cls < 0 || cls >= PC_COUNT) throw Error(MLQ_EINVAL, "bad profile class");
+  *count = prof_count_[cls];
+  *ms = prof_ms_[cls];
+  *bytes = prof_bytes_[cls];
+}
+
+// ---------------------------------------------------------------------------
+// generic launches
+// ---------------------------------------------------------------------------
+template <typename T> template <class F> void Engine<T>::map(const F& f, long long n, int cls) {
+  if (n <= 0) return;
+  long long blocks = (n + MAP_THREADS - 1) / MAP_THREADS;
+  blocks = std::min<long long>(blocks, 256LL * 64);
+  prof_begin(cls);
+  hipLaunchKernelGGL((k_map<F>), dim3((unsigned)blocks), dim3(MAP_THREADS), 0, st_, f, n);
+  MLQ_HIP(hipGetLastError());
+  prof_end(cls, 0);
+}
+
+template <typename T> template <int K, class F>
+void Engine<T>::reduce(const F& f, long long n, double* out) {
+  prof_begin(PC_REDUCE);
+  hipLaunchKernelGGL((k_reduce<K, F>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, f, n, part_);
+  MLQ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st_, part_, RED_BLOCKS, K,
+                     red_dev_);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_REDUCE, 0);
+  MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * K, hipMemcpyDeviceToHost, st_));
+  MLQ_HIP(hipStreamSynchronize(st_));
+  for (int k = 0; k < K; ++k) out[k] = red_host_[k];
+}
+
+template <typename T> double Engine<T>::dot(const T* a, const T* b) {
+  double r;
+  reduce<1>(R_dot<T>{a, b}, nr_, &r);
+  return r;
+}
+template <typename T> double Engine<T>::sum(const T* a) {
+  double r;
+  reduce<1>(R_sum<T>{a}, nr_, &r);
+  return r;
+}
+template <typename T> bool Engine<T>::has_nan(const T* a, long long n) {
+  double r;
+  reduce<1>(R_nan<T>{a}, n, &r);
+  return r != 0.0;
+}
+template <typename T> double Engine<T>::spec_dot(const C* a, const C* b, int kind) {
+  double r;
+  const double s = 1.5 - 1.0 / p_.q;
+  reduce<1>(R_specdot<T>{a, b, n_, px_, kind, p_.lambda1, s}, ns_, &r);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// batched 3-D transforms
+// ---------------------------------------------------------------------------
+template <typename T> struct StridedCfg {
+  static constexpr int cmin = sizeof(T) == 4 ? 16 : 8;  // 128-byte row segments
+};
+template <int N, typename T> constexpr int strided_C() {
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int c = StridedCfg<T>::cmin > 256 / Tf ? StridedCfg<T>::cmin : 256 / Tf;
+  return c;
+}
+template <int N> constexpr int xpass_NF() {
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int nf = 256 / Tf > 0 ? 256 / Tf : 1;
+  return nf < N * N / 2 ? nf : N * N / 2;
+}
+
+template <typename T> template <int N>
+void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {
+  constexpr int NF = xpass_NF<N>();
+  constexpr int CC = strided_C<N, T>();
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int PX = N / 2 + 1;
+  XArgs<T> xa{};
+  for (int b = 0; b < nb; ++b) xa.rsrc[b] = src[b];
+  xa.spec = scratch_;
+  xa.tw = tw_;
+  xa.sstride = ns_;
+  xa.nb = nb;
+  xa.scale = (T)1;
+  const long long pairs = (long long)nb * N * N / 2;
+  const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
+  prof_begin(PC_XFWD);
+  hipLaunchKernelGGL((k_xfwd<T, N, NF>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_XFWD, rb + sb);
+  SArgs<T> sa{};
+  sa.spec = scratch_;
+  sa.tw = tw_;
+  sa.sstride = ns_;
+  prof_begin(PC_YFWD);
+  hipLaunchKernelGGL((k_strided<T, N, CC, -1, 0>), dim3((PX + CC - 1) / CC, N * nb), dim3(CC * Tf),
+                     0, st_, sa, PX, (long long)PX, (long long)N * PX);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_YFWD, 2 * sb);
+  for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
+  prof_begin(PC_ZFWD);
+  hipLaunchKernelGGL((k_strided<T, N, CC, -1, 2>), dim3((N * PX + CC - 1) / CC, nb), dim3(CC * Tf),
+                     0, st_, sa, N * PX, (long long)N * PX, (long long)ns_);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_ZFWD, 2 * sb);
+}
+
+template <typename T> template <int N>
+void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst) {
+  constexpr int NF = xpass_NF<N>();
+  constexpr int CC = strided_C<N, T>();
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int PX = N / 2 + 1;
+  SArgs<T> sa{};
+  sa.spec = scratch_;
+  sa.tw = tw_;
+  sa.sstride = ns_;
+  double zin = 0;
+  for (int b = 0; b < nb; ++b) {
+    sa.nsrc[b] = src[b].ns;
+    for (int i = 0; i < src[b].ns; ++i) { sa.src[b][i] = src[b].s[i]; sa.coef[b][i] = src[b].c[i]; }
+    sa.op[b] = ops[b];
+    zin += src[b].ns;
+  }
+  const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
+  prof_begin(PC_ZINV);
+  hipLaunchKernelGGL((k_strided<T, N, CC, 1, 1>), dim3((N * PX + CC - 1) / CC, nb), dim3(CC * Tf),
+                     0, st_, sa, N * PX, (long long)N * PX, (long long)ns_);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_ZINV, sb + zin * spec_bytes());
+  prof_begin(PC_YINV);
+  hipLaunchKernelGGL((k_strided<T, N, CC, 1, 0>), dim3((PX + CC - 1) / CC, N * nb), dim3(CC * Tf),
+                     0, st_, sa, PX, (long long)PX, (long long)N * PX);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_YINV, 2 * sb);
+  XArgs<T> xa{};
+  for (int b = 0; b < nb; ++b) xa.rdst[b] = dst[b];
+  xa.spec = scratch_;
+  xa.tw = tw_;
+  xa.sstride = ns_;
+  xa.nb = nb;
+  xa.scale = (T)(1.0 / ((double)N * N * N));
+  const long long pairs = (long long)nb * N * N / 2;
+  prof_begin(PC_XINV);
+  hipLaunchKernelGGL((k_xinv<T, N, NF>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_XINV, rb + sb);
+}
+
+#define MLQ_DISPATCH_N(FN, ...)                                              \
+  switch (n_) {                                                              \
+    case 8: FN<8>(__VA_ARGS__); break;                                       \
+    case 16: FN<16>(__VA_ARGS__); break;                                     \
+    case 32: FN<32>(__VA_ARGS__); break;                                     \
+    case 64: FN<64>(__VA_ARGS__); break;                                     \
+    case 128: FN<128>(__VA_ARGS__); break;                                   \
+    case 256: FN<256>(__VA_ARGS__); break;                                   \
+    case 512: FN<512>(__VA_ARGS__); break;                                   \
+    case 1024: FN<1024>(__VA_ARGS__); break;                                 \
+    default: throw Error(MLQ_EINVAL, "unsupported n " + std::to_string(n_)); \
+  }
+
+template <typename T>
+void Engine<T>::fwd(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {
+  for (int b0 = 0; b0 < nb; b0 += MAXB) {
+    const int k = std::min(MAXB, nb - b0);
+    MLQ_DISPATCH_N(fwd_n, k, src + b0, dst + b0, ops + b0);
+  }
+}
+template <typename T>
+void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst) {
+  for (int b0 = 0; b0 < nb; b0 += MAXB) {
+    const int k = std::min(MAXB, nb - b0);
+    MLQ_DISPATCH_N(inv_n, k, src + b0, ops + b0, dst + b0);
+  }
+}
+
+template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double order) {
+  const int mc = mcut_of(order);
+  if (mask_is_identity(mc)) return;   // identity mask: the round trip only adds round-off
+  std::vector<ST> sp;
+  std::vector<C*> dp;
+  std::vector<SpecOp> ops(nb, sp_mask(mc));
+  std::vector<InvSrc> is(nb);
+  std::vector<SpecOp> none(nb, sp_none());
+  for (int b = 0; b < nb; ++b) {
+    sp.push_back(stmp());
+    dp.push_back(sp.back().get());
+    is[b] = InvSrc{{dp[b], nullptr, nullptr}, {1.0, 0, 0}, 1};
+  }
+  fwd(nb, f, dp.data(), ops.data());
+  inv(nb, is.data(), none.data(), f);
+}
+
+template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms) {
+  batch = std::max(1, std::min(batch, MAXB));
+  std::vector<RT> r;
+  std::vector<ST> s;
+  std::vector<T*> rp;
+  std::vector<C*> sp;
+  for (int b = 0; b < batch; ++b) {
+    r.push_back(rtmp()); s.push_back(stmp());
+    rp.push_back(r.back().get()); sp.push_back(s.back().get());
+    map(F_noise<T>{rp[b], 12345ULL, nr_, b}, nr_);
+  }
+  std::vector<SpecOp> ops(batch, sp_none());
+  std::vector<InvSrc> is(batch);
+  for (int b = 0; b < batch; ++b) is[b] = InvSrc{{sp[b], nullptr, nullptr}, {1.0, 0, 0}, 1};
+  const bool was = prof_;
+  prof_enable(true);
+  for (int i = 0; i < reps; ++i) {
+    fwd(batch, rp.data(), sp.data(), ops.data());
+    inv(batch, is.data(), ops.data(), rp.data());
+  }
+  prof_flush();
+  for (int c = 0; c < 6; ++c) t_ms[c] = prof_count_[c] ? prof_ms_[c] / prof_count_[c] : 0.0;
+  prof_enable(was);
+}
+
+}  // namespace mlq

@@ -1,0 +1,14 @@
+// FP32 instantiation of the engine (the mixed path: FP32 fields and
+// transforms, FP64 reductions and host scalars).
+#include "mlq_ops_core.hpp"
+#include "mlq_rcg.hpp"
+
+namespace mlq {
+
+template class Engine<float>;
+
+std::unique_ptr<IEngine> make_engine_fp32(int n, const mlq_params& p) {
+  return std::unique_ptr<IEngine>(new Engine<float>(n, p));
+}
+
+}  // namespace mlq

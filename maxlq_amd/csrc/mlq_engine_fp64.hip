@@ -1,0 +1,29 @@
+// FP64 instantiation of the engine (the parity path).
+#include "mlq_ops_core.hpp"
+#include "mlq_rcg.hpp"
+
+namespace mlq {
+
+__global__ void k_reduce_final(const double* part, int nblk, int K, double* out) {
+  __shared__ double sh[RED_THREADS];
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(long long)i * K + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
+    __syncthreads();
+  }
+}
+
+template class Engine<double>;
+
+std::unique_ptr<IEngine> make_engine_fp64(int n, const mlq_params& p) {
+  return std::unique_ptr<IEngine>(new Engine<double>(n, p));
+}
+
+}  // namespace mlq

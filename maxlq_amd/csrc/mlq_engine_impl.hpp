@@ -1,0 +1,184 @@
+// Engine<T>: device memory, batched 3-D transforms, primitives and the
+// reference field operators for one grid, in precision T (double: the FP64
+// parity path; float: the mixed path, whose reductions and scalars stay FP64).
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "mlq_engine.hpp"
+#include "mlq_pointwise.hpp"
+
+namespace mlq {
+
+template <typename T> class Engine;
+
+// RAII handle of a pooled temporary: k contiguous real fields or half spectra.
+template <typename T, typename P> class Tmp {
+ public:
+  Tmp() = default;
+  Tmp(Engine<T>* e, P* p, size_t bytes) : e_(e), p_(p), bytes_(bytes) {}
+  Tmp(Tmp&& o) noexcept : e_(o.e_), p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+  Tmp& operator=(Tmp&& o) noexcept {
+    reset();
+    e_ = o.e_; p_ = o.p_; bytes_ = o.bytes_; o.p_ = nullptr;
+    return *this;
+  }
+  Tmp(const Tmp&) = delete;
+  Tmp& operator=(const Tmp&) = delete;
+  ~Tmp() { reset(); }
+  void reset();
+  P* get() const { return p_; }
+  operator P*() const { return p_; }
+
+ private:
+  Engine<T>* e_ = nullptr;
+  P* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+template <typename T> class Engine final : public IEngine {
+ public:
+  using C = c2<T>;
+  using RT = Tmp<T, T>;
+  using ST = Tmp<T, C>;
+
+  Engine(int n, const mlq_params& p);
+  ~Engine() override;
+
+  int n() const override { return n_; }
+  const mlq_params& params() const override { return p_; }
+  void set_params(const mlq_params& p) override { p_ = p; resolve_params(); }
+  void sync() override { MLQ_HIP(hipStreamSynchronize(st_)); }
+  double real_bytes() const override { return (double)nr_ * sizeof(T); }
+  double spec_bytes() const override { return (double)ns_ * sizeof(C); }
+
+  int field_alloc(int ncomp) override;
+  void field_free(int h) override;
+  void upload(int h, const double* host) override;
+  void download(int h, double* host) override;
+  int field_ncomp(int h) const override { return fld(h).ncomp; }
+
+  void fft_c2c(const double* in, double* out, bool inverse, int ncomp) override;
+  void fft_r2c_host(const double* in, double* out) override;
+  void fft_c2r_host(const double* in, double* out) override;
+
+  void gradient(int u_sc, int out_vec) override;
+  void divergence(int u_vec, int out_sc) override;
+  void laplacian(int f_vec) override;
+  void solve_poisson(int f_sc, double l, int out_sc) override;
+  void dealias_scalar(int g_sc, double order) override;
+  void divAvg_free(int f_vec) override;
+  void HilbertGradient(int f_vec, double order) override;
+  void calc0thFourierModes(int v, double out[3]) override;
+  void calc_uk(int u_vec, double k, int out_sc) override;
+  void calc_gk_order2(int g_sc, double k, int out_sc) override;
+  double Lq_norm(int u_vec) override;
+  void rescale(int u_vec, double B) override;
+  void constraint_derivative(int u_vec, int out_vec) override;
+  void vector_transport(int u, double q, double B, int eta, int xi, int out) override;
+  double inner(int f, int g, int kind) override;
+  void dLqdt_parts(int u_vec, double q, double out[3]) override;
+  void grad_L2(int u_vec, int out_vec) override;
+  double eval_J(int f_vec, int system) override;
+  void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) override;
+  void seeded(int f_vec, unsigned long long seed, int kind, double a) override;
+  void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) override;
+  void time_fft(int batch, int reps, double* t_ms) override;
+  void prof_enable(bool on) override;
+  void prof_get(int cls, long long* count, double* ms, double* bytes) override;
+  void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }
+
+  // pool of device temporaries, keyed by size
+  RT rtmp(int k = 1);
+  ST stmp(int k = 1);
+  void pool_put(void* p, size_t bytes) { pool_[bytes].push_back(p); }
+
+ private:
+  // ---- set-up ----
+  void resolve_params();
+  void* dmalloc(size_t bytes);
+  void dfree(void* p, size_t bytes);
+  DevField& fld(int h);
+  const DevField& fld(int h) const;
+  T* fphys(int h, int c = 0) { return (T*)fld(h).phys + (long long)c * nr_; }
+
+  // ---- launch helpers ----
+  template <class F> void map(const F& f, long long n, int cls = PC_POINTWISE);
+  template <class F> void smap(const F& f) { map(f, ns_, PC_SPECTRAL); }
+  template <int K, class F> void reduce(const F& f, long long n, double* out);
+  void prof_begin(int cls);
+  void prof_end(int cls, double bytes);
+
+  // ---- transforms ----
+  // forward: src[b] (physical) -> dst[b] (spectrum) = op[b] * FFT(src[b])
+  void fwd(int nb, const T* const* src, C* const* dst, const SpecOp* ops);
+  // inverse: dst[b] = REAL(IFFT(op[b] * sum_i coef[b][i] * src[b][i]))
+  struct InvSrc { const C* s[3]; double c[3]; int ns; };
+  void inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst);
+  template <int N> void fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops);
+  template <int N> void inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst);
+
+  // convenience
+  void fwd1(const T* src, C* dst, SpecOp op = sp_none()) { fwd(1, &src, &dst, &op); }
+  void inv1(const C* src, T* dst, SpecOp op = sp_none()) {
+    InvSrc s{{src, nullptr, nullptr}, {1.0, 0.0, 0.0}, 1};
+    inv(1, &s, &op, &dst);
+  }
+  void dealias_dev(T* const* f, int nb, double order);  // in place, batch
+  int mcut_of(double order) const { return dealias_mcut(n_, order); }
+  bool mask_is_identity(int mcut) const { return mcut >= n_ / 2; }
+
+  // ---- operators on device pointers (vector = 3 components, stride nr_) ----
+  struct Chain {            // |u|^k with the calc_uk / calc_gk_order2 recursion
+    RT g, G2;
+    std::map<long long, RT> memo;
+    bool haveG2 = false;
+  };
+  void chain_init(Chain& ch, const T* u, const T* s_pre = nullptr);
+  const T* chain_pow(Chain& ch, double k);
+  void spec3(const T* u, C* const* uh);                      // 3 forward transforms
+  void grad9(C* const* uh, T* const* G);                     // G[i*3+j] = d_j u_i
+  void dLqdt_dev(const T* u, double q, double out[3]);
+  void gradL2_dev(const T* u, T* const* V);
+  double lq_norm_dev(const T* u);
+  double spec_dot(const C* a, const C* b, int kind);         // weighted Parseval sum
+  double dot(const T* a, const T* b);
+  double sum(const T* a);
+  bool has_nan(const T* a, long long n);
+
+  // ---- state ----
+  int n_, px_;
+  long long nr_, ns_;
+  mlq_params p_;
+  bool use_e_u_ = false;
+  hipStream_t st_ = nullptr;
+  C* tw_ = nullptr;
+  C* scratch_ = nullptr;
+  double* part_ = nullptr;
+  double* red_dev_ = nullptr;
+  double* red_host_ = nullptr;
+  std::vector<DevField> fields_;
+  std::map<size_t, std::vector<void*>> pool_;
+  size_t used_ = 0, peak_ = 0;
+  // profiling
+  bool prof_ = false;
+  struct ProfRec { int cls; hipEvent_t a, b; double bytes; };
+  std::vector<ProfRec> prof_recs_;
+  std::vector<hipEvent_t> ev_pool_;
+  long long prof_count_[PC_COUNT] = {};
+  double prof_ms_[PC_COUNT] = {};
+  double prof_bytes_[PC_COUNT] = {};
+  void prof_flush();
+
+  template <typename U> friend class RcgDriver;
+};
+
+template <typename T, typename P> void Tmp<T, P>::reset() {
+  if (p_ && e_) e_->pool_put(p_, bytes_);
+  p_ = nullptr;
+}
+
+}  // namespace mlq

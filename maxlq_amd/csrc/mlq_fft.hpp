@@ -1,0 +1,170 @@
+// Device FFT building blocks for gfx950 (64-wide wavefronts).
+//
+// One 1-D transform of length N (a power of two, 8..1024) is computed by Tf =
+// N/E cooperating threads, each owning E elements: element t + m*Tf lives in
+// register slot m of thread t on entry and on exit.  The transform is a
+// Stockham auto-sort sequence of radix-8/4/2 passes; the first pass works on
+// the registers loaded straight from HBM, later passes exchange through LDS,
+// and the last pass leaves natural-order results in the same register slots,
+// so global loads and stores of every pass kernel use the same
+// (t + m*Tf) pattern and stay coalesced across the batch/column dimension.
+#pragma once
+#include "mlq_common.hpp"
+
+namespace mlq {
+
+template <typename T> __device__ __forceinline__ c2<T> cmk(T x, T y) {
+  c2<T> r; r.x = x; r.y = y; return r;
+}
+#define MLQ_CPLX_OPS(V, S)                                                              \
+  __device__ __forceinline__ V cadd(V a, V b) { return cmk<S>(a.x + b.x, a.y + b.y); } \
+  __device__ __forceinline__ V csub(V a, V b) { return cmk<S>(a.x - b.x, a.y - b.y); } \
+  __device__ __forceinline__ V cmul(V a, V b) {                                         \
+    return cmk<S>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);                        \
+  }                                                                                     \
+  __device__ __forceinline__ V cscale(V a, S s) { return cmk<S>(a.x * s, a.y * s); }
+MLQ_CPLX_OPS(float2, float)
+MLQ_CPLX_OPS(double2, double)
+#undef MLQ_CPLX_OPS
+// multiply by (DIR * i): DIR = -1 forward, +1 inverse
+template <typename T, int DIR> __device__ __forceinline__ c2<T> cmul_i(c2<T> a) {
+  return DIR > 0 ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
+}
+
+// ---- compile-time radix plan ------------------------------------------------
+constexpr int radix_for(int rem, int E) {
+  return (rem % 8 == 0 && E % 8 == 0) ? 8 : ((rem % 4 == 0 && E % 4 == 0) ? 4 : 2);
+}
+constexpr int radix_at(int N, int E, int p) {
+  int rem = N;
+  for (int i = 0; i < p; ++i) rem /= radix_for(rem, E);
+  return radix_for(rem, E);
+}
+constexpr int ns_at(int N, int E, int p) {
+  int ns = 1, rem = N;
+  for (int i = 0; i < p; ++i) { int r = radix_for(rem, E); ns *= r; rem /= r; }
+  return ns;
+}
+constexpr int npasses(int N, int E) {
+  int p = 0, rem = N;
+  while (rem > 1) { rem /= radix_for(rem, E); ++p; }
+  return p;
+}
+template <int N> struct FftShape {
+  static constexpr int E = (N >= 1024) ? 16 : 8;
+  static constexpr int Tf = N / E;
+};
+
+// ---- small DFTs, natural order in and out ------------------------------------
+template <typename T, int DIR> __device__ __forceinline__ void dft2(c2<T>* x) {
+  c2<T> a = x[0], b = x[1];
+  x[0] = cadd(a, b);
+  x[1] = csub(a, b);
+}
+template <typename T, int DIR> __device__ __forceinline__ void dft4(c2<T>* x) {
+  c2<T> s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+  c2<T> s13 = cadd(x[1], x[3]), d13 = cmul_i<T, DIR>(csub(x[1], x[3]));
+  x[0] = cadd(s02, s13);
+  x[2] = csub(s02, s13);
+  x[1] = cadd(d02, d13);
+  x[3] = csub(d02, d13);
+}
+template <typename T, int DIR> __device__ __forceinline__ void dft8(c2<T>* x) {
+  c2<T> e[4] = {x[0], x[2], x[4], x[6]};
+  c2<T> o[4] = {x[1], x[3], x[5], x[7]};
+  dft4<T, DIR>(e);
+  dft4<T, DIR>(o);
+  const T h = (T)0.70710678118654752440084436210485;
+  // o[s] *= w8^s, w8 = exp(DIR i pi/4)
+  c2<T> o1 = cmk<T>(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+  c2<T> o2 = cmul_i<T, DIR>(o[2]);
+  c2<T> o3 = cmk<T>(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+  x[0] = cadd(e[0], o[0]);
+  x[4] = csub(e[0], o[0]);
+  x[1] = cadd(e[1], o1);
+  x[5] = csub(e[1], o1);
+  x[2] = cadd(e[2], o2);
+  x[6] = csub(e[2], o2);
+  x[3] = cadd(e[3], o3);
+  x[7] = csub(e[3], o3);
+}
+template <typename T, int R, int DIR> __device__ __forceinline__ void dftR(c2<T>* x) {
+  if constexpr (R == 8) dft8<T, DIR>(x);
+  else if constexpr (R == 4) dft4<T, DIR>(x);
+  else dft2<T, DIR>(x);
+}
+
+// ---- LDS views -------------------------------------------------------------------
+// Row view: one transform per contiguous LDS row, one pad slot every 8
+// elements so that the stride-R writes of the first passes hit distinct banks.
+template <typename T> struct LdsRow {
+  c2<T>* base;
+  static __host__ __device__ constexpr int pitch(int N) { return N + N / 8; }
+  __device__ __forceinline__ c2<T>& at(int n) const { return base[n + (n >> 3)]; }
+};
+// Column view: C transforms interleaved (element n of column c at n*C + c), so
+// consecutive threads (consecutive columns) touch consecutive LDS words.
+template <typename T, int C> struct LdsCol {
+  c2<T>* base;
+  int c;
+  __device__ __forceinline__ c2<T>& at(int n) const { return base[n * C + c]; }
+};
+
+// ---- one Stockham pass -----------------------------------------------------------
+template <typename T, int N, int E, int DIR, int P, class L>
+__device__ __forceinline__ void fft_pass(c2<T> (&v)[E], int t, const c2<T>* __restrict__ tw,
+                                         const L& lds) {
+  constexpr int Tf = N / E;
+  constexpr int R = radix_at(N, E, P);
+  constexpr int Ns = ns_at(N, E, P);
+  constexpr int NB = E / R;  // butterflies per thread
+  constexpr bool last = (P == npasses(N, E) - 1);
+  if constexpr (P > 0) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * Tf;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = lds.at(j + r * (N / R));
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * Tf;
+    c2<T> x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+    if constexpr (Ns > 1) {
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        c2<T> w = tw[r * k * (N / (Ns * R))];
+        if (DIR > 0) w.y = -w.y;
+        x[r] = cmul(x[r], w);
+      }
+    }
+    dftR<T, R, DIR>(x);
+    if constexpr (!last) {
+      const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.at(idxD + r * Ns) = x[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = x[r];
+    }
+  }
+  if constexpr (!last) {
+    __syncthreads();
+    fft_pass<T, N, E, DIR, P + 1, L>(v, t, tw, lds);
+  }
+}
+
+// Full length-N transform; DIR = -1 forward (exp(-i...)), +1 inverse
+// (unnormalised).  tw[k] = exp(-2 pi i k / N).
+template <typename T, int N, int E, int DIR, class L>
+__device__ __forceinline__ void fft_core(c2<T> (&v)[E], int t, const c2<T>* __restrict__ tw,
+                                         const L& lds) {
+  fft_pass<T, N, E, DIR, 0, L>(v, t, tw, lds);
+}
+
+}  // namespace mlq

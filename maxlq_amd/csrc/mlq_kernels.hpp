@@ -1,0 +1,258 @@
+// Pass kernels of the batched 3-D real transforms and the elementwise /
+// spectral / reduction kernels of the maxLq hot path.
+//
+// Data layout in HBM (per field):
+//   physical  T[n3][n2][n1]            (x fastest: the reference's Fortran order)
+//   spectrum  c2<T>[n3][n2][px], px = n1/2+1  (kx fastest; kx in [0, n1/2])
+// A 3-D forward transform is x (R2C, contiguous rows) -> y -> z passes; the
+// inverse runs z -> y -> x.  Spectral multipliers are fused into the z passes
+// (epilogue of the forward, prologue of the inverse), the 1/N^3 of fftbwd
+// (fftwfunction.f90:109) into the inverse x pass.
+#pragma once
+#include "mlq_fft.hpp"
+
+namespace mlq {
+
+constexpr int MAXB = 12;
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+template <typename T> struct XArgs {
+  const T* rsrc[MAXB];
+  T* rdst[MAXB];
+  c2<T>* spec;        // batch-contiguous spectra, stride sstride
+  const c2<T>* tw;
+  long long sstride;
+  int nb;
+  T scale;
+};
+
+template <typename T> struct SArgs {
+  c2<T>* spec;        // scratch spectra, batch-contiguous
+  const c2<T>* src[MAXB][3];
+  double coef[MAXB][3];
+  c2<T>* dst[MAXB];
+  SpecOp op[MAXB];
+  int nsrc[MAXB];
+  const c2<T>* tw;
+  long long sstride;
+};
+
+// Spectral factor of one half-spectrum mode (see SpecOp in mlq_common.hpp).
+__device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, int kz, int n) {
+  const int h = n >> 1;
+  const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+  double2 f = make_double2(o.scale, 0.0);
+  const int fl = o.flags;
+  if (fl == 0) return f;
+  if (fl & SP_MASK) {
+    int mm = max(abs(mx), max(abs(my), abs(mz)));
+    if (mm > o.mcut) return make_double2(0.0, 0.0);
+  }
+  if ((fl & SP_ZEROK0) && mx == 0 && my == 0 && mz == 0) return make_double2(0.0, 0.0);
+  const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
+  const double ksq = K1 * K1 + K2 * K2 + K3 * K3;
+  if (fl & SP_NEGKSQ) f.x *= -ksq;
+  if (fl & SP_INVKSQ) {
+    if (ksq == 0.0) return make_double2(0.0, 0.0);
+    f.x *= -1.0 / ksq;
+  }
+  if (fl & SP_HILBERT) f.x /= (1.0 + pow(o.hil_a * sqrt(ksq), o.hil_e));
+  if (fl & SP_IK) {
+    const int md = o.dim == 0 ? mx : (o.dim == 1 ? my : mz);
+    if (md == h) return make_double2(0.0, 0.0);  // own-dimension Nyquist
+    const double Kd = TWO_PI * md;
+    f = make_double2(0.0, f.x * Kd);
+  }
+  return f;
+}
+
+template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, double2 f) {
+  const T fx = (T)f.x, fy = (T)f.y;
+  return cmk<T>(v.x * fx - v.y * fy, v.x * fy + v.y * fx);
+}
+
+// ---------------------------------------------------------------------------
+// x pass, forward: two real rows packed as one complex row (a + i b), one
+// N-point FFT, split into the two half spectra.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int NF>
+__global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+  constexpr int LP = LdsRow<T>::pitch(N);
+  __shared__ c2<T> sm[NF * LP];
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const long long pairs_per = (long long)N * N / 2;
+  const long long gp = (long long)blockIdx.x * NF + f;
+  const int b = (int)(gp / pairs_per);
+  const long long lp = gp - (long long)b * pairs_per;
+  const long long row0 = 2 * lp;
+  const T* s = a.rsrc[b] + row0 * N;
+  c2<T> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmk<T>(s[t + m * Tf], s[N + t + m * Tf]);
+  LdsRow<T> L{sm + f * LP};
+  fft_core<T, N, E, -1>(v, t, a.tw, L);
+#pragma unroll
+  for (int m = 0; m < E; ++m) L.at(t + m * Tf) = v[m];
+  __syncthreads();
+  c2<T>* d = a.spec + b * a.sstride + row0 * PX;
+  const T hf = (T)0.5;
+#pragma unroll
+  for (int m = 0; m < E / 2; ++m) {
+    const int k = t + m * Tf;
+    const c2<T> zk = L.at(k), zn = L.at((N - k) & (N - 1));
+    d[k] = cmk<T>(hf * (zk.x + zn.x), hf * (zk.y - zn.y));
+    d[PX + k] = cmk<T>(hf * (zk.y + zn.y), -hf * (zk.x - zn.x));
+  }
+  if (t == 0) {
+    const c2<T> z = L.at(N / 2);
+    d[N / 2] = cmk<T>(z.x, (T)0);
+    d[PX + N / 2] = cmk<T>(z.y, (T)0);
+  }
+}
+
+// x pass, inverse (C2R): rebuild the packed complex row from the two half
+// spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
+// inverse FFT, unpack, scale.
+template <typename T, int N, int NF>
+__global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+  constexpr int LP = LdsRow<T>::pitch(N);
+  __shared__ c2<T> sm[NF * LP];
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const long long pairs_per = (long long)N * N / 2;
+  const long long gp = (long long)blockIdx.x * NF + f;
+  const int b = (int)(gp / pairs_per);
+  const long long lp = gp - (long long)b * pairs_per;
+  const long long row0 = 2 * lp;
+  const c2<T>* sp = a.spec + b * a.sstride + row0 * PX;
+  c2<T> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + m * Tf;
+    const int kk = k <= N / 2 ? k : N - k;
+    c2<T> A = sp[kk], B = sp[PX + kk];
+    if (k == 0 || k == N / 2) { A.y = (T)0; B.y = (T)0; }
+    if (k > N / 2) { A.y = -A.y; B.y = -B.y; }
+    v[m] = cmk<T>(A.x - B.y, A.y + B.x);
+  }
+  LdsRow<T> L{sm + f * LP};
+  fft_core<T, N, E, +1>(v, t, a.tw, L);
+  T* o = a.rdst[b] + row0 * N;
+  const T sc = a.scale;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    o[t + m * Tf] = v[m].x * sc;
+    o[N + t + m * Tf] = v[m].y * sc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Strided pass along y or z: C adjacent columns per workgroup, each column a
+// length-N transform.  MODE 0: plain in-place pass on the scratch spectra
+// (y pass).  MODE 1: inverse z pass reading sum_i coef_i * src_i times the
+// spectral factor (prologue), writing scratch.  MODE 2: forward z pass
+// reading scratch, writing factor * result to dst (epilogue).
+// ---------------------------------------------------------------------------
+template <typename T, int N, int C, int DIR, int MODE>
+__global__ __launch_bounds__(C* FftShape<N>::Tf) void k_strided(SArgs<T> a, int ncols,
+                                                                 long long estride,
+                                                                 long long pstride) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+  __shared__ c2<T> sm[N * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const bool valid = col < ncols;
+  const int plane = blockIdx.y;
+  c2<T> v[E];
+  int kx = 0, ky = 0;
+  if (MODE != 0) { ky = col / PX; kx = col - ky * PX; }
+  if (MODE == 1) {
+    const int bb = plane;
+    const SpecOp op = a.op[bb];
+    const int ns = a.nsrc[bb];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int kz = t + m * Tf;
+      c2<T> x = cmk<T>((T)0, (T)0);
+      if (valid) {
+        const long long off = (long long)kz * estride + col;
+        for (int i = 0; i < ns; ++i) {
+          const c2<T> s = a.src[bb][i][off];
+          const T cf = (T)a.coef[bb][i];
+          x.x += cf * s.x;
+          x.y += cf * s.y;
+        }
+        x = apply_factor<T>(x, spec_factor(op, kx, ky, kz, N));
+      }
+      v[m] = x;
+    }
+  } else {
+    const c2<T>* s = a.spec + plane * pstride + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = valid ? s[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+  }
+  LdsCol<T, C> L{sm, c};
+  fft_core<T, N, E, DIR>(v, t, a.tw, L);
+  if (!valid) return;
+  if (MODE == 2) {
+    const int bb = plane;
+    const SpecOp op = a.op[bb];
+    c2<T>* d = a.dst[bb] + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int kz = t + m * Tf;
+      d[(long long)kz * estride] = apply_factor<T>(v[m], spec_factor(op, kx, ky, kz, N));
+    }
+  } else {
+    c2<T>* d = a.spec + plane * pstride + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions: per-block partial sums (wavefront shuffles + LDS), then a
+// single-block final sum; fixed block count, so results are deterministic.
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAXK = 4;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <int K> __device__ __forceinline__ void block_reduce_store(double (&acc)[K], double* out) {
+  __shared__ double sh[RED_THREADS / 64][K];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = wave_sum(acc[k]);
+    if (lane == 0) sh[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int i = 0; i < RED_THREADS / 64; ++i) s += sh[i][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+template <int K, class F>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce(F f, long long n, double* part) {
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * RED_THREADS)
+    f(i, acc);
+  block_reduce_store<K>(acc, part + (long long)blockIdx.x * K);
+}
+
+__global__ void k_reduce_final(const double* part, int nblk, int K, double* out);
+
+}  // namespace mlq

@@ -1,0 +1,171 @@
+// The RCG solve for one constraint value B: maxdLqdt, optimization.f90:7-514.
+// Algorithmic body only (the per-iteration prints, spectra, field saves and
+// kappa tests of the reference are diagnostics); every record the reference
+// writes from inside the loop is delivered to the log sink.
+#pragma once
+#include <chrono>
+#include <vector>
+
+#include "mlq_linesearch.hpp"
+#include "mlq_ops_core.hpp"
+
+namespace mlq {
+
+template <typename T>
+void Engine<T>::rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) {
+  auto t0 = std::chrono::steady_clock::now();
+  const double q = p_.q, B = p_.B;
+  const long long n3 = nr_ * 3;
+  // state fields (optimization.f90:15, 28-50); gradJ0, diff_gradJ and uvec0 are
+  // never read by the algorithm and are not kept
+  const int g1 = field_alloc(3), nH = field_alloc(3), gproj = field_alloc(3), gu1 = field_alloc(3),
+            gu0 = field_alloc(3), d1 = field_alloc(3), vg0 = field_alloc(3), vd0 = field_alloc(3),
+            tmp = field_alloc(3), eta = field_alloc(3), fs = field_alloc(1);
+  struct Cleanup {
+    Engine* e; std::vector<int> h;
+    ~Cleanup() { for (int x : h) try { e->field_free(x); } catch (...) {} }
+  } cleanup{this, {g1, nH, gproj, gu1, gu0, d1, vg0, vd0, tmp, eta, fs}};
+  auto axpby = [&](int x, double a, int y, double b, int o) {
+    map(F_axpby<T>{fphys(x), fphys(y), fphys(o), (T)a, (T)b}, n3);
+  };
+  auto msg = [&](int iter, int code) { log.emit(MLQ_LOG_MSG, iter, code, 0, nullptr); };
+  const int HL = MLQ_IP_HL;
+
+  rescale(u, B);                                                    // :61
+  divAvg_free(u);                                                   // :62
+  double J0 = eval_J(u, MLQ_J_MAXDLQDT);                            // :78
+  double J1 = J0;
+  int iter = cfg.start_iter;
+  const int start = cfg.start_iter;
+  double tau0 = 100.0, tau1 = tau0;                                 // :92-94
+  const int nroll = std::max(1, cfg.rolling);
+  std::vector<double> rel(nroll, 0.0);
+  double avg = 1.0;
+  int while_flag = 0, stop = 0;
+  long long nJ = 0, nJc = 0;
+  {
+    double d[3] = {0.0, 0.0, J0};
+    log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                       // :117
+  }
+  const double s = 1.5 - 1.0 / q;
+  while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
+    iter += 1;
+    grad_L2(u, g1);                                                 // :146
+    HilbertGradient(g1, s);                                         // :151
+    divAvg_free(g1);                                                // :153
+    constraint_derivative(u, nH);                                   // :162
+    HilbertGradient(nH, s);                                         // :164
+    divAvg_free(nH);                                                // :169
+    const double alpha = inner(nH, g1, HL) / inner(nH, nH, HL);     // :170-171
+    axpby(g1, 1.0, nH, -alpha, gproj);                              // :177
+    axpby(gproj, 1.0, nH, 1.0, tmp);
+    const double t1 = std::fabs(inner(gproj, nH, HL) / inner(tmp, tmp, HL));   // :180
+    if (t1 > 9.9999999600419720e-13) msg(iter, 42);
+    divergence(gproj, fs);                                          // :188
+    const double t2 = inner(fs, fs, MLQ_IP_L2) / inner(gproj, gproj, HL);
+    if (t2 > 1.0000000036274937e-15) msg(iter, 43);
+    // normalised tangential gradient (:201-214)
+    const double nrm = std::sqrt(inner(u, u, HL) / inner(gproj, gproj, HL));
+    axpby(gproj, nrm, gproj, 0.0, gu1);
+    // Polak-Ribiere with transported vectors (:226-240)
+    axpby(gu1, 1.0, vg0, -1.0, tmp);
+    double beta = inner(gu1, tmp, HL);
+    const double g0n = inner(gu0, gu0, HL);
+    if (g0n < MACH_EPSILON || beta < MACH_EPSILON) beta = 0.0;
+    else beta = beta / g0n;
+    if (cfg.reset_every > 0 && iter % cfg.reset_every == 0) beta = 0.0;
+    axpby(gu1, 1.0, vd0, beta, d1);
+    double m0[3];
+    calc0thFourierModes(d1, m0);                                    // :251-261
+    if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
+    calc0thFourierModes(u, m0);
+    if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
+    // line search on J(tau) = eval_J(u + tau d1, "LineMin")
+    JFun J = [&](double tau) {
+      double v;
+      eval_J_batch(u, d1, &tau, 1, &v);
+      ++nJ; ++nJc;
+      if (std::isnan(v)) throw Error(MLQ_ENAN, "nan in J");
+      return v;
+    };
+    LineminLog llog = [&](int mode, int bi, double tA, double tB, double tC, double FA, double FB,
+                          double FC) {
+      double d[6] = {tA, tB, tC, FA, FB, FC};
+      log.emit(MLQ_LOG_LINEMIN, iter, bi, mode, d);
+    };
+    BrentLog blog = [&](int j, double X, double FX) {
+      double d[2] = {X, FX};
+      log.emit(MLQ_LOG_BRENT, iter, j, 0, d);
+    };
+    msg(iter, 20);
+    BracketResult br = mnbrak(J, 0.0, tau0, llog);                  // :269
+    if (br.flag != 0) {
+      if (br.flag == 1) msg(iter, 22);
+      log.emit(MLQ_LOG_ERR, iter, br.flag, 0, nullptr);
+      double d[3] = {tau1, beta, J0};
+      log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :285
+      while_flag = 1;
+      stop = br.flag == 1 ? 2 : 3;
+      break;
+    }
+    msg(iter, 21);
+    msg(iter, 30);
+    BrentResult bres = brent(J, br.tA, br.tC, blog);                // :297
+    tau1 = bres.X;
+    if (tau1 > br.tC) {
+      if (tau1 > 2.0 * br.tC) msg(iter, 32);
+    } else {
+      msg(iter, 31);
+    }
+    if (tau1 > TAU_MAX) {                                           // :310-318
+      msg(iter, 32);
+      while_flag = 1;
+      tau1 = 0.0;
+      stop = 4;
+    } else if (tau1 > std::sqrt(TAU_MAX)) {
+      msg(iter, 41);
+    }
+    // vector transport at the old u with eta = tau1 d1 (:326-327)
+    axpby(d1, tau1, d1, 0.0, eta);
+    vector_transport(u, q, B, eta, gu1, vg0);
+    vector_transport(u, q, B, eta, d1, vd0);
+    // update (:344-350)
+    axpby(u, 1.0, d1, tau1, u);
+    rescale(u, B);
+    J1 = eval_J(u, MLQ_J_MAXDLQDT);
+    const double deltaJ = (J1 - J0) / std::fabs(J0);
+    if (deltaJ < -MACH_EPSILON) {                                   // :351-365
+      double d[3] = {tau1, beta, J0};
+      log.emit(MLQ_LOG_ITER, iter, iter + 1, 0, d);
+      msg(iter, 0);
+      while_flag = 1;
+      stop = 5;
+      break;
+    }
+    {
+      double d[3] = {tau1, beta, J1};
+      log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :375
+    }
+    J0 = J1;                                                        // :432-436
+    axpby(gu1, 1.0, gu1, 0.0, gu0);
+    tau0 = tau1;
+    for (int i = nroll - 1; i > 0; --i) rel[i] = rel[i - 1];        // :439-445
+    rel[0] = deltaJ;
+    double sr = 0.0;
+    for (double x : rel) sr += x;
+    avg = sr / (double)(iter - start);
+  }
+  const bool ok = (iter < cfg.max_iter) && while_flag < 1;          // :451
+  if (stop == 0 && !ok) stop = 1;
+  msg(iter, 1);
+  res.J1 = J1;
+  res.iters = iter;
+  res.successful = ok ? 1 : 0;
+  res.while_flag = while_flag;
+  res.stop_reason = stop;
+  res.n_J = (int)nJ;
+  res.n_J_computed = (int)nJc;
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace mlq

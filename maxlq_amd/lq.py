@@ -1,0 +1,312 @@
+"""Host-side mirror of the reference's Fortran interface for the RCG hot path.
+
+``MaxLq`` owns one device context (one GPU, one HIP stream).  Its methods
+carry the reference routine names and argument meaning (fftwfunction.f90,
+function_ops.f90, optimization.f90); arrays use the reference layout: a
+scalar field is (n3, n2, n1) C-ordered float64 (= Fortran (n1,n2,n3), x
+fastest), a vector field (3, n3, n2, n1).  Routines that are ``intent(inout)``
+in the reference update the numpy array in place.  Every call runs on the GPU
+through libmaxlq.so; errors raise ``MlqError`` (NaN -> MLQ_ENAN where the
+reference stops with ``stop 1``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .capi import MlqError, _p
+
+_IP = {"L2": capi.MLQ_IP_L2, "H_l^(3/2-1/q)": capi.MLQ_IP_HL, "H^(3/2-1/q)": capi.MLQ_IP_H,
+       "H^((3q-1)/(2q))": capi.MLQ_IP_OLD_H, "H_l^((3q-1)/(2q))": capi.MLQ_IP_OLD_HL}
+_SYS = {"maxdLqdt": capi.MLQ_J_MAXDLQDT, "LineMin": capi.MLQ_J_LINEMIN,
+        "||u||_q": capi.MLQ_J_LQNORM}
+
+
+def _f64(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+@dataclass
+class RcgTrace:
+    """Records the reference writes from inside maxdLqdt (data_ops.f90)."""
+    iteration_info: list = field(default_factory=list)   # (iter, tau, beta, J)
+    linemin: list = field(default_factory=list)          # (iter, brakIter, mode, tA..FC)
+    brent: list = field(default_factory=list)            # (iter, j, X, FX)
+    msgs: list = field(default_factory=list)             # (iter, code)
+    errors: list = field(default_factory=list)           # (iter, code)
+
+
+class MaxLq:
+    """One device context for an n^3 grid (replaces init_fft / initialize)."""
+
+    def __init__(self, n: int, q: float = 4.0, B: float = 1.0, precision: str = "fp64", **kw):
+        self.lib = capi.load()
+        self.n = int(n)
+        prec = capi.MLQ_MIXED if precision in ("mixed", "fp32") else capi.MLQ_FP64
+        self.params = capi.default_params(q, B=float(B), precision=prec, **kw)
+        self.ctx = C.c_void_p()
+        rc = self.lib.mlq_init(C.byref(self.ctx), self.n, self.n, self.n, C.byref(self.params))
+        if rc != capi.MLQ_OK:
+            raise MlqError(rc, "mlq_init failed")
+        self.shape = (self.n, self.n, self.n)
+        self.vshape = (3,) + self.shape
+        self.hshape = (self.n, self.n, self.n // 2 + 1)
+
+    # -- plumbing ---------------------------------------------------------------
+    def _chk(self, rc):
+        if rc != capi.MLQ_OK:
+            raise MlqError(rc, self.lib.mlq_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.mlq_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._chk(self.lib.mlq_set_params(self.ctx, C.byref(self.params)))
+
+    @property
+    def q(self):
+        return self.params.q
+
+    # -- fftwfunction ---------------------------------------------------------------
+    def fftfwd(self, u):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        out = np.empty(self.shape, np.complex128)
+        self._chk(self.lib.mlq_fftfwd(self.ctx, _p(u.view(np.float64)), _p(out.view(np.float64))))
+        return out
+
+    def fftbwd(self, fu):
+        fu = np.ascontiguousarray(fu, dtype=np.complex128)
+        out = np.empty(self.shape, np.complex128)
+        self._chk(self.lib.mlq_fftbwd(self.ctx, _p(fu.view(np.float64)), _p(out.view(np.float64))))
+        return out
+
+    def fftfwdr(self, u):
+        u = _f64(u, self.shape)
+        out = np.empty(self.hshape, np.complex128)
+        self._chk(self.lib.mlq_fftfwdr(self.ctx, _p(u), _p(out.view(np.float64))))
+        return out
+
+    def fftbwdr(self, fu):
+        fu = np.ascontiguousarray(fu, dtype=np.complex128)
+        out = np.empty(self.shape, np.float64)
+        self._chk(self.lib.mlq_fftbwdr(self.ctx, _p(fu.view(np.float64)), _p(out)))
+        return out
+
+    # -- function_ops -----------------------------------------------------------------
+    def gradient(self, u):
+        u = _f64(u, self.shape)
+        out = np.empty(self.vshape)
+        self._chk(self.lib.mlq_gradient(self.ctx, _p(u), _p(out)))
+        return out
+
+    def divergence(self, u):
+        u = _f64(u, self.vshape)
+        out = np.empty(self.shape)
+        self._chk(self.lib.mlq_divergence(self.ctx, _p(u), _p(out)))
+        return out
+
+    def laplacian(self, f):
+        a = _f64(f, self.vshape).copy()
+        self._chk(self.lib.mlq_laplacian(self.ctx, _p(a)))
+        return a
+
+    def solve_poisson(self, f, l=1.0):
+        f = _f64(f, self.shape)
+        out = np.empty(self.shape)
+        self._chk(self.lib.mlq_solve_poisson(self.ctx, _p(f), float(l), _p(out)))
+        return out
+
+    def dealias_scalar(self, g, nonlinOrder=2.0):
+        a = _f64(g, self.shape).copy()
+        self._chk(self.lib.mlq_dealias_scalar(self.ctx, _p(a), float(nonlinOrder)))
+        return a
+
+    def divAvg_free(self, f):
+        a = _f64(f, self.vshape).copy()
+        self._chk(self.lib.mlq_divAvg_free(self.ctx, _p(a)))
+        return a
+
+    def HilbertGradient(self, grad, order):
+        a = _f64(grad, self.vshape).copy()
+        self._chk(self.lib.mlq_HilbertGradient(self.ctx, _p(a), float(order)))
+        return a
+
+    def calc0thFourierModes(self, v):
+        v = _f64(v, self.vshape)
+        out = np.empty(3)
+        self._chk(self.lib.mlq_calc0thFourierModes(self.ctx, _p(v), _p(out)))
+        return out
+
+    def calc_uk(self, u, k):
+        u = _f64(u, self.vshape)
+        out = np.empty(self.shape)
+        self._chk(self.lib.mlq_calc_uk(self.ctx, _p(u), float(k), _p(out)))
+        return out
+
+    def calc_gk_order2(self, g, k):
+        g = _f64(g, self.shape)
+        out = np.empty(self.shape)
+        self._chk(self.lib.mlq_calc_gk_order2(self.ctx, _p(g), float(k), _p(out)))
+        return out
+
+    def calc_global_Lq_norm(self, u):
+        u = _f64(u, self.vshape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_calc_global_Lq_norm(self.ctx, _p(u), C.byref(r)))
+        return r.value
+
+    def rescaleLqNorm(self, u, B):
+        a = _f64(u, self.vshape).copy()
+        self._chk(self.lib.mlq_rescaleLqNorm(self.ctx, _p(a), float(B)))
+        return a
+
+    def calcConstraintDerivativeL2(self, u):
+        u = _f64(u, self.vshape)
+        out = np.empty(self.vshape)
+        self._chk(self.lib.mlq_calcConstraintDerivativeL2(self.ctx, _p(u), _p(out)))
+        return out
+
+    def vectorTransport(self, u, q, B, eta, xi):
+        u, eta, xi = (_f64(a, self.vshape) for a in (u, eta, xi))
+        out = np.empty(self.vshape)
+        self._chk(self.lib.mlq_vectorTransport(self.ctx, _p(u), float(q), float(B), _p(eta),
+                                               _p(xi), _p(out)))
+        return out
+
+    def inner_product(self, f, g, mytype="L2"):
+        f, g = _f64(f, self.shape), _f64(g, self.shape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_inner_product(self.ctx, _p(f), _p(g), _IP[mytype], C.byref(r)))
+        return r.value
+
+    global_inner_product = inner_product
+
+    def global_summed_field_inner_product(self, f, g, mytype="L2"):
+        f, g = _f64(f, self.vshape), _f64(g, self.vshape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_global_summed_field_inner_product(self.ctx, _p(f), _p(g),
+                                                                 _IP[mytype], C.byref(r)))
+        return r.value
+
+    def calc_local_dLqdt_inclParts(self, u, q=None):
+        u = _f64(u, self.vshape)
+        out = np.empty(3)
+        qq = self.q if q is None else float(q)
+        self._chk(self.lib.mlq_calc_local_dLqdt_inclParts(self.ctx, _p(u), qq, _p(out)))
+        return out
+
+    calc_global_dLqdt_inclParts = calc_local_dLqdt_inclParts
+
+    def calc_local_dLqdt(self, u, q=None):
+        return float(self.calc_local_dLqdt_inclParts(u, q)[0])
+
+    def GradL2ForLq(self, U):
+        U = _f64(U, self.vshape)
+        out = np.empty(self.vshape)
+        self._chk(self.lib.mlq_GradL2ForLq(self.ctx, _p(U), _p(out)))
+        return out
+
+    def eval_J(self, myfield, mysystem):
+        a = _f64(myfield, self.vshape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_eval_J(self.ctx, _p(a), _SYS[mysystem], C.byref(r)))
+        return r.value
+
+    # -- device-resident fields ----------------------------------------------------------
+    def field(self, ncomp=3):
+        h = C.c_int()
+        self._chk(self.lib.mlq_field_alloc(self.ctx, ncomp, C.byref(h)))
+        return h.value
+
+    def free(self, h):
+        self._chk(self.lib.mlq_field_free(self.ctx, h))
+
+    def upload(self, h, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.lib.mlq_upload(self.ctx, h, _p(a)))
+
+    def download(self, h, ncomp=3):
+        out = np.empty(self.vshape if ncomp == 3 else self.shape)
+        self._chk(self.lib.mlq_download(self.ctx, h, _p(out)))
+        return out
+
+    def seeded_field(self, h, seed, kind="gauss", a=16.0):
+        k = {"gauss": 0, "k2gauss": 1}[kind]
+        self._chk(self.lib.mlq_seeded_field(self.ctx, h, C.c_ulonglong(seed), k, float(a)))
+
+    def eval_J_batch(self, hu, hd, taus):
+        taus = np.ascontiguousarray(taus, dtype=np.float64)
+        out = np.empty(len(taus))
+        self._chk(self.lib.mlq_eval_J_batch(self.ctx, hu, hd, _p(taus), len(taus), _p(out)))
+        return out
+
+    # -- optimization: maxdLqdt ------------------------------------------------------------
+    def maxdLqdt(self, hu, max_iter=99999, optim_tol=None, start_iter=0, batch_linesearch=0,
+                 reset_every=25, rolling=10, trace=True):
+        """One RCG solve for the current B on the device-resident field ``hu``
+        (optimization.f90:7-514).  Returns (RcgResult, RcgTrace)."""
+        if optim_tol is None:
+            optim_tol = 1.0e-4 if self.n >= 1024 else 1.0e-5       # initialize.f90:95-100
+        cfg = capi.RcgCfg(int(max_iter), float(optim_tol), int(start_iter), int(batch_linesearch),
+                          int(reset_every), int(rolling), 0)
+        res = capi.RcgResult()
+        tr = RcgTrace()
+
+        def cb(user, kind, it, i0, i1, d):
+            if kind == capi.MLQ_LOG_ITER:
+                tr.iteration_info.append((i0, d[0], d[1], d[2]))
+            elif kind == capi.MLQ_LOG_LINEMIN:
+                tr.linemin.append((it, i0, i1) + tuple(d[k] for k in range(6)))
+            elif kind == capi.MLQ_LOG_BRENT:
+                tr.brent.append((it, i0, d[0], d[1]))
+            elif kind == capi.MLQ_LOG_MSG:
+                tr.msgs.append((it, i0))
+            elif kind == capi.MLQ_LOG_ERR:
+                tr.errors.append((it, i0))
+
+        cbf = capi.LOG_CB(cb) if trace else C.cast(None, capi.LOG_CB)
+        self._chk(self.lib.mlq_rcg_solve(self.ctx, hu, C.byref(cfg), C.byref(res), cbf, None))
+        return res, tr
+
+    # -- measurement ------------------------------------------------------------------------
+    def time_fft(self, batch=3, reps=5):
+        out = np.zeros(6)
+        self._chk(self.lib.mlq_time_fft(self.ctx, batch, reps, _p(out)))
+        return out
+
+    def sizes(self):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.mlq_sizes(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def prof_enable(self, on=True):
+        self._chk(self.lib.mlq_prof_enable(self.ctx, 1 if on else 0))
+
+    def prof(self):
+        out = {}
+        for cls in range(10):
+            cnt, ms, by = C.c_longlong(), C.c_double(), C.c_double()
+            self._chk(self.lib.mlq_prof_get(self.ctx, cls, C.byref(cnt), C.byref(ms), C.byref(by)))
+            out[self.lib.mlq_prof_name(cls).decode()] = (cnt.value, ms.value, by.value)
+        return out
+
+    def mem_info(self):
+        v = [C.c_double() for _ in range(4)]
+        self._chk(self.lib.mlq_mem_info(self.ctx, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
