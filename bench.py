@@ -1,0 +1,196 @@
+"""Benchmark: RCG iterations/s at 512^3, q = 4 (mixed precision with batched
+line search vs the plain FP64 path) and FFT pass bandwidth, on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--q 4]
+
+A step is one RCG iteration of maxdLqdt (GradL2ForLq, Hilbert gradient,
+Leray projection, tangent projection, CG direction, mnbrak + brent line
+search, vector transport, retraction) on the seeded divergence-free field
+with spectrum exp(-|m|^2/256), B = 2 (synthetic data; the reference's input
+fields are not available).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0):
+    """Run warmup+steps RCG iterations; time the last `steps` from the
+    library's per-iteration records (each iteration ends in a synchronising
+    reduction, so host stamps at the record are iteration boundaries)."""
+    from maxlq_amd.lq import MaxLq
+    import ctypes as C
+    from maxlq_amd import capi
+
+    g = MaxLq(n, q=q, B=B, precision=precision)
+    h = g.field(3)
+    g.seeded_field(h, seed, "gauss", a)
+    g.lib.mlq_sync(g.ctx)
+    stamps = {}
+    nJ = {}
+    cur = {"it": 0, "nJ": 0}
+
+    def cb(user, kind, it, i0, i1, d):
+        if kind == capi.MLQ_LOG_ITER:
+            stamps[i0] = time.perf_counter()
+            log(f"{precision} iter {i0} J={d[2]:.10e} tau={d[0]:.4e} beta={d[1]:.4e}")
+        elif kind == capi.MLQ_LOG_BRENT and i0 == 999:
+            pass
+
+    cfg = capi.RcgCfg(warmup + steps, 0.0, 0, int(batch), 25, 10, 0)
+    res = capi.RcgResult()
+    t0 = time.perf_counter()
+    rc = g.lib.mlq_rcg_solve(g.ctx, h, C.byref(cfg), C.byref(res), capi.LOG_CB(cb), None)
+    g._chk(rc)
+    t1 = time.perf_counter()
+    if res.iters < warmup + steps:
+        raise RuntimeError(f"{precision}: solve stopped after {res.iters} iterations "
+                           f"(reason {res.stop_reason})")
+    t_it = (stamps[warmup + steps] - stamps[warmup]) / steps
+    return g, h, res, t_it, t1 - t0
+
+
+def fft_roofline(g, batch=12, reps=5):
+    """Per-pass average launch durations (HIP events on the library stream)
+    and algorithmic bytes (input + output buffer bytes per launch)."""
+    t = g.time_fft(batch, reps)
+    rb, sb = g.sizes()
+    passes = ["x_fwd", "y_fwd", "z_fwd", "z_inv", "y_inv", "x_inv"]
+    byts = [batch * (rb + sb), 2 * batch * sb, 2 * batch * sb,
+            2 * batch * sb, 2 * batch * sb, batch * (rb + sb)]
+    out = {}
+    for p, ms, by in zip(passes, t, byts):
+        out[p] = {"ms": ms, "bytes": by, "GBs": by / (ms * 1e-3) / 1e9}
+    tot_b = sum(byts)
+    tot_t = sum(t) * 1e-3
+    return out, tot_b / tot_t / 1e9, (batch * rb + 2 * batch * sb) / (0.5 * tot_t) / 1e9
+
+
+def cpu_baseline(q, B, threads):
+    """Bounded sample of the reference-structured FP64 CPU path (the oracle):
+    one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
+    iteration with the reference's transform counts (SURVEY: 548 C2C
+    transforms per iteration outside the line search, 94 per J evaluation)."""
+    from oracle import lq_oracle as L
+    from oracle.seeded import seeded_field
+    L.set_threads(threads)
+    n = 256
+    u = seeded_field(n, 4, "gauss", 256.0 / 4.0, q=q, B=B)
+    orc = L.Oracle(L.Params(n=n, q=q, B=B))
+    t0 = time.perf_counter()
+    orc.eval_J(u, "LineMin")
+    tJ = time.perf_counter() - t0
+    return tJ
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--q", type=float, default=4.0)
+    ap.add_argument("--B", type=float, default=2.0)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--fft-only", choices=["mixed", "fp64"], default=None,
+                    help="only the batched transform sweep and a copy kernel (for profilers)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    n, q, B = args.n, args.q, args.B
+    if args.fft_only:
+        from maxlq_amd.lq import MaxLq
+        g = MaxLq(n, q=q, B=B, precision=args.fft_only)
+        passes, gbs, strict = fft_roofline(g)
+        rb, _ = g.sizes()
+        tc = g.time_copy(12, 5)
+        print(json.dumps({"fft_passes": passes, "fft_pass_GBs": gbs, "fft_strict_GBs": strict,
+                          "copy": {"fields": 12, "ms": tc, "bytes": 2 * 12 * rb,
+                                   "GBs": 2 * 12 * rb / (tc * 1e-3) / 1e9}}), flush=True)
+        return
+
+    log(f"mixed precision, batched line search: {n}^3 q={q} B={B}")
+    gm, hm, resm, t_mixed, _ = run_rcg(n, q, B, "mixed", 1, args.warmup, args.steps)
+    nJm = resm.n_J / max(resm.iters, 1)
+    fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
+    log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
+
+    t_fp64 = None
+    dR = None
+    if not args.skip_fp64:
+        log("plain FP64 path")
+        g64, h64, res64, t_fp64, _ = run_rcg(n, q, B, "fp64", 0, args.warmup, args.steps)
+        # |dR|/R: R of the same field in both precisions (mixed iterate)
+        u = gm.download(hm)
+        Rm = gm.calc_local_dLqdt_inclParts(u)[0]
+        R64 = g64.calc_local_dLqdt_inclParts(u)[0]
+        dR = abs(Rm - R64) / abs(R64)
+        del u
+        g64.close()
+
+    # dominant pass kernel for the roofline block
+    dom = max(fft_passes, key=lambda k: fft_passes[k]["ms"])
+    dp = fft_passes[dom]
+    roof = {"bound": "hbm", "achieved": round(dp["GBs"], 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(dp["GBs"] / PEAK_HBM_GBS, 4), "traffic": None,
+            "kernel": f"fft_{dom}"}
+
+    cpu = None
+    if not args.skip_cpu:
+        thr = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline (oracle) with {thr} threads")
+        tJ = cpu_baseline(q, B, thr)
+        scale = 8.0 * (27.0 / 24.0)                     # (512/256)^3 * log ratio
+        t_iter_cpu = (548.0 / 94.0 + nJm) * tJ * scale
+        cpu = {"value": 1.0 / t_iter_cpu, "unit": "iter/s", "cores": thr, "kind": "port",
+               "sample": f"oracle eval_J(LineMin) at 256^3 took {tJ:.2f}s; scaled x{scale:.2f} "
+                         f"to 512^3 and x(548/94 + n_J={nJm:.1f}) J-equivalents per iteration"}
+
+    out = {
+        "metric": "RCG iter/s at 512^3 q=4: mixed vs FP64 (|dR|/R<1e-7); FFT HBM GB/s",
+        "value": 1.0 / t_mixed,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"RCG iteration {n}^3 q={q:g} B={B:g} seeded exp(-|m|^2/256)",
+                   "n": n, "q": q, "B": B, "seed": 4,
+                   "fp64_iter_per_s": (1.0 / t_fp64) if t_fp64 else None,
+                   "mixed_over_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
+                   "dR_over_R": dR, "n_J_per_iter_mixed": nJm,
+                   "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
+                   "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -180,6 +180,9 @@ int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_res
  * fields on the context stream with HIP events; per-pass average kernel
  * durations (ms) are returned in t_ms[0..5] = {x fwd, y fwd, z fwd, z inv, y inv, x inv} */
 int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms);
+/* time `reps` launches of a plain device copy of `nfields` real fields (the
+ * calibration kernel for HBM byte counters); average ms per launch in *t_ms */
+int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms);
 /* bytes of one real field and of one half spectrum in the active precision */
 int mlq_sizes(const mlq_ctx* ctx, double* real_bytes, double* spec_bytes);
 /* per-kernel-class timing accumulated while profiling is on (HIP events on the

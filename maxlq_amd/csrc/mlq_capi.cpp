@@ -347,6 +347,9 @@ int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_res
 int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms) {
   return guarded(ctx, [&] { ctx->eng->time_fft(batch, reps, t_ms); });
 }
+int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms) {
+  return guarded(ctx, [&] { ctx->eng->time_copy(nfields, reps, t_ms); });
+}
 int mlq_sizes(const mlq_ctx* ctx, double* rb, double* sb) {
   if (!ctx || !rb || !sb) return MLQ_EINVAL;
   *rb = ctx->eng->real_bytes();

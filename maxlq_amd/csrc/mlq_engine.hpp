@@ -81,6 +81,7 @@ class IEngine {
 
   // measurement
   virtual void time_fft(int batch, int reps, double* t_ms) = 0;
+  virtual void time_copy(int nfields, int reps, double* t_ms) = 0;
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;
   virtual void mem_info(double* used, double* peak) = 0;

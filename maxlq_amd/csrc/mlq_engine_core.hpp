@@ -409,4 +409,16 @@ template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms
   prof_enable(was);
 }
 
+template <typename T> void Engine<T>::time_copy(int nfields, int reps, double* t_ms) {
+  nfields = std::max(1, nfields);
+  RT a = rtmp(nfields), b = rtmp(nfields);
+  map(F_fill<T>{a, (T)1}, nr_ * nfields);
+  const bool was = prof_;
+  prof_enable(true);
+  for (int i = 0; i < reps; ++i) map(F_copy<T>{a, b}, nr_ * nfields);
+  prof_flush();
+  *t_ms = prof_count_[PC_POINTWISE] ? prof_ms_[PC_POINTWISE] / prof_count_[PC_POINTWISE] : 0.0;
+  prof_enable(was);
+}
+
 }  // namespace mlq

@@ -87,6 +87,7 @@ template <typename T> class Engine final : public IEngine {
   void seeded(int f_vec, unsigned long long seed, int kind, double a) override;
   void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) override;
   void time_fft(int batch, int reps, double* t_ms) override;
+  void time_copy(int nfields, int reps, double* t_ms) override;
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
   void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }

@@ -290,6 +290,11 @@ class MaxLq:
         self._chk(self.lib.mlq_time_fft(self.ctx, batch, reps, _p(out)))
         return out
 
+    def time_copy(self, nfields=3, reps=5):
+        r = C.c_double()
+        self._chk(self.lib.mlq_time_copy(self.ctx, nfields, reps, C.byref(r)))
+        return r.value
+
     def sizes(self):
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.mlq_sizes(self.ctx, C.byref(a), C.byref(b)))
