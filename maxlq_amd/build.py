@@ -49,7 +49,34 @@ def build(jobs: int = 3, verbose: bool = True) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_fortran(verbose)
     return OUT
+
+
+FDIR = os.path.join(HERE, "fortran")
+FSMOKE = os.path.join(FDIR, "mlq_fsmoke")
+FC = os.environ.get("MLQ_FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = True) -> str | None:
+    """The Fortran interface module and the Fortran smoke caller (amdflang)."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("amdflang not found: Fortran caller not built")
+        return None
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FC, "-O2", "-J", mod, os.path.join(FDIR, "maxlq_c.f90"),
+           os.path.join(FDIR, "mlq_fsmoke.f90"), "-o", FSMOKE + ".tmp",
+           "-L", HERE, "-lmaxlq", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mod)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran build failed")
+    os.replace(FSMOKE + ".tmp", FSMOKE)
+    if verbose:
+        print(f"built {FSMOKE}")
+    return FSMOKE
 
 
 if __name__ == "__main__":

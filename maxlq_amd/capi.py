@@ -31,7 +31,7 @@ EXPORTS = [
     "mlq_calc_local_dLqdt_inclParts", "mlq_calc_global_dLqdt_inclParts", "mlq_GradL2ForLq",
     "mlq_eval_J", "mlq_field_alloc", "mlq_field_free", "mlq_upload", "mlq_download",
     "mlq_seeded_field", "mlq_eval_J_batch", "mlq_rcg_solve", "mlq_time_fft", "mlq_time_copy",
-    "mlq_sizes",
+    "mlq_autotune", "mlq_sizes",
     "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info",
 ]
 
@@ -113,6 +113,7 @@ def load(path: str = LIB_PATH):
         "mlq_rcg_solve": (ci, [vp, ci, C.POINTER(RcgCfg), C.POINTER(RcgResult), LOG_CB, vp]),
         "mlq_time_fft": (ci, [vp, ci, ci, dp]),
         "mlq_time_copy": (ci, [vp, ci, ci, dp]),
+        "mlq_autotune": (ci, [vp, ci, dp, C.POINTER(ci)]),
         "mlq_sizes": (ci, [vp, dp, dp]),
         "mlq_prof_enable": (ci, [vp, ci]),
         "mlq_prof_get": (ci, [vp, ci, C.POINTER(C.c_longlong), dp, dp]),

@@ -1,6 +1,7 @@
 // extern "C" entry points of libmaxlq.so (declared in include/maxlq.h).
 // Host arrays use the reference layout (FP64, x fastest, component blocks).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -82,6 +83,8 @@ int mlq_init(mlq_ctx** ctx, int n1, int n2, int n3, const mlq_params* p) {
   c->n = n;
   int rc = guarded(c, [&] {
     c->eng = p->precision == MLQ_MIXED ? make_engine_fp32(n, *p) : make_engine_fp64(n, *p);
+    const char* at = std::getenv("MLQ_AUTOTUNE");
+    if (n >= 128 && !(at && at[0] == '0')) c->eng->autotune(3, nullptr, nullptr);
   });
   if (rc != MLQ_OK) { delete c; return rc; }
   *ctx = c;
@@ -346,6 +349,9 @@ int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_res
 // ---- measurement ----
 int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms) {
   return guarded(ctx, [&] { ctx->eng->time_fft(batch, reps, t_ms); });
+}
+int mlq_autotune(mlq_ctx* ctx, int batch, double* times, int* choice) {
+  return guarded(ctx, [&] { ctx->eng->autotune(batch, times, choice); });
 }
 int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms) {
   return guarded(ctx, [&] { ctx->eng->time_copy(nfields, reps, t_ms); });

@@ -23,10 +23,10 @@ struct Error : std::runtime_error {
 
 #define MLQ_HIP(call)                                                        \
   do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess)                                                    \
+    hipError_t mlq_hip_rc_ = (call);                                         \
+    if (mlq_hip_rc_ != hipSuccess)                                           \
       throw ::mlq::Error(MLQ_EHIP, std::string(#call " failed: ") +          \
-                                       hipGetErrorString(e_));               \
+                                       hipGetErrorString(mlq_hip_rc_));      \
   } while (0)
 
 // Spectral multiplier applied per half-spectrum mode (kx, ky, kz).  All wave

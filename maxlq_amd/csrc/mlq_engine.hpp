@@ -82,6 +82,9 @@ class IEngine {
   // measurement
   virtual void time_fft(int batch, int reps, double* t_ms) = 0;
   virtual void time_copy(int nfields, int reps, double* t_ms) = 0;
+  // time the tile variants of the strided passes and keep the fastest;
+  // times[pass*4+var] in ms per launch, choice[pass]
+  virtual void autotune(int batch, double* times, int* choice) = 0;
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;
   virtual void mem_info(double* used, double* peak) = 0;

@@ -10,8 +10,9 @@ namespace mlq {
 template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
   n_ = n;
   px_ = n / 2 + 1;
+  pp_ = spec_pitch<T>(n);
   nr_ = (long long)n * n * n;
-  ns_ = (long long)n * n * px_;
+  ns_ = (long long)n * n * pp_;
   resolve_params();
   MLQ_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
   // twiddles exp(-2 pi i k/N), computed in FP64 on the host
@@ -240,7 +241,7 @@ template <typename T> bool Engine<T>::has_nan(const T* a, long long n) {
 template <typename T> double Engine<T>::spec_dot(const C* a, const C* b, int kind) {
   double r;
   const double s = 1.5 - 1.0 / p_.q;
-  reduce<1>(R_specdot<T>{a, b, n_, px_, kind, p_.lambda1, s}, ns_, &r);
+  reduce<1>(R_specdot<T>{a, b, n_, pp_, kind, p_.lambda1, s}, ns_, &r);
   return r;
 }
 
@@ -255,6 +256,39 @@ template <int N, typename T> constexpr int strided_C() {
   constexpr int c = StridedCfg<T>::cmin > 256 / Tf ? StridedCfg<T>::cmin : 256 / Tf;
   return c;
 }
+// Tile variants of the strided (y/z) passes: E elements per thread, C adjacent
+// columns per workgroup.  Variant 0 is the default; 1-3 are the candidates
+// the autotuner times for N >= 128 (invalid shapes fall back to variant 0).
+template <typename T, int N, int V> struct SVar {
+  static constexpr int cmin = sizeof(T) == 4 ? 16 : 8;  // 128-byte row segments
+  static constexpr int E0 = FftShape<N>::E;
+  static constexpr int C0 = cmin > 256 / (N / E0) ? cmin : 256 / (N / E0);
+  static constexpr int Er = V == 0 ? E0 : (V == 3 ? 8 : 16);
+  static constexpr int Cr = V == 0 ? C0 : (V == 1 ? cmin : (V == 2 ? 2 * cmin : cmin / 2));
+  static constexpr bool ok = V == 0 || (N >= 128 && Er <= N && (N / Er) * Cr <= 1024 &&
+                                        (long long)N * Cr * 2 * (long long)sizeof(T) <= 160 * 1024);
+  static constexpr int E = ok ? Er : E0;
+  static constexpr int C = ok ? Cr : C0;
+};
+
+template <typename T, int N, int V, int DIR, int MODE>
+void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, long long es, long long ps,
+                      int planes) {
+  using S = SVar<T, N, V>;
+  hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), dim3((ncols + S::C - 1) / S::C, planes),
+                     dim3(S::C * (N / S::E)), 0, st, sa, ncols, es, ps);
+}
+template <typename T, int N, int DIR, int MODE>
+void launch_strided(int var, hipStream_t st, const SArgs<T>& sa, int ncols, long long es,
+                    long long ps, int planes) {
+  switch (var) {
+    case 1: launch_strided_v<T, N, 1, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
+    case 2: launch_strided_v<T, N, 2, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
+    case 3: launch_strided_v<T, N, 3, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
+    default: launch_strided_v<T, N, 0, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
+  }
+}
+
 template <int N> constexpr int xpass_NF() {
   constexpr int Tf = FftShape<N>::Tf;
   constexpr int nf = 256 / Tf > 0 ? 256 / Tf : 1;
@@ -264,9 +298,8 @@ template <int N> constexpr int xpass_NF() {
 template <typename T> template <int N>
 void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {
   constexpr int NF = xpass_NF<N>();
-  constexpr int CC = strided_C<N, T>();
   constexpr int Tf = FftShape<N>::Tf;
-  constexpr int PX = N / 2 + 1;
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
   XArgs<T> xa{};
   for (int b = 0; b < nb; ++b) xa.rsrc[b] = src[b];
   xa.spec = scratch_;
@@ -285,14 +318,12 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   sa.tw = tw_;
   sa.sstride = ns_;
   prof_begin(PC_YFWD);
-  hipLaunchKernelGGL((k_strided<T, N, CC, -1, 0>), dim3((PX + CC - 1) / CC, N * nb), dim3(CC * Tf),
-                     0, st_, sa, PX, (long long)PX, (long long)N * PX);
+  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, PX, (long long)PP, (long long)N * PP, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2 * sb);
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
   prof_begin(PC_ZFWD);
-  hipLaunchKernelGGL((k_strided<T, N, CC, -1, 2>), dim3((N * PX + CC - 1) / CC, nb), dim3(CC * Tf),
-                     0, st_, sa, N * PX, (long long)N * PX, (long long)ns_);
+  launch_strided<T, N, -1, 2>(svar_[1], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZFWD, 2 * sb);
 }
@@ -300,9 +331,8 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
 template <typename T> template <int N>
 void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst) {
   constexpr int NF = xpass_NF<N>();
-  constexpr int CC = strided_C<N, T>();
   constexpr int Tf = FftShape<N>::Tf;
-  constexpr int PX = N / 2 + 1;
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
@@ -316,13 +346,11 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   }
   const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
   prof_begin(PC_ZINV);
-  hipLaunchKernelGGL((k_strided<T, N, CC, 1, 1>), dim3((N * PX + CC - 1) / CC, nb), dim3(CC * Tf),
-                     0, st_, sa, N * PX, (long long)N * PX, (long long)ns_);
+  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZINV, sb + zin * spec_bytes());
   prof_begin(PC_YINV);
-  hipLaunchKernelGGL((k_strided<T, N, CC, 1, 0>), dim3((PX + CC - 1) / CC, N * nb), dim3(CC * Tf),
-                     0, st_, sa, PX, (long long)PX, (long long)N * PX);
+  launch_strided<T, N, 1, 0>(svar_[3], st_, sa, PX, (long long)PP, (long long)N * PP, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YINV, 2 * sb);
   XArgs<T> xa{};
@@ -407,6 +435,72 @@ template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms
   prof_flush();
   for (int c = 0; c < 6; ++c) t_ms[c] = prof_count_[c] ? prof_ms_[c] / prof_count_[c] : 0.0;
   prof_enable(was);
+}
+
+// One strided pass alone, in place on the scratch spectra, timed with HIP
+// events on the engine stream (ms per launch).
+template <typename T> template <int N>
+double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  SArgs<T> sa{};
+  sa.spec = scratch_;
+  sa.tw = tw_;
+  sa.sstride = ns_;
+  for (int b = 0; b < batch; ++b) {
+    sa.src[b][0] = scratch_ + b * ns_;
+    sa.coef[b][0] = 1.0;
+    sa.nsrc[b] = 1;
+    sa.dst[b] = scratch_ + b * ns_;
+    sa.op[b] = sp_none();
+  }
+  hipEvent_t a, z;
+  MLQ_HIP(hipEventCreate(&a));
+  MLQ_HIP(hipEventCreate(&z));
+  auto run = [&]() {
+    switch (pass) {
+      case 0: launch_strided<T, N, -1, 0>(var, st_, sa, PX, PP, (long long)N * PP, N * batch); break;
+      case 1: launch_strided<T, N, -1, 2>(var, st_, sa, N * PP, (long long)N * PP, ns_, batch); break;
+      case 2: launch_strided<T, N, 1, 1>(var, st_, sa, N * PP, (long long)N * PP, ns_, batch); break;
+      default: launch_strided<T, N, 1, 0>(var, st_, sa, PX, PP, (long long)N * PP, N * batch); break;
+    }
+  };
+  run();  // warm-up
+  MLQ_HIP(hipEventRecord(a, st_));
+  for (int i = 0; i < reps; ++i) run();
+  MLQ_HIP(hipEventRecord(z, st_));
+  MLQ_HIP(hipEventSynchronize(z));
+  float ms = 0.f;
+  MLQ_HIP(hipEventElapsedTime(&ms, a, z));
+  MLQ_HIP(hipEventDestroy(a));
+  MLQ_HIP(hipEventDestroy(z));
+  return (double)ms / reps;
+}
+
+template <typename T> void Engine<T>::autotune(int batch, double* times, int* choice) {
+  batch = std::max(1, std::min(batch, MAXB));
+  MLQ_HIP(hipMemsetAsync(scratch_, 0, sizeof(C) * ns_ * batch, st_));
+  for (int pass = 0; pass < 4; ++pass) {
+    double best = 1e300;
+    int bv = 0;
+    for (int v = 0; v < 4; ++v) {
+      double t = 0.0;
+      switch (n_) {
+        case 8: t = time_pass_n<8>(pass, v, batch, 3); break;
+        case 16: t = time_pass_n<16>(pass, v, batch, 3); break;
+        case 32: t = time_pass_n<32>(pass, v, batch, 3); break;
+        case 64: t = time_pass_n<64>(pass, v, batch, 3); break;
+        case 128: t = time_pass_n<128>(pass, v, batch, 3); break;
+        case 256: t = time_pass_n<256>(pass, v, batch, 3); break;
+        case 512: t = time_pass_n<512>(pass, v, batch, 3); break;
+        case 1024: t = time_pass_n<1024>(pass, v, batch, 3); break;
+        default: throw Error(MLQ_EINVAL, "unsupported n");
+      }
+      if (times) times[pass * 4 + v] = t;
+      if (t < best) { best = t; bv = v; }
+    }
+    svar_[pass] = bv;
+    if (choice) choice[pass] = bv;
+  }
 }
 
 template <typename T> void Engine<T>::time_copy(int nfields, int reps, double* t_ms) {

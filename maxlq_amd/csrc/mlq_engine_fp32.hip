@@ -2,6 +2,7 @@
 // transforms, FP64 reductions and host scalars).
 #include "mlq_ops_core.hpp"
 #include "mlq_rcg.hpp"
+#include "mlq_fast.hpp"
 
 namespace mlq {
 

@@ -1,6 +1,7 @@
 // FP64 instantiation of the engine (the parity path).
 #include "mlq_ops_core.hpp"
 #include "mlq_rcg.hpp"
+#include "mlq_fast.hpp"
 
 namespace mlq {
 

@@ -14,6 +14,9 @@
 namespace mlq {
 
 template <typename T> class Engine;
+template <typename T> class FastRcg;
+template <typename T>
+void rcg_solve_fast(Engine<T>&, int, const mlq_rcg_cfg&, mlq_rcg_result&, const LogSink&);
 
 // RAII handle of a pooled temporary: k contiguous real fields or half spectra.
 template <typename T, typename P> class Tmp {
@@ -53,7 +56,8 @@ template <typename T> class Engine final : public IEngine {
   void set_params(const mlq_params& p) override { p_ = p; resolve_params(); }
   void sync() override { MLQ_HIP(hipStreamSynchronize(st_)); }
   double real_bytes() const override { return (double)nr_ * sizeof(T); }
-  double spec_bytes() const override { return (double)ns_ * sizeof(C); }
+  // algorithmic bytes of one half spectrum (the row padding is never touched)
+  double spec_bytes() const override { return (double)n_ * n_ * px_ * sizeof(C); }
 
   int field_alloc(int ncomp) override;
   void field_free(int h) override;
@@ -88,6 +92,7 @@ template <typename T> class Engine final : public IEngine {
   void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) override;
   void time_fft(int batch, int reps, double* t_ms) override;
   void time_copy(int nfields, int reps, double* t_ms) override;
+  void autotune(int batch, double* times, int* choice) override;
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
   void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }
@@ -151,7 +156,10 @@ template <typename T> class Engine final : public IEngine {
   bool has_nan(const T* a, long long n);
 
   // ---- state ----
-  int n_, px_;
+  int n_, px_, pp_;   // grid size, n/2+1 modes along x, spectral row pitch
+  // tile variant of the strided passes {y fwd, z fwd, z inv, y inv} (autotuned)
+  int svar_[4] = {0, 0, 0, 0};
+  template <int N> double time_pass_n(int pass, int var, int batch, int reps);
   long long nr_, ns_;
   mlq_params p_;
   bool use_e_u_ = false;
@@ -174,7 +182,14 @@ template <typename T> class Engine final : public IEngine {
   double prof_bytes_[PC_COUNT] = {};
   void prof_flush();
 
-  template <typename U> friend class RcgDriver;
+  // the reference-structured solve (every operator recomputed from the field)
+  void rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log);
+  void eval_J_batch_ref(int u, int d, const double* taus, int ntau, double* J);
+  bool fast_path_ok() const { return !use_e_u_ && p_.q >= 4.0 - 1e-12 && p_.toDealias; }
+
+  template <typename U> friend class FastRcg;
+  template <typename U>
+  friend void rcg_solve_fast(Engine<U>&, int, const mlq_rcg_cfg&, mlq_rcg_result&, const LogSink&);
 };
 
 template <typename T, typename P> void Tmp<T, P>::reset() {

@@ -1,0 +1,679 @@
+// Resident-state RCG iteration (the fast path of maxdLqdt, q >= 4 without the
+// e_u variant).  Same mathematics as optimization.f90:7-514 with the
+// reference operators; what changes is where the work happens:
+//
+//  * u, grad u, D(u.u), |u|^(q-2), the pressure spectrum, the spectra of
+//    D(|u|^(q-2) u) and of |grad u|^2 are kept in HBM across iterations; after
+//    the step they follow from the line-search fields by linear combination
+//    and by the homogeneity of every operator under u -> s u.
+//  * Gradients, Hilbert weights, Leray projections, H_l inner products, the
+//    CG direction and the vector transport stay in spectral space.
+//  * For phi = u + tau d every dealiased quadratic quantity of the line
+//    search (D(phi.phi), w_j = D(sum_i phi_i d_j phi_i), the pressure,
+//    |grad phi|^2) is a quadratic polynomial in tau whose coefficient fields
+//    are computed once per iteration; J(tau) then needs only the nonlinear
+//    |phi|^k chain (for q = 4 a single forward transform of g^2, with R2 and
+//    R3 polynomials in tau and R1, ||phi||_q^q spectral sums) and the
+//    retraction B phi/||phi||_q enters through R(s phi) = s^q (R1+R2)(phi) +
+//    s^(q+1) R3(phi).
+#pragma once
+#include <map>
+#include <vector>
+
+#include "mlq_fast_kernels.hpp"
+#include "mlq_linesearch.hpp"
+#include "mlq_ops_core.hpp"
+
+namespace mlq {
+
+template <typename T> class FastRcg {
+  using E_ = Engine<T>;
+  using C = c2<T>;
+  using RT = typename E_::RT;
+  using ST = typename E_::ST;
+  using InvSrc = typename E_::InvSrc;
+
+ public:
+  FastRcg(E_& e) : e_(e), p_(e.p_) {
+    q_ = p_.q;
+    B_ = p_.B;
+    nuvc_ = p_.visc * p_.viscCoefficient;
+    pc_ = p_.pressureCoefficient;
+    n_ = e.n_;
+    pp_ = e.pp_;
+    nr_ = e.nr_;
+    ns_ = e.ns_;
+    M_ = e.mcut_of(2.0);
+    q4_ = std::fabs(q_ - 4.0) < 1e-14;
+    N3_ = (double)nr_;
+    N6_ = N3_ * N3_;
+  }
+
+  // ---- state from a physical field ----------------------------------------------
+  void init_state(const T* u_in) {
+    u_ = e_.rtmp(3);
+    e_.map(F_copy<T>{u_in, u_}, nr_ * 3);
+    uh_ = e_.stmp(3);
+    spec3(u_, uh_);
+    G_ = e_.rtmp(9);
+    grad9(uh_, G_);
+    s_ = e_.rtmp();
+    e_.map(F_dot3self<T>{u_, nr_, s_}, nr_);
+    dealias1(s_);
+    chain_state_from_s();
+    // pressure spectrum: M FFT(grad u : grad u^T) / |K|^2  (solve_poisson(-aux))
+    ph_ = e_.stmp();
+    {
+      RT X = e_.rtmp();
+      e_.map(F_gradgradT<T>{G_, nr_, X}, nr_);
+      e_.fwd1(X, ph_, poisson_op());
+    }
+    // W_i = M FFT(|u|^(q-2) u_i)
+    W_ = e_.stmp(3);
+    fwd_Au(A_, u_, W_);
+    // w_j = D(sum_i u_i d_j u_i)
+    a_ = e_.rtmp(3);
+    e_.map(F_vgradu<T>{u_, G_, nr_, a_, a_ + nr_, a_ + 2 * nr_}, nr_);
+    dealias3(a_);
+    // |grad u|^2: masked spectrum (q = 4) or physical (other q)
+    Q0_ = e_.rtmp();
+    e_.map(F_gradsq<T>{G_, nr_, Q0_}, nr_);
+    Q0h_ = e_.stmp();
+    e_.fwd1(Q0_, Q0h_, sp_mask(M_));
+  }
+
+  void download_u(T* dst) { e_.map(F_copy<T>{u_, dst}, nr_ * 3); }
+
+  // direction d given as a physical field (for batched J evaluations through
+  // the C ABI)
+  void set_direction_phys(const T* d) {
+    dh_ = e_.stmp(3);
+    spec3(d, dh_);
+  }
+
+  // ---- one RCG iteration up to the CG direction (optimization.f90:146-261) --------
+  // Returns the message codes raised by the checks.
+  struct DirOut { double beta; std::vector<int> msgs; };
+  DirOut direction(int iter, int reset_every) {
+    DirOut out;
+    const double s = 1.5 - 1.0 / q_;
+    // grad p and Laplacian of u
+    RT gp = e_.rtmp(3), lap = e_.rtmp(3);
+    {
+      InvSrc is[6];
+      SpecOp ops[6];
+      T* dst[6];
+      for (int j = 0; j < 3; ++j) {
+        is[j] = InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[j] = sp_ik(j);
+        dst[j] = gp + j * nr_;
+        is[3 + j] = InvSrc{{uh_ + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        SpecOp l; l.flags = SP_NEGKSQ; ops[3 + j] = l;
+        dst[3 + j] = lap + j * nr_;
+      }
+      e_.inv(6, is, ops, dst);
+    }
+    // aux2 = D(|u|^(q-4) D(u.(nu Lap u - grad p)))           (:1131-1138)
+    RT aux2 = e_.rtmp();
+    e_.map(F_udotforce<T>{u_, lap, gp, nr_, (T)nuvc_, (T)pc_, aux2}, nr_);
+    dealias1(aux2);
+    if (!q4_) {
+      e_.map(F_mul<T>{Bq_, aux2, aux2}, nr_);
+      dealias1(aux2);
+    }
+    // grad phi, phi = Lap^-1 div D(|u|^(q-2) u)               (:1155-1158)
+    RT gphi = e_.rtmp(3);
+    {
+      ST dv = e_.stmp();
+      e_.smap(S_div<T>{W_, W_ + ns_, W_ + 2 * ns_, dv, n_, pp_});
+      InvSrc is[3];
+      SpecOp ops[3];
+      T* dst[3];
+      for (int j = 0; j < 3; ++j) {
+        is[j] = InvSrc{{dv.get(), nullptr, nullptr}, {1.0, 0, 0}, 1};
+        SpecOp o; o.flags = SP_IK | SP_INVKSQ; o.dim = j; ops[j] = o;
+        dst[j] = gphi + j * nr_;
+      }
+      e_.inv(3, is, ops, dst);
+    }
+    // V_hat = M FFT(Y) - nu |K|^2 W  (the final sum of GradL2ForLq, :1186-1190)
+    ST Vh = e_.stmp(3);
+    {
+      RT Y = e_.rtmp(3);
+      e_.map(F_grad_terms<T>{aux2, u_, A_, gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+                             (T)nuvc_, Y}, nr_);
+      fwd3(Y, Vh, sp_mask(M_));
+      e_.smap(S_add_lap<T>{Vh, W_, ns_, n_, pp_, nuvc_});
+    }
+    // Hilbert gradient + Leray (:150-153)
+    e_.smap(S_hilbert_leray<T>{Vh, Vh + ns_, Vh + 2 * ns_, n_, pp_, p_.lambda1, 2.0 * s, 1.0});
+    // normal: Leray(Hilbert(||u||_q^(1-q) W))                  (:162-169)
+    ST nH = e_.stmp(3);
+    e_.smap(S_axpby3<T>{W_, nullptr, nH, ns_, std::pow(L_, 1.0 - q_), 0.0});
+    e_.smap(S_hilbert_leray<T>{nH, nH + ns_, nH + 2 * ns_, n_, pp_, p_.lambda1, 2.0 * s, 1.0});
+    const double nn = hl(nH, nH);
+    const double alpha = hl(nH, Vh) / nn;                        // :170-171
+    ST gproj = e_.stmp(3);
+    e_.smap(S_axpby3<T>{Vh, nH, gproj, ns_, 1.0, -alpha});       // :177
+    {
+      ST sum = e_.stmp(3);
+      e_.smap(S_axpby3<T>{gproj, nH, sum, ns_, 1.0, 1.0});
+      const double t1 = std::fabs(hl(gproj, nH) / hl(sum, sum)); // :180
+      if (t1 > 9.999999960041972e-13) out.msgs.push_back(42);
+    }
+    const double gg = hl(gproj, gproj);
+    {
+      double dn;
+      e_.template reduce<1>(R_divnorm<T>{gproj, ns_, n_, pp_}, ns_, &dn);
+      const double t2 = (dn / N6_) / gg;                          // :188-189
+      if (t2 > 1.0000000036274937e-15) out.msgs.push_back(43);
+    }
+    // normalised tangential gradient (:201-214)
+    const double c = std::sqrt(hl(uh_, uh_) / gg);
+    gu1_ = e_.stmp(3);
+    e_.smap(S_axpby3<T>{gproj, nullptr, gu1_, ns_, c, 0.0});
+    // Polak-Ribiere with transported vectors (:226-240)
+    double beta = 0.0;
+    if (have_cg_) {
+      ST diff = e_.stmp(3);
+      e_.smap(S_axpby3<T>{gu1_, vg0_, diff, ns_, 1.0, -1.0});
+      beta = hl(gu1_, diff);
+      const double g0n = hl(gu0_, gu0_);
+      if (g0n < MACH_EPSILON || beta < MACH_EPSILON) beta = 0.0;
+      else beta = beta / g0n;
+    }
+    if (reset_every > 0 && iter % reset_every == 0) beta = 0.0;
+    dh_ = e_.stmp(3);
+    if (have_cg_) e_.smap(S_axpby3<T>{gu1_, vd0_, dh_, ns_, 1.0, beta});
+    else e_.smap(S_axpby3<T>{gu1_, nullptr, dh_, ns_, 1.0, 0.0});
+    // zero-mode checks (:251-261): the k = 0 coefficient is the unnormalised mean
+    {
+      double m0 = k0_abs_sum(dh_);
+      if (m0 > 1.0000000474974513e-03 * B_) out.msgs.push_back(44);
+      m0 = k0_abs_sum(uh_);
+      if (m0 > 1.0000000474974513e-03 * B_) out.msgs.push_back(44);
+    }
+    out.beta = beta;
+    return out;
+  }
+
+  // ---- line-search coefficient fields for phi = u + tau d --------------------------
+  void precompute_ls() {
+    d_ = e_.rtmp(3);
+    Gd_ = e_.rtmp(9);
+    {
+      InvSrc is[12];
+      SpecOp ops[12];
+      T* dst[12];
+      for (int i = 0; i < 3; ++i) {
+        is[i] = InvSrc{{dh_ + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[i] = sp_none();
+        dst[i] = d_ + i * nr_;
+        for (int j = 0; j < 3; ++j) {
+          is[3 + i * 3 + j] = is[i];
+          ops[3 + i * 3 + j] = sp_ik(j);
+          dst[3 + i * 3 + j] = Gd_ + (i * 3 + j) * nr_;
+        }
+      }
+      e_.inv(12, is, ops, dst);
+    }
+    S1_ = e_.rtmp(); S2_ = e_.rtmp(); b_ = e_.rtmp(3); ee_ = e_.rtmp(3);
+    RT X12 = e_.rtmp(2);
+    Q1_ = e_.rtmp(); Q2_ = e_.rtmp();
+    e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_},
+           nr_);
+    // dealias the quadratic coefficients (batch of 8)
+    {
+      T* f[8] = {S1_, S2_, b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_};
+      e_.dealias_dev(f, 8, 2.0);
+    }
+    // pressure coefficients P1, P2 (spectra kept for the state update) and P0
+    P12h_ = e_.stmp(2);
+    {
+      const T* src[2] = {X12, X12 + nr_};
+      C* dst[2] = {P12h_, P12h_ + ns_};
+      SpecOp ops[2] = {poisson_op(), poisson_op()};
+      e_.fwd(2, src, dst, ops);
+    }
+    P_ = e_.rtmp(3);
+    {
+      InvSrc is[3] = {InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
+                      InvSrc{{P12h_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
+                      InvSrc{{P12h_ + ns_, nullptr, nullptr}, {1.0, 0, 0}, 1}};
+      SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
+      T* dst[3] = {P_, P_ + nr_, P_ + 2 * nr_};
+      e_.inv(3, is, ops, dst);
+    }
+    // |grad phi|^2 coefficient spectra
+    Q12h_ = e_.stmp(2);
+    {
+      const T* src[2] = {Q1_, Q2_};
+      C* dst[2] = {Q12h_, Q12h_ + ns_};
+      SpecOp ops[2] = {sp_mask(M_), sp_mask(M_)};
+      e_.fwd(2, src, dst, ops);
+    }
+    if (q4_) {
+      double acc[11];
+      e_.template reduce<11>(R_ls_poly<T>{a_, b_, ee_, u_, d_, P_, P_ + nr_, P_ + 2 * nr_, nr_},
+                             nr_, acc);
+      for (int k = 0; k < 5; ++k) cpoly_[k] = acc[k];
+      for (int k = 0; k < 6; ++k) rpoly_[k] = acc[5 + k];
+    }
+    cache_.clear();
+  }
+
+  // ---- J_LineMin(tau) = -R(B phi/||phi||_q) for a batch of step sizes ---------------
+  void eval_batch(const double* taus, int nb, double* J, double* Lout = nullptr) {
+    for (int b0 = 0; b0 < nb; b0 += MAXB) {
+      const int k = std::min(MAXB, nb - b0);
+      if (q4_) eval_q4(taus + b0, k, J + b0, Lout ? Lout + b0 : nullptr);
+      else
+        for (int i = 0; i < k; ++i) eval_generic(taus[b0 + i], J + b0 + i, Lout ? Lout + b0 + i : nullptr);
+    }
+  }
+
+  // J from the cache (evaluated if absent) without counting it as a line-search
+  // request: J1 of the iteration is -J_LineMin(tau1) (optimization.f90:349)
+  double J_value(double tau) {
+    auto it = cache_.find(tau);
+    if (it != cache_.end()) return it->second;
+    const std::vector<double> none;
+    const double v = J(tau, none);
+    --nJ_;
+    return v;
+  }
+
+  // cached J with optional speculation list
+  double J(double tau, const std::vector<double>& spec) {
+    auto it = cache_.find(tau);
+    if (it != cache_.end()) { ++nJ_; return it->second; }
+    std::vector<double> batch{tau};
+    for (double t : spec) {
+      if ((int)batch.size() >= batch_) break;
+      if (cache_.count(t) || std::find(batch.begin(), batch.end(), t) != batch.end()) continue;
+      batch.push_back(t);
+    }
+    std::vector<double> Jv(batch.size());
+    eval_batch(batch.data(), (int)batch.size(), Jv.data());
+    nJc_ += (long long)batch.size();
+    ++nJ_;
+    for (size_t i = 0; i < batch.size(); ++i) {
+      if (std::isnan(Jv[i])) throw Error(MLQ_ENAN, "nan in J");
+      cache_[batch[i]] = Jv[i];
+    }
+    return Jv[0];
+  }
+
+  // ---- vector transport at the old u and the retraction (:326-346) -----------------
+  // Returns ||u + tau1 d||_q.
+  double transport_and_update(double tau1) {
+    const double q = q_;
+    // phi = u + tau1 d, its spectrum and the |phi|^k chain
+    RT phi = e_.rtmp(3);
+    e_.map(F_axpby<T>{u_, d_, phi, (T)1, (T)tau1}, nr_ * 3);
+    ST phih = e_.stmp(3);
+    e_.smap(S_axpby3<T>{uh_, dh_, phih, ns_, 1.0, tau1});
+    RT s1 = e_.rtmp();
+    e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
+    typename E_::Chain ch;
+    e_.chain_init(ch, nullptr, s1);
+    const T* A1 = e_.chain_pow(ch, q - 2.0);
+    const T* L1h = e_.chain_pow(ch, q / 2.0);
+    const double L1 = std::pow(e_.dot(L1h, L1h) / N3_, 1.0 / q);
+    ST Wphi = e_.stmp(3);
+    fwd_Au(A1, phi, Wphi);
+    // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1
+    double ir[2];
+    e_.template reduce<2>(R_kept_dot2<T>{Wphi, gu1_, dh_, ns_, n_, pp_, M_},
+                          (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), ir);
+    ir[0] /= N6_;
+    ir[1] /= N6_;
+    const double sc = B_ / L1;
+    const double Lmq = std::pow(L1, -q);
+    vg0_ = e_.stmp(3);
+    vd0_ = e_.stmp(3);
+    e_.smap(S_axpby3<T>{gu1_, phih, vg0_, ns_, sc, -sc * Lmq * ir[0]});
+    e_.smap(S_axpby3<T>{dh_, phih, vd0_, ns_, sc, -sc * Lmq * ir[1]});
+    gu0_ = std::move(gu1_);
+    have_cg_ = true;
+    // retraction and state update by homogeneity
+    e_.map(F_scale<T>{phi, (T)sc}, nr_ * 3);
+    std::swap(u_, phi);
+    e_.smap(S_axpby3<T>{phih, nullptr, uh_, ns_, sc, 0.0});
+    e_.map(F_axpby<T>{G_, Gd_, G_, (T)sc, (T)(sc * tau1)}, nr_ * 9);
+    e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
+    std::swap(s_, s1);
+    for (int j = 0; j < 3; ++j)
+      e_.map(F_quad<T>{a_ + j * nr_, b_ + j * nr_, ee_ + j * nr_, a_ + j * nr_, (T)tau1,
+                       (T)(sc * sc)}, nr_);
+    e_.smap(S_quad<T>{ph_, P12h_, P12h_ + ns_, ph_, tau1, sc * sc});
+    e_.smap(S_quad<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, tau1, sc * sc});
+    e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
+    e_.smap(S_axpby3<T>{Wphi, nullptr, W_, ns_, std::pow(sc, q - 1.0), 0.0});
+    // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
+    A_ = e_.rtmp();
+    e_.map(F_copy<T>{A1, A_}, nr_);
+    e_.map(F_scale<T>{A_, (T)std::pow(sc, q - 2.0)}, nr_);
+    if (!q4_) {
+      const T* Bq1 = e_.chain_pow(ch, q - 4.0);
+      Bq_ = e_.rtmp();
+      e_.map(F_copy<T>{Bq1, Bq_}, nr_);
+      e_.map(F_scale<T>{Bq_, (T)std::pow(sc, q - 4.0)}, nr_);
+    }
+    L_ = sc * L1;
+    release_ls();
+    return L1;
+  }
+
+  void release_ls() {
+    d_ = RT(); Gd_ = RT(); S1_ = RT(); S2_ = RT(); b_ = RT(); ee_ = RT(); Q1_ = RT(); Q2_ = RT();
+    P_ = RT(); P12h_ = ST(); Q12h_ = ST(); dh_ = ST();
+  }
+
+  double L() const { return L_; }
+  long long nJ_ = 0, nJc_ = 0;
+  int batch_ = 1;
+
+ private:
+  // ---- helpers ---------------------------------------------------------------------
+  SpecOp poisson_op() const {
+    SpecOp op;
+    op.flags = SP_MASK | SP_INVKSQ;
+    op.mcut = M_;
+    op.scale = -1.0;
+    return op;
+  }
+  void spec3(const T* u, C* uh) {
+    const T* src[3] = {u, u + nr_, u + 2 * nr_};
+    C* dst[3] = {uh, uh + ns_, uh + 2 * ns_};
+    SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
+    e_.fwd(3, src, dst, ops);
+  }
+  void fwd3(const T* f, C* h, SpecOp op) {
+    const T* src[3] = {f, f + nr_, f + 2 * nr_};
+    C* dst[3] = {h, h + ns_, h + 2 * ns_};
+    SpecOp ops[3] = {op, op, op};
+    e_.fwd(3, src, dst, ops);
+  }
+  void grad9(const C* uh, T* G) {
+    InvSrc is[9];
+    SpecOp ops[9];
+    T* dst[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        is[i * 3 + j] = InvSrc{{uh + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[i * 3 + j] = sp_ik(j);
+        dst[i * 3 + j] = G + (i * 3 + j) * nr_;
+      }
+    e_.inv(9, is, ops, dst);
+  }
+  void dealias1(T* f) { e_.dealias_dev(&f, 1, 2.0); }
+  void dealias3(T* f) {
+    T* g[3] = {f, f + nr_, f + 2 * nr_};
+    e_.dealias_dev(g, 3, 2.0);
+  }
+  // W_i = M FFT(A u_i)
+  void fwd_Au(const T* A, const T* u, C* W) {
+    RT t = e_.rtmp(3);
+    for (int i = 0; i < 3; ++i) e_.map(F_mul<T>{A, u + i * nr_, t + i * nr_}, nr_);
+    fwd3(t, W, sp_mask(M_));
+  }
+  double hl(const C* a, const C* b) {
+    double r;
+    e_.template reduce<1>(R_specdot3<T>{a, b, ns_, n_, pp_, 1, p_.lambda1, 1.5 - 1.0 / q_}, ns_, &r);
+    return r / N6_;
+  }
+  double k0_abs_sum(const C* h) {
+    C v[3];
+    for (int c = 0; c < 3; ++c)
+      MLQ_HIP(hipMemcpyAsync(&v[c], h + c * ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+    e_.sync();
+    return std::fabs((double)v[0].x) + std::fabs((double)v[1].x) + std::fabs((double)v[2].x);
+  }
+  void chain_state_from_s() {
+    typename E_::Chain ch;
+    e_.chain_init(ch, nullptr, s_);
+    const T* A = e_.chain_pow(ch, q_ - 2.0);
+    A_ = e_.rtmp();
+    e_.map(F_copy<T>{A, A_}, nr_);
+    if (!q4_) {
+      Bq_ = e_.rtmp();
+      e_.map(F_copy<T>{e_.chain_pow(ch, q_ - 4.0), Bq_}, nr_);
+    }
+    const T* Lh = e_.chain_pow(ch, q_ / 2.0);
+    L_ = std::pow(e_.dot(Lh, Lh) / N3_, 1.0 / q_);
+  }
+
+  void finish_J(double tau, double R12, double R3, double Lq_q, double* J, double* Lout) {
+    const double L = std::pow(Lq_q, 1.0 / q_);
+    const double sc = B_ / L;
+    *J = -(std::pow(sc, q_) * R12 + std::pow(sc, q_ + 1.0) * R3);
+    if (Lout) *Lout = L;
+    (void)tau;
+  }
+
+  void eval_q4(const double* taus, int nb, double* J, double* Lout) {
+    std::vector<RT> gg;
+    std::vector<ST> Ah;
+    F_ls_g<T> fg{};
+    fg.S0 = s_; fg.S1 = S1_; fg.S2 = S2_; fg.nb = nb;
+    std::vector<const T*> src(nb);
+    std::vector<C*> dst(nb);
+    std::vector<SpecOp> ops(nb, sp_mask(M_));
+    for (int b = 0; b < nb; ++b) {
+      gg.push_back(e_.rtmp());
+      Ah.push_back(e_.stmp());
+      fg.g[b] = nullptr;
+      fg.gg[b] = gg[b];
+      fg.tau[b] = taus[b];
+      src[b] = gg[b];
+      dst[b] = Ah[b];
+    }
+    e_.map(fg, nr_, PC_LS_POINT);
+    e_.fwd(nb, src.data(), dst.data(), ops.data());
+    R_ls_spec<T> rf{};
+    for (int b = 0; b < nb; ++b) { rf.A[b] = Ah[b]; rf.tau[b] = taus[b]; }
+    rf.Q0 = Q0h_; rf.Q1 = Q12h_; rf.Q2 = Q12h_ + ns_;
+    rf.nb = nb; rf.n = n_; rf.px = pp_; rf.M = M_;
+    double acc[2 * MAXB];
+    e_.template reduce<2 * MAXB>(rf, (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), acc);
+    for (int b = 0; b < nb; ++b) {
+      const double t = taus[b];
+      const double Lq_q = acc[2 * b] / N6_;
+      const double R1 = -nuvc_ * acc[2 * b + 1] / N6_;
+      double cp = 0.0, rp = 0.0;
+      for (int k = 4; k >= 0; --k) cp = cp * t + cpoly_[k];
+      for (int k = 5; k >= 0; --k) rp = rp * t + rpoly_[k];
+      const double R2 = -nuvc_ * (q_ - 2.0) * cp / N3_;
+      const double R3 = pc_ * (q_ - 2.0) * rp / N3_;
+      finish_J(t, R1 + R2, R3, Lq_q, J + b, Lout ? Lout + b : nullptr);
+    }
+  }
+
+  void eval_generic(double t, double* J, double* Lout) {
+    RT s1 = e_.rtmp();
+    e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)t, (T)1}, nr_);
+    typename E_::Chain ch;
+    e_.chain_init(ch, nullptr, s1);
+    const T* A = e_.chain_pow(ch, q_ - 2.0);
+    const T* Bq = e_.chain_pow(ch, q_ - 4.0);
+    const T* Lh = e_.chain_pow(ch, q_ / 2.0);
+    // D(Bq) unless |phi|^(q-4) is already band-limited by the 2/3 mask
+    RT Bw;
+    const T* Bwp = Bq;
+    if (q_ - 4.0 + 2.0e-16 < 2.0) {
+      Bw = e_.rtmp();
+      e_.map(F_copy<T>{Bq, Bw}, nr_);
+      dealias1(Bw);
+      Bwp = Bw;
+    }
+    RT Pd = e_.rtmp();
+    e_.map(F_pB<T>{P_, P_ + nr_, P_ + 2 * nr_, Bq, Pd, (T)t}, nr_);
+    dealias1(Pd);
+    // physical |grad phi|^2 coefficients: Q0 (state), Q1, Q2
+    double acc[4];
+    e_.template reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, nr_, t},
+                          nr_, acc);
+    const double R1 = -nuvc_ * acc[0] / N3_;
+    const double R2 = -nuvc_ * (q_ - 2.0) * acc[1] / N3_;
+    const double R3 = pc_ * (q_ - 2.0) * acc[2] / N3_;
+    finish_J(t, R1 + R2, R3, acc[3] / N3_, J, Lout);
+  }
+
+  E_& e_;
+  const mlq_params p_;
+  double q_, B_, nuvc_, pc_, N3_, N6_;
+  int n_, pp_, M_;
+  long long nr_, ns_;
+  bool q4_;
+  // state
+  RT u_, G_, s_, A_, Bq_, a_, Q0_;
+  ST uh_, ph_, W_, Q0h_;
+  double L_ = 1.0;
+  // CG state
+  ST gu0_, vg0_, vd0_, gu1_, dh_;
+  bool have_cg_ = false;
+  // line-search coefficients
+  RT d_, Gd_, S1_, S2_, b_, ee_, Q1_, Q2_, P_;
+  ST P12h_, Q12h_;
+  double cpoly_[5] = {0}, rpoly_[6] = {0};
+  std::map<double, double> cache_;
+};
+
+// ---------------------------------------------------------------------------
+// The solve loop on the resident state (optimization.f90:7-514)
+// ---------------------------------------------------------------------------
+template <typename T>
+void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res,
+                    const LogSink& log) {
+  auto t0 = std::chrono::steady_clock::now();
+  const double B = e.p_.B;
+  auto msg = [&](int iter, int code) { log.emit(MLQ_LOG_MSG, iter, code, 0, nullptr); };
+  e.rescale(u, B);                                                 // :61
+  e.divAvg_free(u);                                                // :62
+  double J0 = e.eval_J(u, MLQ_J_MAXDLQDT);                         // :78
+  double J1 = J0;
+  FastRcg<T> F(e);
+  F.batch_ = cfg.batch_linesearch > 0 ? std::min(MAXB, std::max(1, cfg.batch_linesearch)) : 1;
+  F.init_state(e.fphys(u));
+  int iter = cfg.start_iter;
+  const int start = cfg.start_iter;
+  double tau0 = 100.0, tau1 = tau0;
+  const int nroll = std::max(1, cfg.rolling);
+  std::vector<double> rel(nroll, 0.0);
+  double avg = 1.0;
+  int while_flag = 0, stop = 0;
+  {
+    double d[3] = {0.0, 0.0, J0};
+    log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
+  }
+  while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
+    iter += 1;
+    auto dir = F.direction(iter, cfg.reset_every);
+    for (int m : dir.msgs) msg(iter, m);
+    const double beta = dir.beta;
+    F.precompute_ls();
+    // speculative step sizes: the first mnbrak requests (0, tau0, GOLD*tau0)
+    const double tB0 = std::max(tau0, MACH_EPSILON);
+    std::vector<double> spec = {0.0, tB0, MNBRAK_GOLD * tB0, MNBRAK_CGOLD * tB0,
+                                MNBRAK_GOLD * (MNBRAK_GOLD * tB0), MNBRAK_CGOLD * (MNBRAK_CGOLD * tB0)};
+    JFun J = [&](double tau) { return F.J(tau, spec); };
+    LineminLog llog = [&](int mode, int bi, double tA, double tB, double tC, double FA, double FB,
+                          double FC) {
+      double d[6] = {tA, tB, tC, FA, FB, FC};
+      log.emit(MLQ_LOG_LINEMIN, iter, bi, mode, d);
+    };
+    BrentLog blog = [&](int j, double X, double FX) {
+      double d[2] = {X, FX};
+      log.emit(MLQ_LOG_BRENT, iter, j, 0, d);
+    };
+    msg(iter, 20);
+    BracketResult br = mnbrak(J, 0.0, tau0, llog);                 // :269
+    if (br.flag != 0) {
+      if (br.flag == 1) msg(iter, 22);
+      log.emit(MLQ_LOG_ERR, iter, br.flag, 0, nullptr);
+      double d[3] = {tau1, beta, J0};
+      log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
+      while_flag = 1;
+      stop = br.flag == 1 ? 2 : 3;
+      break;
+    }
+    msg(iter, 21);
+    msg(iter, 30);
+    spec.clear();
+    BrentResult bres = brent(J, br.tA, br.tC, blog);               // :297
+    tau1 = bres.X;
+    if (tau1 > br.tC) {
+      if (tau1 > 2.0 * br.tC) msg(iter, 32);
+    } else {
+      msg(iter, 31);
+    }
+    if (tau1 > TAU_MAX) {                                          // :310-318
+      msg(iter, 32);
+      while_flag = 1;
+      tau1 = 0.0;
+      stop = 4;
+    } else if (tau1 > std::sqrt(TAU_MAX)) {
+      msg(iter, 41);
+    }
+    // J1 = R(rescale(u + tau1 d1)) = -J_LineMin(tau1)                 (:349)
+    J1 = -F.J_value(tau1);
+    F.transport_and_update(tau1);
+    const double deltaJ = (J1 - J0) / std::fabs(J0);
+    if (deltaJ < -MACH_EPSILON) {                                  // :351-365
+      double d[3] = {tau1, beta, J0};
+      log.emit(MLQ_LOG_ITER, iter, iter + 1, 0, d);
+      msg(iter, 0);
+      while_flag = 1;
+      stop = 5;
+      break;
+    }
+    {
+      double d[3] = {tau1, beta, J1};
+      log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
+    }
+    J0 = J1;
+    tau0 = tau1;
+    for (int i = nroll - 1; i > 0; --i) rel[i] = rel[i - 1];
+    rel[0] = deltaJ;
+    double sr = 0.0;
+    for (double x : rel) sr += x;
+    avg = sr / (double)(iter - start);
+  }
+  F.download_u(e.fphys(u));
+  const bool ok = (iter < cfg.max_iter) && while_flag < 1;
+  if (stop == 0 && !ok) stop = 1;
+  msg(iter, 1);
+  res.J1 = J1;
+  res.iters = iter;
+  res.successful = ok ? 1 : 0;
+  res.while_flag = while_flag;
+  res.stop_reason = stop;
+  res.n_J = (int)F.nJ_;
+  res.n_J_computed = (int)F.nJc_;
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace mlq
+
+namespace mlq {
+
+template <typename T>
+void Engine<T>::rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) {
+  if ((cfg.flags & MLQ_RCG_REFERENCE) || !fast_path_ok()) rcg_solve_ref(u, cfg, res, log);
+  else rcg_solve_fast(*this, u, cfg, res, log);
+}
+
+// J_LineMin at a batch of step sizes for phi = u + tau d: one coefficient
+// precompute, then the batched evaluation.
+template <typename T>
+void Engine<T>::eval_J_batch(int u, int d, const double* taus, int ntau, double* J) {
+  if (!fast_path_ok()) { eval_J_batch_ref(u, d, taus, ntau, J); return; }
+  FastRcg<T> F(*this);
+  F.init_state(fphys(u));
+  F.set_direction_phys(fphys(d));
+  F.precompute_ls();
+  F.eval_batch(taus, ntau, J);
+}
+
+}  // namespace mlq

@@ -1,0 +1,327 @@
+// Kernels of the resident-state RCG iteration: the products that make the
+// line-search quantities quadratic polynomials in the step size, their
+// global polynomial coefficients, and spectral reductions over the modes the
+// 2/3 mask keeps.
+#pragma once
+#include "mlq_pointwise.hpp"
+
+namespace mlq {
+
+constexpr int RED_MAXK_BIG = 48;
+
+// Products of u, d and their gradients (G[i*3+j] = d_j u_i, Gd likewise)
+// whose dealiased transforms are the tau-coefficients of the line search:
+//   s(tau)   = D(phi.phi)                     -> S1' = 2 u.d, S2' = d.d
+//   w_j(tau) = D(sum_i phi_i d_j phi_i)       -> b'_j, e'_j
+//   X(tau)   = sum_ij d_i phi_j d_j phi_i     -> X1, X2 (pressure source)
+//   Q(tau)   = |grad phi|^2                   -> Q1, Q2
+template <typename T> struct F_ls_products {
+  const T* u; const T* d; const T* G; const T* Gd; long long nr;
+  T* S1; T* S2; T* b; T* e; T* X1; T* X2; T* Q1; T* Q2;   // b, e: 3 comps (stride nr)
+  __device__ void operator()(long long i) const {
+    T uu[3], dd[3], g[9], h[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i]; }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i]; }
+    S1[i] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
+    S2[i] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T bj = 0, ej = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        bj += uu[a] * h[a * 3 + j] + dd[a] * g[a * 3 + j];
+        ej += dd[a] * h[a * 3 + j];
+      }
+      b[j * nr + i] = bj;
+      e[j * nr + i] = ej;
+    }
+    T x1 = 0, x2 = 0, q1 = 0, q2 = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        x1 += g[c * 3 + a] * h[a * 3 + c] + h[c * 3 + a] * g[a * 3 + c];
+        x2 += h[c * 3 + a] * h[a * 3 + c];
+        q1 += g[a * 3 + c] * h[a * 3 + c];
+        q2 += h[a * 3 + c] * h[a * 3 + c];
+      }
+    X1[i] = x1;
+    X2[i] = x2;
+    Q1[i] = (T)2 * q1;
+    Q2[i] = q2;
+  }
+};
+
+// q = 4 polynomial coefficients (11 global sums):
+//   sum_j w_j(tau)^2 with w = a + tau b + tau^2 e          -> c0..c4
+//   p(tau) * sum_j phi_j w_j, p = P0 + tau P1 + tau^2 P2   -> r0..r5
+template <typename T> struct R_ls_poly {
+  const T* a; const T* b; const T* e; const T* u; const T* d;
+  const T* P0; const T* P1; const T* P2; long long nr;
+  __device__ void operator()(long long i, double (&acc)[11]) const {
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double aj = a[j * nr + i], bj = b[j * nr + i], ej = e[j * nr + i];
+      const double uj = u[j * nr + i], dj = d[j * nr + i];
+      c0 += aj * aj;
+      c1 += 2.0 * aj * bj;
+      c2 += bj * bj + 2.0 * aj * ej;
+      c3 += 2.0 * bj * ej;
+      c4 += ej * ej;
+      v0 += uj * aj;
+      v1 += uj * bj + dj * aj;
+      v2 += uj * ej + dj * bj;
+      v3 += dj * ej;
+    }
+    const double p0 = P0[i], p1 = P1[i], p2 = P2[i];
+    acc[0] += c0; acc[1] += c1; acc[2] += c2; acc[3] += c3; acc[4] += c4;
+    acc[5] += p0 * v0;
+    acc[6] += p0 * v1 + p1 * v0;
+    acc[7] += p0 * v2 + p1 * v1 + p2 * v0;
+    acc[8] += p0 * v3 + p1 * v2 + p2 * v1;
+    acc[9] += p1 * v3 + p2 * v2;
+    acc[10] += p2 * v3;
+  }
+};
+
+// g_b = sqrt(max(S0 + tau_b S1 + tau_b^2 S2, 0)) and gg_b = g_b * g_b for a
+// batch of step sizes (calc_uk, function_ops.f90:2332-2339, at phi = u + tau d)
+template <typename T> struct F_ls_g {
+  const T* S0; const T* S1; const T* S2;
+  T* g[MAXB]; T* gg[MAXB]; double tau[MAXB]; int nb;
+  __device__ void operator()(long long i) const {
+    const T s0 = S0[i], s1 = S1[i], s2 = S2[i];
+    for (int b = 0; b < nb; ++b) {
+      const T t = (T)tau[b];
+      const T s = s0 + t * (s1 + t * s2);
+      const T gv = s <= (T)0 ? (T)0 : sqrt(s);
+      if (g[b]) g[b][i] = gv;
+      gg[b][i] = gv * gv;
+    }
+  }
+};
+
+// o = sc * (x0 + t x1 + t^2 x2)
+template <typename T> struct F_quad {
+  const T* x0; const T* x1; const T* x2; T* o; T t, sc;
+  __device__ void operator()(long long i) const { o[i] = sc * (x0[i] + t * (x1[i] + t * x2[i])); }
+};
+template <typename T> struct S_quad {
+  const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; double t, sc;
+  __device__ void operator()(long long i) const {
+    const c2<T> a = x0[i], b = x1[i], c = x2[i];
+    o[i] = cmk<T>((T)(sc * (a.x + t * (b.x + t * c.x))), (T)(sc * (a.y + t * (b.y + t * c.y))));
+  }
+};
+
+// Y_i = q2 * aux2 * u_i - pc * A * grad_i p + 2 pc * sum_j d_i u_j d_j phi + vnu * A * Lap u_i
+// (the terms of GradL2ForLq that are dealiased before the final sum,
+// function_ops.f90:1140-1143, 1165-1169, 1178-1190)
+template <typename T> struct F_grad_terms {
+  const T* aux2; const T* u; const T* A; const T* gp; const T* G; const T* gphi; const T* lap;
+  long long nr; T q2, pc, vnu; T* Y;
+  __device__ void operator()(long long i) const {
+    const T a2 = aux2[i], Ai = A ? A[i] : (T)1;
+    const T f0 = gphi[i], f1 = gphi[nr + i], f2 = gphi[2 * nr + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T t5 = G[(0 * 3 + c) * nr + i] * f0 + G[(1 * 3 + c) * nr + i] * f1 +
+                   G[(2 * 3 + c) * nr + i] * f2;
+      Y[c * nr + i] = q2 * a2 * u[c * nr + i] - pc * Ai * gp[c * nr + i] + (T)2 * pc * t5 +
+                      vnu * Ai * lap[c * nr + i];
+    }
+  }
+};
+
+// Hilbert weight then Leray projection on three half spectra, in place
+// (HilbertGradient :3611 followed by divAvg_free :2879-2909), optional scale.
+template <typename T> struct S_hilbert_leray {
+  c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px; double lam, e2s, scale;
+  __device__ void operator()(long long i) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
+    const double ksq = K1 * K1 + K2 * K2 + K3 * K3;
+    if (ksq <= 2.0e-16) {
+      u0[i] = cmk<T>(0, 0); u1[i] = cmk<T>(0, 0); u2[i] = cmk<T>(0, 0);
+      return;
+    }
+    const double w = scale / (1.0 + pow(lam * sqrt(ksq), e2s));
+    const double e1 = mx == h ? 0.0 : K1, e2 = my == h ? 0.0 : K2, e3 = mz == h ? 0.0 : K3;
+    const c2<T> a = u0[i], b = u1[i], c = u2[i];
+    const double ar = w * a.x, ai = w * a.y, br = w * b.x, bi = w * b.y, cr = w * c.x, ci = w * c.y;
+    const double dr = (e1 * ar + e2 * br + e3 * cr) / ksq;
+    const double di = (e1 * ai + e2 * bi + e3 * ci) / ksq;
+    u0[i] = cmk<T>((T)(ar - e1 * dr), (T)(ai - e1 * di));
+    u1[i] = cmk<T>((T)(br - e2 * dr), (T)(bi - e2 * di));
+    u2[i] = cmk<T>((T)(cr - e3 * dr), (T)(ci - e3 * di));
+  }
+};
+
+// o = a*x + b*y on three spectra at once (x, y, o: 3 comps, stride ns)
+template <typename T> struct S_axpby3 {
+  const c2<T>* x; const c2<T>* y; c2<T>* o; long long ns; double a, b;
+  __device__ void operator()(long long i) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> p = x[c * ns + i], q = y ? y[c * ns + i] : cmk<T>(0, 0);
+      o[c * ns + i] = cmk<T>((T)(a * p.x + b * q.x), (T)(a * p.y + b * q.y));
+    }
+  }
+};
+// V_hat = Vm + c * (-|K|^2) W   (aux7 of GradL2ForLq added in spectral space)
+template <typename T> struct S_add_lap {
+  c2<T>* V; const c2<T>* W; long long ns; int n, px; double c;
+  __device__ void operator()(long long i) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double ksq = TWO_PI * TWO_PI * ((double)kx * kx + (double)my * my + (double)mz * mz);
+    const double f = -c * ksq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const c2<T> v = V[k * ns + i], w = W[k * ns + i];
+      V[k * ns + i] = cmk<T>((T)(v.x + f * w.x), (T)(v.y + f * w.y));
+    }
+  }
+};
+
+// Weighted Parseval sums over three-component spectra:
+//   sum_c sum_k m(kx) w(k)^2 Re(a_c conj b_c), kind as R_specdot.
+template <typename T> struct R_specdot3 {
+  const c2<T>* a; const c2<T>* b; long long ns; int n, px, kind; double lam, s;
+  __device__ void operator()(long long i, double (&acc)[1]) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    double w = 1.0;
+    if (kind != 0) {
+      const double K = TWO_PI * sqrt((double)kx * kx + (double)my * my + (double)mz * mz);
+      w = kind == 1 ? 1.0 + pow(lam * K, s) : pow(1.0 + K, s);
+    }
+    const double m = (kx == 0 || kx == h) ? 1.0 : 2.0;
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> x = a[c * ns + i], y = b[c * ns + i];
+      t += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+    }
+    acc[0] += m * w * w * t;
+  }
+};
+// sum_k |sum_j i K'_j g_j|^2 with half-spectrum weights (||div g||^2 by Parseval)
+template <typename T> struct R_divnorm {
+  const c2<T>* g; long long ns; int n, px;
+  __device__ void operator()(long long i, double (&acc)[1]) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double e1 = mx == h ? 0.0 : TWO_PI * mx, e2 = my == h ? 0.0 : TWO_PI * my,
+                 e3 = mz == h ? 0.0 : TWO_PI * mz;
+    const c2<T> a = g[i], b = g[ns + i], c = g[2 * ns + i];
+    const double re = e1 * a.x + e2 * b.x + e3 * c.x, im = e1 * a.y + e2 * b.y + e3 * c.y;
+    const double m = (kx == 0 || kx == h) ? 1.0 : 2.0;
+    acc[0] += m * (re * re + im * im);
+  }
+};
+
+// Line-search spectral sums over the modes kept by the 2/3 mask, for a batch
+// of step sizes: for each b,
+//   acc[2b]   = sum_k m |A_b|^2                              (||phi||_q^q)
+//   acc[2b+1] = sum_k m Re(conj(A_b) (Q0 + t Q1 + t^2 Q2))   (R1 numerator)
+// The index space is the kept cube kx <= M, |ky| <= M, |kz| <= M.
+template <typename T> struct R_ls_spec {
+  const c2<T>* A[MAXB]; const c2<T>* Q0; const c2<T>* Q1; const c2<T>* Q2;
+  double tau[MAXB]; int nb, n, px, M;
+  __device__ void operator()(long long i, double (&acc)[2 * MAXB]) const {
+    const int W = 2 * M + 1;
+    const int kx = (int)(i % (M + 1));
+    const long long r = i / (M + 1);
+    const int jy = (int)(r % W), jz = (int)(r / W);
+    const int my = jy - M, mz = jz - M;
+    const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
+    const long long off = ((long long)kz * n + ky) * px + kx;
+    const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
+    const c2<T> q0 = Q0[off], q1 = Q1[off], q2 = Q2[off];
+    for (int b = 0; b < nb; ++b) {
+      const c2<T> a = A[b][off];
+      const double t = tau[b];
+      const double qr = q0.x + t * (q1.x + t * q2.x), qi = q0.y + t * (q1.y + t * q2.y);
+      acc[2 * b] += m * ((double)a.x * a.x + (double)a.y * a.y);
+      acc[2 * b + 1] += m * ((double)a.x * qr + (double)a.y * qi);
+    }
+  }
+};
+
+// Generic per-step sums for exponents other than 4 (physical space):
+//   acc[0] = sum A (Q0 + t Q1 + t^2 Q2)
+//   acc[1] = sum Bw * sum_j w_j^2,  w = a + t b + t^2 e   (Bw = 1 when null)
+//   acc[2] = sum Pd * sum_j phi_j w_j,  phi = u + t d
+//   acc[3] = sum L^2                                      (||phi||_q^q)
+template <typename T> struct R_ls_generic {
+  const T* A; const T* Bw; const T* Pd; const T* Lh;
+  const T* Q0; const T* Q1; const T* Q2; const T* a; const T* b; const T* e;
+  const T* u; const T* d; long long nr; double t;
+  __device__ void operator()(long long i, double (&acc)[4]) const {
+    const double q = (double)Q0[i] + t * ((double)Q1[i] + t * (double)Q2[i]);
+    acc[0] += (double)A[i] * q;
+    double w2 = 0.0, pw = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double wj = (double)a[j * nr + i] + t * ((double)b[j * nr + i] + t * (double)e[j * nr + i]);
+      w2 += wj * wj;
+      pw += ((double)u[j * nr + i] + t * (double)d[j * nr + i]) * wj;
+    }
+    acc[1] += (Bw ? (double)Bw[i] : 1.0) * w2;
+    acc[2] += (double)Pd[i] * pw;
+    acc[3] += (double)Lh[i] * (double)Lh[i];
+  }
+};
+
+// p(t) * Bq, for the dealiased pressure term at exponents other than 4
+template <typename T> struct F_pB {
+  const T* P0; const T* P1; const T* P2; const T* Bq; T* o; T t;
+  __device__ void operator()(long long i) const {
+    o[i] = (P0[i] + t * (P1[i] + t * P2[i])) * (Bq ? Bq[i] : (T)1);
+  }
+};
+
+// sum_c sum_k m Re(conj(W_c) xi_c) over three-component spectra restricted to
+// the kept cube (W masked), for two right-hand sides at once.
+template <typename T> struct R_kept_dot2 {
+  const c2<T>* W; const c2<T>* x; const c2<T>* y; long long ns; int n, px, M;
+  __device__ void operator()(long long i, double (&acc)[2]) const {
+    const int Wd = 2 * M + 1;
+    const int kx = (int)(i % (M + 1));
+    const long long r = i / (M + 1);
+    const int jy = (int)(r % Wd), jz = (int)(r / Wd);
+    const int my = jy - M, mz = jz - M;
+    const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
+    const long long off = ((long long)kz * n + ky) * px + kx;
+    const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> w = W[c * ns + off], a = x[c * ns + off], b = y[c * ns + off];
+      acc[0] += m * ((double)w.x * a.x + (double)w.y * a.y);
+      acc[1] += m * ((double)w.x * b.x + (double)w.y * b.y);
+    }
+  }
+};
+
+}  // namespace mlq

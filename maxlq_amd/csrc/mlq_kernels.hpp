@@ -16,6 +16,30 @@ namespace mlq {
 constexpr int MAXB = 12;
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
+// Row pitch of a half spectrum in elements: n/2+1 rounded up so that every
+// row starts on a 128-byte boundary (16 complex floats, 8 complex doubles).
+// The padding entries kx in [n/2+1, pitch) are never read or written.
+template <typename T> constexpr int spec_pitch(int n) {
+  constexpr int a = 128 / (int)sizeof(c2<T>);
+  return ((n / 2 + 1) + a - 1) / a * a;
+}
+
+// Decode a flat half-spectrum index (pitch pp) into mode indices; false on
+// padding.  m_y, m_z in (-n/2, n/2] (Nyquist counted +n/2).
+struct KIdx { int kx, ky, kz, mx, my, mz; };
+__device__ __forceinline__ bool kdecode(long long i, int n, int pp, KIdx& k) {
+  const long long row = i / pp;
+  k.kx = (int)(i - row * pp);
+  if (k.kx > (n >> 1)) return false;
+  k.ky = (int)(row % n);
+  k.kz = (int)(row / n);
+  const int h = n >> 1;
+  k.mx = k.kx;
+  k.my = k.ky <= h ? k.ky : k.ky - n;
+  k.mz = k.kz <= h ? k.kz : k.kz - n;
+  return true;
+}
+
 template <typename T> struct XArgs {
   const T* rsrc[MAXB];
   T* rdst[MAXB];
@@ -77,7 +101,7 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
 // ---------------------------------------------------------------------------
 template <typename T, int N, int NF>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
-  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
   __shared__ c2<T> sm[NF * LP];
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
@@ -116,7 +140,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
 // inverse FFT, unpack, scale.
 template <typename T, int N, int NF>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
-  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
   __shared__ c2<T> sm[NF * LP];
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
@@ -154,19 +178,22 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 // spectral factor (prologue), writing scratch.  MODE 2: forward z pass
 // reading scratch, writing factor * result to dst (epilogue).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int C, int DIR, int MODE>
-__global__ __launch_bounds__(C* FftShape<N>::Tf) void k_strided(SArgs<T> a, int ncols,
-                                                                 long long estride,
-                                                                 long long pstride) {
-  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = N / 2 + 1;
+template <typename T, int N, int E, int C, int DIR, int MODE>
+__global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, long long estride,
+                                                       long long pstride) {
+  constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   __shared__ c2<T> sm[N * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int col = blockIdx.x * C + c;
-  const bool valid = col < ncols;
+  bool valid = col < ncols;
   const int plane = blockIdx.y;
   c2<T> v[E];
   int kx = 0, ky = 0;
-  if (MODE != 0) { ky = col / PX; kx = col - ky * PX; }
+  if (MODE != 0) {
+    ky = col / PP;
+    kx = col - ky * PP;
+    valid = valid && kx <= N / 2;   // padding columns carry no data
+  }
   if (MODE == 1) {
     const int bb = plane;
     const SpecOp op = a.op[bb];
@@ -218,7 +245,7 @@ __global__ __launch_bounds__(C* FftShape<N>::Tf) void k_strided(SArgs<T> a, int 
 // ---------------------------------------------------------------------------
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
-constexpr int RED_MAXK = 4;
+constexpr int RED_MAXK = 48;   // widest reduction: 2 sums per step size x 12 (+ spare)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

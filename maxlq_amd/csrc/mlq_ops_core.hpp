@@ -235,7 +235,7 @@ template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
   // grad of Delta^-1 div(|u|^(q-2)u)                                   :1155-1158
   {
     ST dv = stmp();
-    smap(S_div<T>{Wp[0], Wp[1], Wp[2], dv, n_, px_});
+    smap(S_div<T>{Wp[0], Wp[1], Wp[2], dv, n_, pp_});
     InvSrc is[6];
     SpecOp ops[6];
     T* dst[6];
@@ -290,7 +290,7 @@ template <typename T> void Engine<T>::divergence(int u_vec, int out_sc) {
   C* uhp[3] = {uh[0], uh[1], uh[2]};
   spec3(fphys(u_vec), uhp);
   ST dv = stmp();
-  smap(S_div<T>{uhp[0], uhp[1], uhp[2], dv, n_, px_});
+  smap(S_div<T>{uhp[0], uhp[1], uhp[2], dv, n_, pp_});
   inv1(dv, fphys(out_sc));
 }
 
@@ -327,7 +327,7 @@ template <typename T> void Engine<T>::divAvg_free(int f_vec) {
   ST uh[3] = {stmp(), stmp(), stmp()};
   C* uhp[3] = {uh[0], uh[1], uh[2]};
   spec3(fphys(f_vec), uhp);
-  smap(S_leray<T>{uhp[0], uhp[1], uhp[2], n_, px_});
+  smap(S_leray<T>{uhp[0], uhp[1], uhp[2], n_, pp_});
   InvSrc is[3];
   SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
   T* dst[3];
@@ -476,7 +476,7 @@ template <typename T> double Engine<T>::eval_J(int f_vec, int system) {
 }
 
 template <typename T>
-void Engine<T>::eval_J_batch(int u, int d, const double* taus, int ntau, double* J) {
+void Engine<T>::eval_J_batch_ref(int u, int d, const double* taus, int ntau, double* J) {
   RT pb = rtmp(3);
   T* phi = pb.get();
   for (int i = 0; i < ntau; ++i) {
@@ -495,7 +495,7 @@ template <typename T> void Engine<T>::seeded(int f_vec, unsigned long long seed,
   ST uh[3] = {stmp(), stmp(), stmp()};
   C* uhp[3] = {uh[0], uh[1], uh[2]};
   spec3(fphys(f_vec), uhp);
-  smap(S_envelope_leray<T>{uhp[0], uhp[1], uhp[2], n_, px_, kind, a});
+  smap(S_envelope_leray<T>{uhp[0], uhp[1], uhp[2], n_, pp_, kind, a});
   InvSrc is[3];
   SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
   T* dst[3];
@@ -564,18 +564,22 @@ template <typename T> void Engine<T>::fft_r2c_host(const double* in, double* out
   for (long long i = 0; i < nr_; ++i) tr[i] = (T)in[i];
   MLQ_HIP(hipMemcpyAsync(r.get(), tr.data(), sizeof(T) * nr_, hipMemcpyHostToDevice, st_));
   fwd1(r, s);
-  std::vector<C> ts(ns_);
-  MLQ_HIP(hipMemcpyAsync(ts.data(), s.get(), sizeof(C) * ns_, hipMemcpyDeviceToHost, st_));
+  const long long nh = (long long)n_ * n_ * px_;   // host layout: rows of n/2+1
+  std::vector<C> ts(nh);
+  MLQ_HIP(hipMemcpy2DAsync(ts.data(), sizeof(C) * px_, s.get(), sizeof(C) * pp_, sizeof(C) * px_,
+                           (size_t)n_ * n_, hipMemcpyDeviceToHost, st_));
   sync();
-  for (long long i = 0; i < ns_; ++i) { out[2 * i] = ts[i].x; out[2 * i + 1] = ts[i].y; }
+  for (long long i = 0; i < nh; ++i) { out[2 * i] = ts[i].x; out[2 * i + 1] = ts[i].y; }
 }
 
 template <typename T> void Engine<T>::fft_c2r_host(const double* in, double* out) {
   RT r = rtmp();
   ST s = stmp();
-  std::vector<C> ts(ns_);
-  for (long long i = 0; i < ns_; ++i) { ts[i].x = (T)in[2 * i]; ts[i].y = (T)in[2 * i + 1]; }
-  MLQ_HIP(hipMemcpyAsync(s.get(), ts.data(), sizeof(C) * ns_, hipMemcpyHostToDevice, st_));
+  const long long nh = (long long)n_ * n_ * px_;   // host layout: rows of n/2+1
+  std::vector<C> ts(nh);
+  for (long long i = 0; i < nh; ++i) { ts[i].x = (T)in[2 * i]; ts[i].y = (T)in[2 * i + 1]; }
+  MLQ_HIP(hipMemcpy2DAsync(s.get(), sizeof(C) * pp_, ts.data(), sizeof(C) * px_, sizeof(C) * px_,
+                           (size_t)n_ * n_, hipMemcpyHostToDevice, st_));
   inv1(s, r);
   std::vector<T> tr(nr_);
   MLQ_HIP(hipMemcpyAsync(tr.data(), r.get(), sizeof(T) * nr_, hipMemcpyDeviceToHost, st_));

@@ -188,7 +188,8 @@ template <typename T> struct R_specdot {
   const c2<T>* a; const c2<T>* b; int n, px, kind; double lam, s;
   __device__ void operator()(long long i, double (&acc)[1]) const {
     const long long row = i / px;
-    const int kx = (int)(i - row * px);
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
     const int ky = (int)(row % n), kz = (int)(row / n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
@@ -211,7 +212,8 @@ template <typename T> struct S_leray {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px;
   __device__ void operator()(long long i) const {
     const long long row = i / px;
-    const int kx = (int)(i - row * px);
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
     const int ky = (int)(row % n), kz = (int)(row / n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
@@ -235,7 +237,8 @@ template <typename T> struct S_div {
   const c2<T>* w0; const c2<T>* w1; const c2<T>* w2; c2<T>* o; int n, px;
   __device__ void operator()(long long i) const {
     const long long row = i / px;
-    const int kx = (int)(i - row * px);
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
     const int ky = (int)(row % n), kz = (int)(row / n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
@@ -260,7 +263,8 @@ template <typename T> struct S_envelope_leray {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px, kind; double a;
   __device__ void operator()(long long i) const {
     const long long row = i / px;
-    const int kx = (int)(i - row * px);
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
     const int ky = (int)(row % n), kz = (int)(row / n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;

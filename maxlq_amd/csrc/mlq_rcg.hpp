@@ -12,7 +12,8 @@
 namespace mlq {
 
 template <typename T>
-void Engine<T>::rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) {
+void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res,
+                              const LogSink& log) {
   auto t0 = std::chrono::steady_clock::now();
   const double q = p_.q, B = p_.B;
   const long long n3 = nr_ * 3;

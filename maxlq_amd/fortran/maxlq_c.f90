@@ -1,0 +1,258 @@
+! iso_c_binding interface to libmaxlq.so (include/maxlq.h).
+!
+! Host arrays are passed by reference in the reference layout: real(c_double)
+! fields dimensioned (1:n1,1:n2,1:n3[,1:3]) and complex(c_double_complex)
+! spectra, exactly as the reference's fftwfunction and function_ops routines
+! receive them.  Every function returns an MLQ_* status (0 = MLQ_OK).
+module maxlq_c
+   use, intrinsic :: iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: MLQ_OK = 0, MLQ_ENAN = 1, MLQ_EBADKIND = 2, MLQ_EHIP = 3, &
+                                MLQ_EOOM = 4, MLQ_EINVAL = 5, MLQ_EUNSUP = 6
+   integer(c_int), parameter :: MLQ_FP64 = 0, MLQ_MIXED = 1
+   integer(c_int), parameter :: MLQ_IP_L2 = 0, MLQ_IP_HL = 1, MLQ_IP_H = 2, MLQ_IP_OLD_H = 3, &
+                                MLQ_IP_OLD_HL = 4
+   integer(c_int), parameter :: MLQ_J_MAXDLQDT = 0, MLQ_J_LINEMIN = 1, MLQ_J_LQNORM = 2
+   integer(c_int), parameter :: MLQ_LOG_ITER = 1, MLQ_LOG_LINEMIN = 2, MLQ_LOG_BRENT = 3, &
+                                MLQ_LOG_MSG = 4, MLQ_LOG_ERR = 5
+   integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1
+
+   type, bind(C) :: mlq_params
+      real(c_double) :: q, B, visc, lambda1, viscCoefficient, pressureCoefficient
+      integer(c_int) :: toDealias, use_e_u, dealiase_if_mult_by_e_u, precision
+   end type mlq_params
+
+   type, bind(C) :: mlq_rcg_cfg
+      integer(c_int) :: max_iter
+      real(c_double) :: optim_tol
+      integer(c_int) :: start_iter, batch_linesearch, reset_every, rolling, flags
+   end type mlq_rcg_cfg
+
+   type, bind(C) :: mlq_rcg_result
+      real(c_double) :: J1
+      integer(c_int) :: iters, successful, while_flag, stop_reason, n_J, n_J_computed
+      real(c_double) :: seconds
+   end type mlq_rcg_result
+
+   interface
+      subroutine mlq_default_params(p, q) bind(C, name="mlq_default_params")
+         import :: mlq_params, c_double
+         type(mlq_params), intent(out) :: p
+         real(c_double), value :: q
+      end subroutine
+      integer(c_int) function mlq_init(ctx, n1, n2, n3, p) bind(C, name="mlq_init")
+         import :: c_ptr, c_int, mlq_params
+         type(c_ptr), intent(out) :: ctx
+         integer(c_int), value :: n1, n2, n3
+         type(mlq_params), intent(in) :: p
+      end function
+      integer(c_int) function mlq_finalize(ctx) bind(C, name="mlq_finalize")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mlq_set_params(ctx, p) bind(C, name="mlq_set_params")
+         import :: c_ptr, c_int, mlq_params
+         type(c_ptr), value :: ctx
+         type(mlq_params), intent(in) :: p
+      end function
+      integer(c_int) function mlq_local_size_3d(ctx, local_n, local_k_offset) &
+            bind(C, name="mlq_local_size_3d")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: local_n, local_k_offset
+      end function
+      ! fftwfunction
+      integer(c_int) function mlq_fftfwd(ctx, u, fu) bind(C, name="mlq_fftfwd")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: u(*)
+         complex(c_double_complex), intent(out) :: fu(*)
+      end function
+      integer(c_int) function mlq_fftbwd(ctx, fu, u) bind(C, name="mlq_fftbwd")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: fu(*)
+         complex(c_double_complex), intent(out) :: u(*)
+      end function
+      integer(c_int) function mlq_fftfwdV(ctx, v, fv) bind(C, name="mlq_fftfwdV")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: v(*)
+         complex(c_double_complex), intent(out) :: fv(*)
+      end function
+      integer(c_int) function mlq_fftbwdV(ctx, fv, v) bind(C, name="mlq_fftbwdV")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: fv(*)
+         complex(c_double_complex), intent(out) :: v(*)
+      end function
+      integer(c_int) function mlq_fftfwdr(ctx, u, fu) bind(C, name="mlq_fftfwdr")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         complex(c_double_complex), intent(out) :: fu(*)
+      end function
+      integer(c_int) function mlq_fftbwdr(ctx, fu, u) bind(C, name="mlq_fftbwdr")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: fu(*)
+         real(c_double), intent(out) :: u(*)
+      end function
+      ! function_ops field routines
+      integer(c_int) function mlq_gradient(ctx, u, g) bind(C, name="mlq_gradient")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: g(*)
+      end function
+      integer(c_int) function mlq_divergence(ctx, u, d) bind(C, name="mlq_divergence")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: d(*)
+      end function
+      integer(c_int) function mlq_laplacian(ctx, f) bind(C, name="mlq_laplacian")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: f(*)
+      end function
+      integer(c_int) function mlq_solve_poisson(ctx, f, l, phi) bind(C, name="mlq_solve_poisson")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: f(*)
+         real(c_double), value :: l
+         real(c_double), intent(out) :: phi(*)
+      end function
+      integer(c_int) function mlq_dealias_scalar(ctx, g, order) bind(C, name="mlq_dealias_scalar")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: g(*)
+         real(c_double), value :: order
+      end function
+      integer(c_int) function mlq_divAvg_free(ctx, f) bind(C, name="mlq_divAvg_free")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: f(*)
+      end function
+      integer(c_int) function mlq_HilbertGradient(ctx, g, order) bind(C, name="mlq_HilbertGradient")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: g(*)
+         real(c_double), value :: order
+      end function
+      integer(c_int) function mlq_calc0thFourierModes(ctx, v, out) &
+            bind(C, name="mlq_calc0thFourierModes")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: v(*)
+         real(c_double), intent(out) :: out(3)
+      end function
+      integer(c_int) function mlq_calc_uk(ctx, u, k, uk) bind(C, name="mlq_calc_uk")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), value :: k
+         real(c_double), intent(out) :: uk(*)
+      end function
+      integer(c_int) function mlq_calc_global_Lq_norm(ctx, u, nrm) &
+            bind(C, name="mlq_calc_global_Lq_norm")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: nrm
+      end function
+      integer(c_int) function mlq_rescaleLqNorm(ctx, u, B) bind(C, name="mlq_rescaleLqNorm")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: u(*)
+         real(c_double), value :: B
+      end function
+      integer(c_int) function mlq_calcConstraintDerivativeL2(ctx, u, o) &
+            bind(C, name="mlq_calcConstraintDerivativeL2")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: o(*)
+      end function
+      integer(c_int) function mlq_vectorTransport(ctx, u, q, B, eta, xi, o) &
+            bind(C, name="mlq_vectorTransport")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*), eta(*), xi(*)
+         real(c_double), value :: q, B
+         real(c_double), intent(out) :: o(*)
+      end function
+      integer(c_int) function mlq_global_inner_product(ctx, f, g, kind, o) &
+            bind(C, name="mlq_global_inner_product")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: f(*), g(*)
+         integer(c_int), value :: kind
+         real(c_double), intent(out) :: o
+      end function
+      integer(c_int) function mlq_global_summed_field_inner_product(ctx, f, g, kind, o) &
+            bind(C, name="mlq_global_summed_field_inner_product")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: f(*), g(*)
+         integer(c_int), value :: kind
+         real(c_double), intent(out) :: o
+      end function
+      integer(c_int) function mlq_calc_global_dLqdt_inclParts(ctx, u, q, o) &
+            bind(C, name="mlq_calc_global_dLqdt_inclParts")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), value :: q
+         real(c_double), intent(out) :: o(3)
+      end function
+      integer(c_int) function mlq_GradL2ForLq(ctx, U, V) bind(C, name="mlq_GradL2ForLq")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: U(*)
+         real(c_double), intent(out) :: V(*)
+      end function
+      integer(c_int) function mlq_eval_J(ctx, f, sys, J) bind(C, name="mlq_eval_J")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: f(*)
+         integer(c_int), value :: sys
+         real(c_double), intent(out) :: J
+      end function
+      ! device-resident fields and the whole solve
+      integer(c_int) function mlq_field_alloc(ctx, ncomp, h) bind(C, name="mlq_field_alloc")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: ncomp
+         integer(c_int), intent(out) :: h
+      end function
+      integer(c_int) function mlq_field_free(ctx, h) bind(C, name="mlq_field_free")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: h
+      end function
+      integer(c_int) function mlq_upload(ctx, h, a) bind(C, name="mlq_upload")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: h
+         real(c_double), intent(in) :: a(*)
+      end function
+      integer(c_int) function mlq_download(ctx, h, a) bind(C, name="mlq_download")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: h
+         real(c_double), intent(out) :: a(*)
+      end function
+      integer(c_int) function mlq_rcg_solve(ctx, h, cfg, res, cb, user) bind(C, name="mlq_rcg_solve")
+         import :: c_ptr, c_int, c_funptr, mlq_rcg_cfg, mlq_rcg_result
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: h
+         type(mlq_rcg_cfg), intent(in) :: cfg
+         type(mlq_rcg_result), intent(out) :: res
+         type(c_funptr), value :: cb
+         type(c_ptr), value :: user
+      end function
+   end interface
+
+end module maxlq_c

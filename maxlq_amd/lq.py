@@ -290,6 +290,13 @@ class MaxLq:
         self._chk(self.lib.mlq_time_fft(self.ctx, batch, reps, _p(out)))
         return out
 
+    def autotune(self, batch=3):
+        """Time the strided-pass tile variants; returns (times[4][4] ms, choice[4])."""
+        t = np.zeros(16)
+        ch = (C.c_int * 4)()
+        self._chk(self.lib.mlq_autotune(self.ctx, batch, _p(t), ch))
+        return t.reshape(4, 4), list(ch)
+
     def time_copy(self, nfields=3, reps=5):
         r = C.c_double()
         self._chk(self.lib.mlq_time_copy(self.ctx, nfields, reps, C.byref(r)))

@@ -46,3 +46,34 @@ def test_missing_library_fails_loudly(tmp_path):
             capi.load(str(tmp_path / "nope.so"))
     finally:
         capi._lib = saved
+
+
+def test_fortran_interface_compiles(tmp_path):
+    """maxlq_c.f90 (the iso_c_binding block for include/maxlq.h) compiles."""
+    import shutil
+    import subprocess
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not available")
+    src = os.path.join(ROOT, "maxlq_amd", "fortran", "maxlq_c.f90")
+    r = subprocess.run([fc, "-c", src, "-J", str(tmp_path), "-o", str(tmp_path / "m.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_fortran_dropin_compiles_against_reference_globals(tmp_path):
+    """The drop-in fftwfunction / mlq_field_ops modules compile against the
+    reference's own global_variables module (only where the reference is
+    mounted; objects go to a temporary directory)."""
+    import subprocess
+    fc = "/opt/rocm/bin/amdflang"
+    gv = "/root/reference/global_variables.f90"
+    if not (os.path.exists(fc) and os.path.exists(gv)):
+        pytest.skip("amdflang or reference sources not available")
+    fdir = os.path.join(ROOT, "maxlq_amd", "fortran")
+    for src in (gv, os.path.join(fdir, "maxlq_c.f90"), os.path.join(fdir, "fftwfunction_gpu.f90"),
+                os.path.join(fdir, "mlq_field_ops.f90")):
+        r = subprocess.run([fc, "-c", src, "-J", str(tmp_path), "-o",
+                            str(tmp_path / (os.path.basename(src) + ".o"))],
+                           capture_output=True, text=True, cwd=str(tmp_path))
+        assert r.returncode == 0, (src, r.stderr)

@@ -200,3 +200,15 @@ def test_mixed_precision_R(q):
     a = g.calc_local_dLqdt_inclParts(u)
     b = orc.calc_local_dLqdt_inclParts(u)
     assert abs(a[0] - b[0]) < 1e-5 * abs(b[0])
+
+
+def test_fortran_caller_smoke():
+    """The Fortran program maxlq_amd/fortran/mlq_fsmoke (built by build())
+    calls the library through the iso_c_binding module."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "maxlq_amd", "fortran", "mlq_fsmoke")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr

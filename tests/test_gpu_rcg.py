@@ -1,4 +1,4 @@
-"""GPU RCG solve (maxdLqdt) against the oracle's trajectory."""
+"""GPU RCG solve (maxdLqdt) and batched line-search J against the oracle."""
 import numpy as np
 import pytest
 
@@ -8,30 +8,100 @@ from tests.fields import rel
 
 pytestmark = pytest.mark.gpu
 
+REF = 1  # MLQ_RCG_REFERENCE
 
-def _solve_both(n, q, B, iters, seed=1, batch=0, precision="fp64"):
+
+def _solve_gpu(u0, n, q, B, iters, flags=0, batch=0, precision="fp64"):
+    import ctypes as C
+    from maxlq_amd import capi
     from maxlq_amd.lq import MaxLq
-    u0 = seeded_field(n, seed, "gauss", 16.0, q=q, B=B)
-    orc = L.Oracle(L.Params(n=n, q=q, B=B, MAX_ITER=iters))
-    ref = L.maxdLqdt(orc, u0)
     g = MaxLq(n, q=q, B=B, precision=precision)
     h = g.field(3)
     g.upload(h, u0)
-    res, tr = g.maxdLqdt(h, max_iter=iters, batch_linesearch=batch)
-    return ref, res, tr, g.download(h)
+    tr = {"iter": [], "lm": [], "br": []}
+
+    def cb(user, kind, it, i0, i1, d):
+        if kind == capi.MLQ_LOG_ITER:
+            tr["iter"].append((i0, d[0], d[1], d[2]))
+        elif kind == capi.MLQ_LOG_LINEMIN:
+            tr["lm"].append((it, i0, i1) + tuple(d[k] for k in range(6)))
+        elif kind == capi.MLQ_LOG_BRENT:
+            tr["br"].append((it, i0, d[0], d[1]))
+
+    cfg = capi.RcgCfg(iters, 1.0e-5, 0, batch, 25, 10, flags)
+    res = capi.RcgResult()
+    g._chk(g.lib.mlq_rcg_solve(g.ctx, h, C.byref(cfg), C.byref(res), capi.LOG_CB(cb), None))
+    return res, tr, g.download(h)
 
 
-@pytest.mark.parametrize("q", [4.0, 6.0])
-def test_rcg_trajectory_fp64(q):
-    ref, res, tr, u = _solve_both(16, q, 1.0, 4)
+def _oracle_solve(u0, n, q, B, iters):
+    orc = L.Oracle(L.Params(n=n, q=q, B=B, MAX_ITER=iters))
+    brent_log = []
+    return L.maxdLqdt(orc, u0)
+
+
+@pytest.mark.parametrize("q,flags", [(4.0, 0), (4.0, REF), (6.0, 0), (9.0, 0)])
+def test_rcg_trajectory_fp64(q, flags):
+    n, B, iters = 16, 1.0, 4
+    u0 = seeded_field(n, 1, "gauss", 16.0, q=q, B=B)
+    ref = _oracle_solve(u0, n, q, B, iters)
+    res, tr, u = _solve_gpu(u0, n, q, B, iters, flags=flags)
     assert res.iters == ref.iters
-    # iteration-info records (iter, tau, beta, J) after the initial line
-    got = [r for r in tr.iteration_info[1:]]
+    got = tr["iter"][1:]
     assert len(got) == len(ref.records)
     for (it, tau, beta, J), r in zip(got, ref.records):
         assert it == r.iter
         assert abs(tau - r.tau) < 1e-8 * abs(r.tau)
-        assert abs(beta - r.beta) < 1e-8 * max(abs(r.beta), 1e-300) or (beta == 0 and r.beta == 0)
+        assert abs(beta - r.beta) <= 1e-8 * abs(r.beta)
         assert abs(J - r.J1) < 1e-10 * abs(r.J1)
     assert rel(u, ref.u) < 1e-8
     assert abs(res.J1 - ref.J1) < 1e-10 * abs(ref.J1)
+
+
+def test_rcg_fast_matches_reference_path_32():
+    """Fast (resident-state, tau-polynomial) vs reference-structured GPU solve,
+    including the mnbrak / brent records."""
+    n, q, B, iters = 32, 4.0, 1.0, 3
+    u0 = seeded_field(n, 2, "gauss", 16.0, q=q, B=B)
+    r1, t1, u1 = _solve_gpu(u0, n, q, B, iters, flags=REF)
+    r2, t2, u2 = _solve_gpu(u0, n, q, B, iters, flags=0, batch=3)
+    assert r1.iters == r2.iters
+    assert len(t1["br"]) == len(t2["br"]) and len(t1["lm"]) == len(t2["lm"])
+    for a, b in zip(t1["br"], t2["br"]):
+        assert a[:2] == b[:2]
+        assert abs(a[2] - b[2]) <= 1e-9 * max(abs(a[2]), 1e-300)
+        assert abs(a[3] - b[3]) <= 1e-10 * abs(a[3])
+    assert rel(u2, u1) < 1e-9
+    assert r2.n_J == r1.n_J
+    assert r2.n_J_computed <= r2.n_J + 2 * r2.iters
+
+
+@pytest.mark.parametrize("q", [4.0, 6.0, 9.0])
+def test_eval_J_batch_fast_vs_oracle(q):
+    from maxlq_amd.lq import MaxLq
+    n, B = 16, 2.0
+    orc = L.Oracle(L.Params(n=n, q=q, B=B))
+    u = seeded_field(n, 21, "gauss", 16.0, q=q, B=B)
+    d = seeded_field(n, 22, "gauss", 16.0, q=q, B=B)
+    g = MaxLq(n, q=q, B=B)
+    hu, hd = g.field(3), g.field(3)
+    g.upload(hu, u)
+    g.upload(hd, d)
+    taus = 10.0 ** np.linspace(-4, -1, 8)
+    J = g.eval_J_batch(hu, hd, taus)
+    for t, j in zip(taus, J):
+        r = orc.eval_J(u + t * d, "LineMin")
+        assert abs(j - r) < 1e-10 * abs(r), (t, j, r)
+    J1 = np.array([g.eval_J_batch(hu, hd, [t])[0] for t in taus])
+    assert np.max(np.abs(J1 - J)) <= 1e-12 * np.max(np.abs(J))
+
+
+def test_mixed_vs_fp64_rcg_32():
+    n, q, B, iters = 32, 4.0, 1.0, 3
+    u0 = seeded_field(n, 3, "gauss", 16.0, q=q, B=B)
+    r64, t64, u64 = _solve_gpu(u0, n, q, B, iters)
+    r32, t32, u32 = _solve_gpu(u0, n, q, B, iters, precision="mixed", batch=3)
+    assert r32.iters == r64.iters
+    for a, b in zip(t32["iter"], t64["iter"]):
+        print("mixed vs fp64 J per iteration:", a[0], a[3], b[3], abs(a[3] - b[3]) / abs(b[3]))
+        assert abs(a[3] - b[3]) < 1e-4 * abs(b[3])
