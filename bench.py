@@ -130,7 +130,7 @@ def main():
         return
 
     log(f"mixed precision, batched line search: {n}^3 q={q} B={B}")
-    gm, hm, resm, t_mixed, _ = run_rcg(n, q, B, "mixed", 1, args.warmup, args.steps)
+    gm, hm, resm, t_mixed, _ = run_rcg(n, q, B, "mixed", 3, args.warmup, args.steps)
     nJm = resm.n_J / max(resm.iters, 1)
     fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))

@@ -16,14 +16,14 @@ template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
   resolve_params();
   MLQ_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
   // twiddles exp(-2 pi i k/N), computed in FP64 on the host
-  std::vector<C> tw(n);
+  std::vector<double2> tw(n);
   for (int k = 0; k < n; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)n;
-    tw[k].x = (T)std::cos(a);
-    tw[k].y = (T)std::sin(a);
+    tw[k].x = std::cos(a);
+    tw[k].y = std::sin(a);
   }
-  tw_ = (C*)dmalloc(sizeof(C) * n);
-  MLQ_HIP(hipMemcpy(tw_, tw.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+  tw_ = (double2*)dmalloc(sizeof(double2) * n);
+  MLQ_HIP(hipMemcpy(tw_, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
   scratch_ = (C*)dmalloc(sizeof(C) * ns_ * MAXB);
   part_ = (double*)dmalloc(sizeof(double) * RED_BLOCKS * RED_MAXK);
   red_dev_ = (double*)dmalloc(sizeof(double) * 64);

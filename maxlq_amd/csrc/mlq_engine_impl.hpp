@@ -164,7 +164,7 @@ template <typename T> class Engine final : public IEngine {
   mlq_params p_;
   bool use_e_u_ = false;
   hipStream_t st_ = nullptr;
-  C* tw_ = nullptr;
+  double2* tw_ = nullptr;   // FP64 twiddles for every field precision
   C* scratch_ = nullptr;
   double* part_ = nullptr;
   double* red_dev_ = nullptr;

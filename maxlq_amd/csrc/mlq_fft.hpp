@@ -97,17 +97,32 @@ template <typename T, int R, int DIR> __device__ __forceinline__ void dftR(c2<T>
 // ---- LDS views -------------------------------------------------------------------
 // Row view: one transform per contiguous LDS row, one pad slot every 8
 // elements so that the stride-R writes of the first passes hit distinct banks.
-template <typename T> struct LdsRow {
-  c2<T>* base;
+// Storage type S (the field precision) may be narrower than the FP64
+// registers the butterflies run in: ld/st convert, so the mixed path rounds to
+// FP32 only when data is exchanged, never inside a butterfly or twiddle.
+template <typename S> struct LdsRow {
+  c2<S>* base;
   static __host__ __device__ constexpr int pitch(int N) { return N + N / 8; }
-  __device__ __forceinline__ c2<T>& at(int n) const { return base[n + (n >> 3)]; }
+  __device__ __forceinline__ c2<double> ld(int n) const {
+    const c2<S> s = base[n + (n >> 3)];
+    return cmk<double>((double)s.x, (double)s.y);
+  }
+  __device__ __forceinline__ void st(int n, c2<double> v) const {
+    base[n + (n >> 3)] = cmk<S>((S)v.x, (S)v.y);
+  }
 };
 // Column view: C transforms interleaved (element n of column c at n*C + c), so
 // consecutive threads (consecutive columns) touch consecutive LDS words.
-template <typename T, int C> struct LdsCol {
-  c2<T>* base;
+template <typename S, int C> struct LdsCol {
+  c2<S>* base;
   int c;
-  __device__ __forceinline__ c2<T>& at(int n) const { return base[n * C + c]; }
+  __device__ __forceinline__ c2<double> ld(int n) const {
+    const c2<S> s = base[n * C + c];
+    return cmk<double>((double)s.x, (double)s.y);
+  }
+  __device__ __forceinline__ void st(int n, c2<double> v) const {
+    base[n * C + c] = cmk<S>((S)v.x, (S)v.y);
+  }
 };
 
 // ---- one Stockham pass -----------------------------------------------------------
@@ -124,7 +139,7 @@ __device__ __forceinline__ void fft_pass(c2<T> (&v)[E], int t, const c2<T>* __re
     for (int b = 0; b < NB; ++b) {
       const int j = t + b * Tf;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b + r * NB] = lds.at(j + r * (N / R));
+      for (int r = 0; r < R; ++r) v[b + r * NB] = lds.ld(j + r * (N / R));
     }
     __syncthreads();
   }
@@ -147,7 +162,7 @@ __device__ __forceinline__ void fft_pass(c2<T> (&v)[E], int t, const c2<T>* __re
     if constexpr (!last) {
       const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds.at(idxD + r * Ns) = x[r];
+      for (int r = 0; r < R; ++r) lds.st(idxD + r * Ns, x[r]);
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b + r * NB] = x[r];

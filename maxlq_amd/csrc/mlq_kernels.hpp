@@ -44,10 +44,10 @@ template <typename T> struct XArgs {
   const T* rsrc[MAXB];
   T* rdst[MAXB];
   c2<T>* spec;        // batch-contiguous spectra, stride sstride
-  const c2<T>* tw;
+  const double2* tw;  // exp(-2 pi i k/N), FP64 for every field precision
   long long sstride;
   int nb;
-  T scale;
+  double scale;
 };
 
 template <typename T> struct SArgs {
@@ -57,7 +57,7 @@ template <typename T> struct SArgs {
   c2<T>* dst[MAXB];
   SpecOp op[MAXB];
   int nsrc[MAXB];
-  const c2<T>* tw;
+  const double2* tw;
   long long sstride;
 };
 
@@ -111,27 +111,26 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const T* s = a.rsrc[b] + row0 * N;
-  c2<T> v[E];
+  c2<double> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = cmk<T>(s[t + m * Tf], s[N + t + m * Tf]);
+  for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
   LdsRow<T> L{sm + f * LP};
-  fft_core<T, N, E, -1>(v, t, a.tw, L);
+  fft_core<double, N, E, -1>(v, t, a.tw, L);
 #pragma unroll
-  for (int m = 0; m < E; ++m) L.at(t + m * Tf) = v[m];
+  for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
   __syncthreads();
   c2<T>* d = a.spec + b * a.sstride + row0 * PX;
-  const T hf = (T)0.5;
 #pragma unroll
   for (int m = 0; m < E / 2; ++m) {
     const int k = t + m * Tf;
-    const c2<T> zk = L.at(k), zn = L.at((N - k) & (N - 1));
-    d[k] = cmk<T>(hf * (zk.x + zn.x), hf * (zk.y - zn.y));
-    d[PX + k] = cmk<T>(hf * (zk.y + zn.y), -hf * (zk.x - zn.x));
+    const c2<double> zk = L.ld(k), zn = L.ld((N - k) & (N - 1));
+    d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
+    d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
   }
   if (t == 0) {
-    const c2<T> z = L.at(N / 2);
-    d[N / 2] = cmk<T>(z.x, (T)0);
-    d[PX + N / 2] = cmk<T>(z.y, (T)0);
+    const c2<double> z = L.ld(N / 2);
+    d[N / 2] = cmk<T>((T)z.x, (T)0);
+    d[PX + N / 2] = cmk<T>((T)z.y, (T)0);
   }
 }
 
@@ -150,24 +149,25 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const c2<T>* sp = a.spec + b * a.sstride + row0 * PX;
-  c2<T> v[E];
+  c2<double> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + m * Tf;
     const int kk = k <= N / 2 ? k : N - k;
-    c2<T> A = sp[kk], B = sp[PX + kk];
-    if (k == 0 || k == N / 2) { A.y = (T)0; B.y = (T)0; }
-    if (k > N / 2) { A.y = -A.y; B.y = -B.y; }
-    v[m] = cmk<T>(A.x - B.y, A.y + B.x);
+    const c2<T> As = sp[kk], Bs = sp[PX + kk];
+    double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
+    if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
+    if (k > N / 2) { ay = -ay; by = -by; }
+    v[m] = cmk<double>(ax - by, ay + bx);
   }
   LdsRow<T> L{sm + f * LP};
-  fft_core<T, N, E, +1>(v, t, a.tw, L);
+  fft_core<double, N, E, +1>(v, t, a.tw, L);
   T* o = a.rdst[b] + row0 * N;
-  const T sc = a.scale;
+  const double sc = a.scale;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    o[t + m * Tf] = v[m].x * sc;
-    o[N + t + m * Tf] = v[m].y * sc;
+    o[t + m * Tf] = (T)(v[m].x * sc);
+    o[N + t + m * Tf] = (T)(v[m].y * sc);
   }
 }
 
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
   const int col = blockIdx.x * C + c;
   bool valid = col < ncols;
   const int plane = blockIdx.y;
-  c2<T> v[E];
+  c2<double> v[E];
   int kx = 0, ky = 0;
   if (MODE != 0) {
     ky = col / PP;
@@ -198,44 +198,60 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     const int bb = plane;
     const SpecOp op = a.op[bb];
     const int ns = a.nsrc[bb];
+    c2<T> ld[E];
+    // all E loads of a source are issued before any arithmetic on them
+    {
+      const c2<T>* s0 = a.src[bb][0] + col;
+      const double c0 = a.coef[bb][0];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int kz = t + m * Tf;
-      c2<T> x = cmk<T>((T)0, (T)0);
-      if (valid) {
-        const long long off = (long long)kz * estride + col;
-        for (int i = 0; i < ns; ++i) {
-          const c2<T> s = a.src[bb][i][off];
-          const T cf = (T)a.coef[bb][i];
-          x.x += cf * s.x;
-          x.y += cf * s.y;
-        }
-        x = apply_factor<T>(x, spec_factor(op, kx, ky, kz, N));
+      for (int m = 0; m < E; ++m)
+        ld[m] = valid ? s0[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmk<double>(c0 * (double)ld[m].x, c0 * (double)ld[m].y);
+    }
+    for (int i = 1; i < ns; ++i) {
+      const c2<T>* si = a.src[bb][i] + col;
+      const double ci = a.coef[bb][i];
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        ld[m] = valid ? si[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        v[m].x += ci * (double)ld[m].x;
+        v[m].y += ci * (double)ld[m].y;
       }
-      v[m] = x;
+    }
+    if (op.flags != 0 || op.scale != 1.0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], spec_factor(op, kx, ky, t + m * Tf, N));
     }
   } else {
     const c2<T>* s = a.spec + plane * pstride + col;
+    c2<T> ld[E];
 #pragma unroll
     for (int m = 0; m < E; ++m)
-      v[m] = valid ? s[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+      ld[m] = valid ? s[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)ld[m].x, (double)ld[m].y);
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<T, N, E, DIR>(v, t, a.tw, L);
+  fft_core<double, N, E, DIR>(v, t, a.tw, L);
   if (!valid) return;
   if (MODE == 2) {
     const int bb = plane;
     const SpecOp op = a.op[bb];
     c2<T>* d = a.dst[bb] + col;
+    const bool mul = op.flags != 0 || op.scale != 1.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int kz = t + m * Tf;
-      d[(long long)kz * estride] = apply_factor<T>(v[m], spec_factor(op, kx, ky, kz, N));
+      const c2<double> r = mul ? cmul(v[m], spec_factor(op, kx, ky, kz, N)) : v[m];
+      d[(long long)kz * estride] = cmk<T>((T)r.x, (T)r.y);
     }
   } else {
     c2<T>* d = a.spec + plane * pstride + col;
 #pragma unroll
-    for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = v[m];
+    for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)v[m].x, (T)v[m].y);
   }
 }
 
