@@ -31,7 +31,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0):
+def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0, flags=0):
     """Run warmup+steps RCG iterations; time the last `steps` from the
     library's per-iteration records (each iteration ends in a synchronising
     reduction, so host stamps at the record are iteration boundaries)."""
@@ -54,7 +54,7 @@ def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0):
         elif kind == capi.MLQ_LOG_BRENT and i0 == 999:
             pass
 
-    cfg = capi.RcgCfg(warmup + steps, 0.0, 0, int(batch), 25, 10, 0)
+    cfg = capi.RcgCfg(warmup + steps, 0.0, 0, int(batch), 25, 10, int(flags))
     res = capi.RcgResult()
     t0 = time.perf_counter()
     rc = g.lib.mlq_rcg_solve(g.ctx, h, C.byref(cfg), C.byref(res), capi.LOG_CB(cb), None)
@@ -83,6 +83,22 @@ def fft_roofline(g, batch=12, reps=5):
     return out, tot_b / tot_t / 1e9, (batch * rb + 2 * batch * sb) / (0.5 * tot_t) / 1e9
 
 
+def pmc_traffic(kernel, n, precision):
+    """HBM bytes per launch of one pass kernel (batch 12), from the committed
+    summary of the FETCH_SIZE / WRITE_SIZE counter runs of `bench.py
+    --fft-only` (counters scaled by the copy-kernel calibration).  None when
+    no summary for this size and precision is present."""
+    tag = "fp32" if precision == "mixed" else "fp64"
+    path = os.path.join(ROOT, "profiles", f"pmc_fft_passes_{n}_{tag}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(q, B, threads):
     """Bounded sample of the reference-structured FP64 CPU path (the oracle):
     one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
@@ -98,6 +114,11 @@ def cpu_baseline(q, B, threads):
     orc.eval_J(u, "LineMin")
     tJ = time.perf_counter() - t0
     return tJ
+
+
+def capi_flag_reference():
+    from maxlq_amd import capi
+    return capi.MLQ_RCG_REFERENCE
 
 
 def main():
@@ -136,8 +157,12 @@ def main():
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
 
     t_fp64 = None
+    t_ref = None
     dR = None
     if not args.skip_fp64:
+        log("FP64, reference-structured iteration (every operator recomputed from the field)")
+        gr, hr, _, t_ref, _ = run_rcg(n, q, B, "fp64", 0, 0, 1, flags=capi_flag_reference())
+        gr.close()
         log("plain FP64 path")
         g64, h64, res64, t_fp64, _ = run_rcg(n, q, B, "fp64", 0, args.warmup, args.steps)
         # |dR|/R: R of the same field in both precisions (mixed iterate)
@@ -152,8 +177,9 @@ def main():
     dom = max(fft_passes, key=lambda k: fft_passes[k]["ms"])
     dp = fft_passes[dom]
     roof = {"bound": "hbm", "achieved": round(dp["GBs"], 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(dp["GBs"] / PEAK_HBM_GBS, 4), "traffic": None,
-            "kernel": f"fft_{dom}"}
+            "unit": "GB/s", "frac": round(dp["GBs"] / PEAK_HBM_GBS, 4),
+            "traffic": pmc_traffic(f"fft_{dom}", n, "mixed"),
+            "kernel": f"fft_{dom}", "algorithmic_bytes": dp["bytes"], "launch_ms": round(dp["ms"], 4)}
 
     cpu = None
     if not args.skip_cpu:
@@ -183,6 +209,8 @@ def main():
                    "n": n, "q": q, "B": B, "seed": 4,
                    "fp64_iter_per_s": (1.0 / t_fp64) if t_fp64 else None,
                    "mixed_over_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
+                   "fp64_reference_structured_iter_per_s": (1.0 / t_ref) if t_ref else None,
+                   "mixed_over_fp64_reference_structured": (t_ref / t_mixed) if t_ref else None,
                    "dR_over_R": dR, "n_J_per_iter_mixed": nJm,
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()}},
