@@ -36,15 +36,15 @@ struct Error : std::runtime_error {
 //   IK      i K_dim, zero at the own-dimension Nyquist (REAL(ifft) rule)
 //   NEGKSQ  -|K|^2                                   (laplacian, :3272)
 //   INVKSQ  -1/|K|^2, 0 at k = 0                     (solve_poisson, :1977-1981)
-//   HILBERT 1 / (1 + (a |K|)^e)                      (HilbertGradient, :3611)
 //   ZEROK0  zero the k = 0 mode                      (divAvg_free, :2891-2893)
-// times the real ``scale``.
+// times the real ``scale``.  (The Sobolev weight of HilbertGradient needs a
+// pow per mode and runs as its own spectral map, so that the pass kernels
+// carrying these factors stay within 64 VGPRs.)
 enum SpecFlags : int {
   SP_MASK = 1,
   SP_IK = 2,
   SP_NEGKSQ = 4,
   SP_INVKSQ = 8,
-  SP_HILBERT = 16,
   SP_ZEROK0 = 32,
 };
 
@@ -54,7 +54,6 @@ struct SpecOp {
   int mcut = 1 << 30;
   int pad = 0;
   double scale = 1.0;
-  double hil_a = 0.0, hil_e = 0.0;
 };
 
 inline SpecOp sp_none(double s = 1.0) { SpecOp o; o.scale = s; return o; }

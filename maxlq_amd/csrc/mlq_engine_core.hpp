@@ -43,6 +43,7 @@ template <typename T> Engine<T>::~Engine() {
   hipFree(tw_);
   hipFree(scratch_);
   hipFree(part_);
+  if (lspart_) hipFree(lspart_);
   hipFree(red_dev_);
   hipHostFree(red_host_);
   hipStreamDestroy(st_);
@@ -278,10 +279,12 @@ void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, long long e
   hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), dim3((ncols + S::C - 1) / S::C, planes),
                      dim3(S::C * (N / S::E)), 0, st, sa, ncols, es, ps);
 }
+// var: bits 0-1 the tile variant, bit 2 the XCD-contiguous tile order
 template <typename T, int N, int DIR, int MODE>
-void launch_strided(int var, hipStream_t st, const SArgs<T>& sa, int ncols, long long es,
+void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, long long es,
                     long long ps, int planes) {
-  switch (var) {
+  sa.remap = (var >> 2) & 1;
+  switch (var & 3) {
     case 1: launch_strided_v<T, N, 1, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
     case 2: launch_strided_v<T, N, 2, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
     case 3: launch_strided_v<T, N, 3, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
@@ -307,18 +310,30 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   xa.sstride = ns_;
   xa.nb = nb;
   xa.scale = (T)1;
+  // pruned when every output of the batch is masked to a cube smaller than the grid
+  int keep = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= N / 2) { keep = -1; break; }
+    keep = std::max(keep, ops[b].mcut);
+  }
+  xa.keep = keep;
   const long long pairs = (long long)nb * N * N / 2;
   const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
   prof_begin(PC_XFWD);
-  hipLaunchKernelGGL((k_xfwd<T, N, NF>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  if (keep >= 0)
+    hipLaunchKernelGGL((k_xfwd<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  else
+    hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_XFWD, rb + sb);
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
   sa.sstride = ns_;
+  sa.keep = keep;
   prof_begin(PC_YFWD);
-  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, PX, (long long)PP, (long long)N * PP, N * nb);
+  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep >= 0 ? keep + 1 : PX, (long long)PP,
+                              (long long)N * PP, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2 * sb);
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
@@ -329,14 +344,24 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
 }
 
 template <typename T> template <int N>
-void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst) {
+void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {
   constexpr int NF = xpass_NF<N>();
   constexpr int Tf = FftShape<N>::Tf;
   constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  // the effective input op[b] * src[b] is band-limited when the caller says
+  // so or when every op masks
+  int km = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= N / 2) { km = -1; break; }
+    km = std::max(km, ops[b].mcut);
+  }
+  if (km >= 0 && (keep < 0 || km < keep)) keep = km;
+  if (keep >= N / 2) keep = -1;
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
   sa.sstride = ns_;
+  sa.keep = keep;
   double zin = 0;
   for (int b = 0; b < nb; ++b) {
     sa.nsrc[b] = src[b].ns;
@@ -350,7 +375,8 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZINV, sb + zin * spec_bytes());
   prof_begin(PC_YINV);
-  launch_strided<T, N, 1, 0>(svar_[3], st_, sa, PX, (long long)PP, (long long)N * PP, N * nb);
+  launch_strided<T, N, 1, 0>(svar_[3], st_, sa, keep >= 0 ? keep + 1 : PX, (long long)PP,
+                             (long long)N * PP, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YINV, 2 * sb);
   XArgs<T> xa{};
@@ -359,10 +385,14 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   xa.tw = tw_;
   xa.sstride = ns_;
   xa.nb = nb;
+  xa.keep = keep;
   xa.scale = (T)(1.0 / ((double)N * N * N));
   const long long pairs = (long long)nb * N * N / 2;
   prof_begin(PC_XINV);
-  hipLaunchKernelGGL((k_xinv<T, N, NF>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  if (keep >= 0)
+    hipLaunchKernelGGL((k_xinv<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
+  else
+    hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_XINV, rb + sb);
 }
@@ -388,10 +418,10 @@ void Engine<T>::fwd(int nb, const T* const* src, C* const* dst, const SpecOp* op
   }
 }
 template <typename T>
-void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst) {
+void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {
   for (int b0 = 0; b0 < nb; b0 += MAXB) {
     const int k = std::min(MAXB, nb - b0);
-    MLQ_DISPATCH_N(inv_n, k, src + b0, ops + b0, dst + b0);
+    MLQ_DISPATCH_N(inv_n, k, src + b0, ops + b0, dst + b0, keep);
   }
 }
 
@@ -409,7 +439,7 @@ template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double or
     is[b] = InvSrc{{dp[b], nullptr, nullptr}, {1.0, 0, 0}, 1};
   }
   fwd(nb, f, dp.data(), ops.data());
-  inv(nb, is.data(), none.data(), f);
+  inv(nb, is.data(), none.data(), f, mc);
 }
 
 template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms) {
@@ -476,13 +506,15 @@ double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
   return (double)ms / reps;
 }
 
+// candidates per pass: 4 tile shapes x {dispatch order, XCD-contiguous order}
+constexpr int NTILE = 8;
 template <typename T> void Engine<T>::autotune(int batch, double* times, int* choice) {
   batch = std::max(1, std::min(batch, MAXB));
   MLQ_HIP(hipMemsetAsync(scratch_, 0, sizeof(C) * ns_ * batch, st_));
   for (int pass = 0; pass < 4; ++pass) {
     double best = 1e300;
     int bv = 0;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NTILE; ++v) {
       double t = 0.0;
       switch (n_) {
         case 8: t = time_pass_n<8>(pass, v, batch, 3); break;
@@ -495,7 +527,7 @@ template <typename T> void Engine<T>::autotune(int batch, double* times, int* ch
         case 1024: t = time_pass_n<1024>(pass, v, batch, 3); break;
         default: throw Error(MLQ_EINVAL, "unsupported n");
       }
-      if (times) times[pass * 4 + v] = t;
+      if (times) times[pass * NTILE + v] = t;
       if (t < best) { best = t; bv = v; }
     }
     svar_[pass] = bv;

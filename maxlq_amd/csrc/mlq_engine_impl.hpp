@@ -121,11 +121,12 @@ template <typename T> class Engine final : public IEngine {
   // ---- transforms ----
   // forward: src[b] (physical) -> dst[b] (spectrum) = op[b] * FFT(src[b])
   void fwd(int nb, const T* const* src, C* const* dst, const SpecOp* ops);
-  // inverse: dst[b] = REAL(IFFT(op[b] * sum_i coef[b][i] * src[b][i]))
+  // inverse: dst[b] = REAL(IFFT(op[b] * sum_i coef[b][i] * src[b][i])); keep >= 0
+  // declares every input zero outside the cube max|m_j| <= keep (pruned passes)
   struct InvSrc { const C* s[3]; double c[3]; int ns; };
-  void inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst);
+  void inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep = -1);
   template <int N> void fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops);
-  template <int N> void inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst);
+  template <int N> void inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep);
 
   // convenience
   void fwd1(const T* src, C* dst, SpecOp op = sp_none()) { fwd(1, &src, &dst, &op); }
@@ -134,6 +135,14 @@ template <typename T> class Engine final : public IEngine {
     inv(1, &s, &op, &dst);
   }
   void dealias_dev(T* const* f, int nb, double order);  // in place, batch
+  // fused q = 4 line-search sums for a batch of steps (k_xfwd_ls, pruned y
+  // pass, k_zls): out[2b] = sum_kept m |A_b|^2, out[2b+1] = sum_kept m
+  // Re(conj(A_b) Q(tau_b)); requires 0 <= keep < n/2 and nb <= MAXB
+  void ls_q4_sums(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1, const C* Q2,
+                  const double* taus, int nb, int keep, double* out);
+  template <int N>
+  void ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1, const C* Q2,
+                    const double* taus, int nb, int keep, double* out);
   int mcut_of(double order) const { return dealias_mcut(n_, order); }
   bool mask_is_identity(int mcut) const { return mcut >= n_ / 2; }
 
@@ -167,6 +176,8 @@ template <typename T> class Engine final : public IEngine {
   double2* tw_ = nullptr;   // FP64 twiddles for every field precision
   C* scratch_ = nullptr;
   double* part_ = nullptr;
+  double* lspart_ = nullptr;   // partial sums of the fused line-search z pass
+  size_t lspart_bytes_ = 0;
   double* red_dev_ = nullptr;
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;

@@ -453,30 +453,35 @@ template <typename T> class FastRcg {
   }
 
   void eval_q4(const double* taus, int nb, double* J, double* Lout) {
-    std::vector<RT> gg;
-    std::vector<ST> Ah;
-    F_ls_g<T> fg{};
-    fg.S0 = s_; fg.S1 = S1_; fg.S2 = S2_; fg.nb = nb;
-    std::vector<const T*> src(nb);
-    std::vector<C*> dst(nb);
-    std::vector<SpecOp> ops(nb, sp_mask(M_));
-    for (int b = 0; b < nb; ++b) {
-      gg.push_back(e_.rtmp());
-      Ah.push_back(e_.stmp());
-      fg.g[b] = nullptr;
-      fg.gg[b] = gg[b];
-      fg.tau[b] = taus[b];
-      src[b] = gg[b];
-      dst[b] = Ah[b];
-    }
-    e_.map(fg, nr_, PC_LS_POINT);
-    e_.fwd(nb, src.data(), dst.data(), ops.data());
-    R_ls_spec<T> rf{};
-    for (int b = 0; b < nb; ++b) { rf.A[b] = Ah[b]; rf.tau[b] = taus[b]; }
-    rf.Q0 = Q0h_; rf.Q1 = Q12h_; rf.Q2 = Q12h_ + ns_;
-    rf.nb = nb; rf.n = n_; rf.px = pp_; rf.M = M_;
     double acc[2 * MAXB];
-    e_.template reduce<2 * MAXB>(rf, (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), acc);
+    if (M_ < n_ / 2) {
+      e_.ls_q4_sums(s_, S1_, S2_, Q0h_, Q12h_, Q12h_ + ns_, taus, nb, M_, acc);
+    } else {
+      // the mask keeps every mode: unfused path (g^2 fields, transform, sums)
+      std::vector<RT> gg;
+      std::vector<ST> Ah;
+      F_ls_g<T> fg{};
+      fg.S0 = s_; fg.S1 = S1_; fg.S2 = S2_; fg.nb = nb;
+      std::vector<const T*> src(nb);
+      std::vector<C*> dst(nb);
+      std::vector<SpecOp> ops(nb, sp_mask(M_));
+      for (int b = 0; b < nb; ++b) {
+        gg.push_back(e_.rtmp());
+        Ah.push_back(e_.stmp());
+        fg.g[b] = nullptr;
+        fg.gg[b] = gg[b];
+        fg.tau[b] = taus[b];
+        src[b] = gg[b];
+        dst[b] = Ah[b];
+      }
+      e_.map(fg, nr_, PC_LS_POINT);
+      e_.fwd(nb, src.data(), dst.data(), ops.data());
+      R_ls_spec<T> rf{};
+      for (int b = 0; b < nb; ++b) { rf.A[b] = Ah[b]; rf.tau[b] = taus[b]; }
+      rf.Q0 = Q0h_; rf.Q1 = Q12h_; rf.Q2 = Q12h_ + ns_;
+      rf.nb = nb; rf.n = n_; rf.px = pp_; rf.M = M_;
+      e_.template reduce<2 * MAXB>(rf, (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), acc);
+    }
     for (int b = 0; b < nb; ++b) {
       const double t = taus[b];
       const double Lq_q = acc[2 * b] / N6_;
@@ -674,6 +679,88 @@ void Engine<T>::eval_J_batch(int u, int d, const double* taus, int ntau, double*
   F.set_direction_phys(fphys(d));
   F.precompute_ls();
   F.eval_batch(taus, ntau, J);
+}
+
+// ---------------------------------------------------------------------------
+// Fused q = 4 line-search sums (kernels in mlq_fast_kernels.hpp)
+// ---------------------------------------------------------------------------
+template <typename T, int N, int V>
+void launch_zls_v(hipStream_t st, const LsArgs<T>& la, int ncols, long long es, int nb) {
+  using S = SVar<T, N, V>;
+  hipLaunchKernelGGL((k_zls<T, N, S::E, S::C>), dim3((ncols + S::C - 1) / S::C, nb),
+                     dim3(S::C * (N / S::E)), 0, st, la, ncols, es);
+}
+
+template <typename T> template <int N>
+void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1,
+                             const C* Q2, const double* taus, int nb, int keep, double* out) {
+  constexpr int NF = xpass_NF<N>();
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int PP = spec_pitch<T>(N);
+  const int var = svar_[1];   // tile of the forward z pass
+  int C0 = 0;
+  switch (var & 3) {
+    case 1: C0 = SVar<T, N, 1>::C; break;
+    case 2: C0 = SVar<T, N, 2>::C; break;
+    case 3: C0 = SVar<T, N, 3>::C; break;
+    default: C0 = SVar<T, N, 0>::C; break;
+  }
+  const int ncols = (keep + 1) * (2 * keep + 1);
+  const int nbx = (ncols + C0 - 1) / C0;
+  const size_t need = sizeof(double) * (size_t)nbx * 2 * MAXB;
+  if (need > lspart_bytes_) {
+    if (lspart_) dfree(lspart_, lspart_bytes_);
+    lspart_ = (double*)dmalloc(need);
+    lspart_bytes_ = need;
+  }
+  LsArgs<T> la{};
+  la.S0 = S0; la.S1 = S1; la.S2 = S2;
+  la.Q0 = Q0; la.Q1 = Q1; la.Q2 = Q2;
+  la.spec = scratch_;
+  la.sstride = ns_;
+  la.tw = tw_;
+  for (int b = 0; b < nb; ++b) la.tau[b] = taus[b];
+  la.nb = nb;
+  la.keep = keep;
+  la.remap = (var >> 2) & 1;
+  la.part = lspart_;
+  prof_begin(PC_XFWD);
+  hipLaunchKernelGGL((k_xfwd_ls<T, N, NF>), dim3((unsigned)((long long)N * N / 2 / NF)),
+                     dim3(NF * Tf), 0, st_, la);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_XFWD, 3.0 * real_bytes() + nb * spec_bytes());
+  SArgs<T> sa{};
+  sa.spec = scratch_;
+  sa.tw = tw_;
+  sa.sstride = ns_;
+  sa.keep = keep;
+  prof_begin(PC_YFWD);
+  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep + 1, (long long)PP, (long long)N * PP, N * nb);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_YFWD, 2.0 * nb * spec_bytes());
+  prof_begin(PC_ZFWD);
+  switch (var & 3) {
+    case 1: launch_zls_v<T, N, 1>(st_, la, ncols, (long long)N * PP, nb); break;
+    case 2: launch_zls_v<T, N, 2>(st_, la, ncols, (long long)N * PP, nb); break;
+    case 3: launch_zls_v<T, N, 3>(st_, la, ncols, (long long)N * PP, nb); break;
+    default: launch_zls_v<T, N, 0>(st_, la, ncols, (long long)N * PP, nb); break;
+  }
+  MLQ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st_, lspart_, nbx, 2 * nb,
+                     red_dev_);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_ZFWD, 2.0 * nb * spec_bytes());
+  MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st_));
+  MLQ_HIP(hipStreamSynchronize(st_));
+  for (int k = 0; k < 2 * nb; ++k) out[k] = red_host_[k];
+}
+
+template <typename T>
+void Engine<T>::ls_q4_sums(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1,
+                           const C* Q2, const double* taus, int nb, int keep, double* out) {
+  if (nb < 1 || nb > MAXB || keep < 0 || keep >= n_ / 2)
+    throw Error(MLQ_EINVAL, "ls_q4_sums: batch or mask out of range");
+  MLQ_DISPATCH_N(ls_q4_sums_n, S0, S1, S2, Q0, Q1, Q2, taus, nb, keep, out);
 }
 
 }  // namespace mlq

@@ -324,4 +324,126 @@ template <typename T> struct R_kept_dot2 {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Fused q = 4 line-search sums.  For a batch of step sizes tau_b:
+//   A_b = M FFT(g_b^2),  g_b = sqrt(max(S0 + tau_b S1 + tau_b^2 S2, 0))
+//   out[2b]   = sum_kept m |A_b|^2                            (||phi||_q^q)
+//   out[2b+1] = sum_kept m Re(conj(A_b) (Q0 + tau Q1 + tau^2 Q2))
+// (the same quantities as F_ls_g -> masked forward transform -> R_ls_spec).
+// k_xfwd_ls forms g_b^2 from the coefficient rows and runs the x pass for
+// every step of the batch, storing only kx <= keep; the pruned forward y pass
+// follows; k_zls runs the z pass over the kept (kx, ky) columns only and
+// reduces instead of storing the spectra.
+// ---------------------------------------------------------------------------
+template <typename T> struct LsArgs {
+  const T* S0; const T* S1; const T* S2;
+  const c2<T>* Q0; const c2<T>* Q1; const c2<T>* Q2;
+  c2<T>* spec;          // scratch spectra, one per step, stride sstride
+  long long sstride;
+  const double2* tw;
+  double tau[MAXB];
+  int nb, keep, remap;
+  double* part;         // [z-pass block][2 nb] partial sums
+};
+
+template <typename T> __device__ __forceinline__ T ls_gg(T s0, T s1, T s2, T t) {
+  const T s = s0 + t * (s1 + t * s2);
+  const T gv = s <= (T)0 ? (T)0 : sqrt(s);
+  return gv * gv;
+}
+
+template <typename T, int N, int NF>
+__global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
+  constexpr int LP = LdsRow<T>::pitch(N);
+  __shared__ c2<T> sm[NF * LP];
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const long long row0 = 2 * ((long long)blockIdx.x * NF + f);
+  const T* r0 = a.S0 + row0 * N;
+  const T* r1 = a.S1 + row0 * N;
+  const T* r2 = a.S2 + row0 * N;
+  LdsRow<T> L{sm + f * LP};
+  for (int b = 0; b < a.nb; ++b) {
+    const T tb = (T)a.tau[b];
+    c2<double> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int i = t + m * Tf;
+      v[m] = cmk<double>((double)ls_gg(r0[i], r1[i], r2[i], tb),
+                         (double)ls_gg(r0[N + i], r1[N + i], r2[N + i], tb));
+    }
+    fft_core<double, N, E, -1>(v, t, a.tw, L);
+#pragma unroll
+    for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
+    __syncthreads();
+    c2<T>* d = a.spec + b * a.sstride + row0 * PX;
+#pragma unroll
+    for (int m = 0; m < E / 2; ++m) {
+      const int k = t + m * Tf;
+      if (k > a.keep) continue;
+      const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
+      d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
+      d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
+    }
+    __syncthreads();   // the LDS rows are reused by the next step's transform
+  }
+}
+
+template <typename T, int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void k_zls(LsArgs<T> a, int ncols, long long estride) {
+  constexpr int Tf = N / E, PP = spec_pitch<T>(N);
+  constexpr int NW = C * Tf / 64;
+  __shared__ c2<T> sm[N * C];
+  __shared__ double red[NW > 0 ? NW : 1][2];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  unsigned bx, by;
+  xcd_tile(a.remap != 0, bx, by);
+  const int col = (int)bx * C + c;
+  const bool valid = col < ncols;
+  const int b = (int)by;
+  const int K1 = a.keep + 1;
+  const int kx = valid ? col % K1 : 0;
+  const int my = (valid ? col / K1 : a.keep) - a.keep;
+  const int ky = my < 0 ? my + N : my;
+  const long long off = (long long)ky * PP + kx;   // offset of the column in a plane
+  const c2<T>* s = a.spec + b * a.sstride + off;
+  c2<T> v[E];
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = s[(long long)(t + m * Tf) * estride];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
+  }
+  LdsCol<T, C> L{sm, c};
+  fft_core<T, N, E, -1>(v, t, a.tw, L);
+  double s0 = 0.0, s1 = 0.0;
+  if (valid) {
+    const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+    const double tb = a.tau[b];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int kz = t + m * Tf;
+      const bool in = mabs(kz, N) <= a.keep;
+      const long long o = (long long)(in ? kz : 0) * estride + off;   // unconditional load
+      const c2<T> q0 = a.Q0[o], q1 = a.Q1[o], q2 = a.Q2[o];
+      const double qr = q0.x + tb * (q1.x + tb * q2.x), qi = q0.y + tb * (q1.y + tb * q2.y);
+      const double ax = v[m].x, ay = v[m].y;
+      const double wi = in ? w : 0.0;
+      s0 += wi * (ax * ax + ay * ay);
+      s1 += wi * (ax * qr + ay * qi);
+    }
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[wv][0] = s0; red[wv][1] = s1; }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double r = 0.0;
+    for (int i = 0; i < NW; ++i) r += red[i][threadIdx.x];
+    a.part[(long long)bx * (2 * a.nb) + 2 * b + threadIdx.x] = r;
+  }
+}
+
 }  // namespace mlq

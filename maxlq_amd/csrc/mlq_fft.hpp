@@ -97,16 +97,13 @@ template <typename T, int R, int DIR> __device__ __forceinline__ void dftR(c2<T>
 // ---- LDS views -------------------------------------------------------------------
 // Row view: one transform per contiguous LDS row, one pad slot every 8
 // elements so that the stride-R writes of the first passes hit distinct banks.
-// Storage type S (the field precision) may be narrower than the FP64
-// registers the butterflies run in: ld/st convert, so the mixed path rounds to
-// FP32 only when data is exchanged, never inside a butterfly or twiddle.
+// Storage type S is the field precision; ld returns it unchanged and st rounds
+// an FP64 butterfly result to it.
 template <typename S> struct LdsRow {
+  using Store = S;
   c2<S>* base;
   static __host__ __device__ constexpr int pitch(int N) { return N + N / 8; }
-  __device__ __forceinline__ c2<double> ld(int n) const {
-    const c2<S> s = base[n + (n >> 3)];
-    return cmk<double>((double)s.x, (double)s.y);
-  }
+  __device__ __forceinline__ c2<S> ld(int n) const { return base[n + (n >> 3)]; }
   __device__ __forceinline__ void st(int n, c2<double> v) const {
     base[n + (n >> 3)] = cmk<S>((S)v.x, (S)v.y);
   }
@@ -114,20 +111,32 @@ template <typename S> struct LdsRow {
 // Column view: C transforms interleaved (element n of column c at n*C + c), so
 // consecutive threads (consecutive columns) touch consecutive LDS words.
 template <typename S, int C> struct LdsCol {
+  using Store = S;
   c2<S>* base;
   int c;
-  __device__ __forceinline__ c2<double> ld(int n) const {
-    const c2<S> s = base[n * C + c];
-    return cmk<double>((double)s.x, (double)s.y);
-  }
+  __device__ __forceinline__ c2<S> ld(int n) const { return base[n * C + c]; }
   __device__ __forceinline__ void st(int n, c2<double> v) const {
     base[n * C + c] = cmk<S>((S)v.x, (S)v.y);
   }
 };
 
+// precision conversion of a complex value (c2<S> is not deducible: one
+// overload per source type)
+template <typename D> __device__ __forceinline__ c2<D> ccvt(float2 a) {
+  return cmk<D>((D)a.x, (D)a.y);
+}
+template <typename D> __device__ __forceinline__ c2<D> ccvt(double2 a) {
+  return cmk<D>((D)a.x, (D)a.y);
+}
+
 // ---- one Stockham pass -----------------------------------------------------------
-template <typename T, int N, int E, int DIR, int P, class L>
-__device__ __forceinline__ void fft_pass(c2<T> (&v)[E], int t, const c2<T>* __restrict__ tw,
+// Precision: every twiddle product and butterfly is evaluated in FP64.  The
+// per-thread register array v has type RT: the storage precision in the
+// strided pass kernels (an FP32 field then holds E complex floats between
+// passes, the same rounding as the LDS exchange, and the kernel stays within
+// 64 VGPRs), FP64 where the caller keeps more registers to spare.
+template <typename RT, int N, int E, int DIR, int P, class L>
+__device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
                                          const L& lds) {
   constexpr int Tf = N / E;
   constexpr int R = radix_at(N, E, P);
@@ -139,47 +148,47 @@ __device__ __forceinline__ void fft_pass(c2<T> (&v)[E], int t, const c2<T>* __re
     for (int b = 0; b < NB; ++b) {
       const int j = t + b * Tf;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b + r * NB] = lds.ld(j + r * (N / R));
+      for (int r = 0; r < R; ++r) v[b + r * NB] = ccvt<RT>(lds.ld(j + r * (N / R)));
     }
     __syncthreads();
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * Tf;
-    c2<T> x[R];
+    c2<double> x[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+    for (int r = 0; r < R; ++r) x[r] = ccvt<double>(v[b + r * NB]);
     if constexpr (Ns > 1) {
       const int k = j & (Ns - 1);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        c2<T> w = tw[r * k * (N / (Ns * R))];
+        double2 w = tw[r * k * (N / (Ns * R))];
         if (DIR > 0) w.y = -w.y;
         x[r] = cmul(x[r], w);
       }
     }
-    dftR<T, R, DIR>(x);
+    dftR<double, R, DIR>(x);
     if constexpr (!last) {
       const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
       for (int r = 0; r < R; ++r) lds.st(idxD + r * Ns, x[r]);
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b + r * NB] = x[r];
+      for (int r = 0; r < R; ++r) v[b + r * NB] = ccvt<RT>(x[r]);
     }
   }
   if constexpr (!last) {
     __syncthreads();
-    fft_pass<T, N, E, DIR, P + 1, L>(v, t, tw, lds);
+    fft_pass<RT, N, E, DIR, P + 1, L>(v, t, tw, lds);
   }
 }
 
 // Full length-N transform; DIR = -1 forward (exp(-i...)), +1 inverse
-// (unnormalised).  tw[k] = exp(-2 pi i k / N).
-template <typename T, int N, int E, int DIR, class L>
-__device__ __forceinline__ void fft_core(c2<T> (&v)[E], int t, const c2<T>* __restrict__ tw,
+// (unnormalised).  tw[k] = exp(-2 pi i k / N) in FP64.
+template <typename RT, int N, int E, int DIR, class L>
+__device__ __forceinline__ void fft_core(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
                                          const L& lds) {
-  fft_pass<T, N, E, DIR, 0, L>(v, t, tw, lds);
+  fft_pass<RT, N, E, DIR, 0, L>(v, t, tw, lds);
 }
 
 }  // namespace mlq

@@ -40,6 +40,14 @@ __device__ __forceinline__ bool kdecode(long long i, int n, int pp, KIdx& k) {
   return true;
 }
 
+// Band-limited (pruned) transforms: with keep >= 0 every spectrum of the
+// launch is zero outside the cube max_j |m_j| <= keep (the 2/3 dealias mask),
+// either because the forward pass's spectral factor masks it or because the
+// inverse's input is already masked.  The passes then skip the HBM traffic of
+// the modes outside: the forward x pass writes only kx <= keep, the forward y
+// pass transforms only those columns and writes only the kept ky rows, the
+// inverse passes read only kept modes and treat the rest as zero.  keep < 0:
+// full transforms.
 template <typename T> struct XArgs {
   const T* rsrc[MAXB];
   T* rdst[MAXB];
@@ -47,6 +55,7 @@ template <typename T> struct XArgs {
   const double2* tw;  // exp(-2 pi i k/N), FP64 for every field precision
   long long sstride;
   int nb;
+  int keep = -1;
   double scale;
 };
 
@@ -59,7 +68,26 @@ template <typename T> struct SArgs {
   int nsrc[MAXB];
   const double2* tw;
   long long sstride;
+  int keep = -1;
+  int remap = 0;   // 1: XCD-contiguous tile order (xcd_tile)
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remap the linear workgroup id so that XCD k runs a contiguous range of
+// logical tiles (in order): tiles that share 128-byte lines (adjacent column
+// groups) then go through one L2 instead of being fetched once per XCD.
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ void xcd_tile(bool on, unsigned& bx, unsigned& by) {
+  const unsigned nx = gridDim.x, total = nx * gridDim.y;
+  const unsigned pid = blockIdx.y * nx + blockIdx.x;
+  const unsigned lid =
+      (!on || (total % NUM_XCD)) ? pid : (pid % NUM_XCD) * (total / NUM_XCD) + pid / NUM_XCD;
+  bx = lid % nx;
+  by = lid / nx;
+}
+
+// |m| of FFT index k on an n-point axis (Nyquist counted +n/2)
+__device__ __forceinline__ int mabs(int k, int n) { return k <= (n >> 1) ? k : n - k; }
 
 // Spectral factor of one half-spectrum mode (see SpecOp in mlq_common.hpp).
 __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, int kz, int n) {
@@ -80,7 +108,6 @@ __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, 
     if (ksq == 0.0) return make_double2(0.0, 0.0);
     f.x *= -1.0 / ksq;
   }
-  if (fl & SP_HILBERT) f.x /= (1.0 + pow(o.hil_a * sqrt(ksq), o.hil_e));
   if (fl & SP_IK) {
     const int md = o.dim == 0 ? mx : (o.dim == 1 ? my : mz);
     if (md == h) return make_double2(0.0, 0.0);  // own-dimension Nyquist
@@ -99,7 +126,7 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
 // x pass, forward: two real rows packed as one complex row (a + i b), one
 // N-point FFT, split into the two half spectra.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int NF>
+template <typename T, int N, int NF, bool PR>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -120,15 +147,17 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
   __syncthreads();
   c2<T>* d = a.spec + b * a.sstride + row0 * PX;
+  const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
 #pragma unroll
   for (int m = 0; m < E / 2; ++m) {
     const int k = t + m * Tf;
-    const c2<double> zk = L.ld(k), zn = L.ld((N - k) & (N - 1));
+    if (k > kmax) continue;
+    const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
     d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
     d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
   }
-  if (t == 0) {
-    const c2<double> z = L.ld(N / 2);
+  if (t == 0 && kmax == N / 2) {
+    const c2<double> z = ccvt<double>(L.ld(N / 2));
     d[N / 2] = cmk<T>((T)z.x, (T)0);
     d[PX + N / 2] = cmk<T>((T)z.y, (T)0);
   }
@@ -137,7 +166,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
 // spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
 // inverse FFT, unpack, scale.
-template <typename T, int N, int NF>
+template <typename T, int N, int NF, bool PR>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -149,13 +178,19 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const c2<T>* sp = a.spec + b * a.sstride + row0 * PX;
+  const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   c2<double> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + m * Tf;
     const int kk = k <= N / 2 ? k : N - k;
-    const c2<T> As = sp[kk], Bs = sp[PX + kk];
+    // modes beyond the kept cube are zero: load from kx = 0 (unconditional
+    // load, no branch) and discard
+    const bool in = kk <= kmax;
+    const int ks = in ? kk : 0;
+    const c2<T> As = sp[ks], Bs = sp[PX + ks];
     double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
+    if (!in) { ax = 0.0; ay = 0.0; bx = 0.0; by = 0.0; }
     if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
     if (k > N / 2) { ay = -ay; by = -by; }
     v[m] = cmk<double>(ax - by, ay + bx);
@@ -184,74 +219,130 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   __shared__ c2<T> sm[N * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
+  unsigned bx, by;
+  xcd_tile(a.remap != 0, bx, by);
+  const int col = (int)bx * C + c;
   bool valid = col < ncols;
-  const int plane = blockIdx.y;
-  c2<double> v[E];
+  const int plane = (int)by;
+  c2<T> v[E];
   int kx = 0, ky = 0;
+  const int keep = a.keep;
+  // kept: the column carries modes inside the kept cube (z passes); rowin(n):
+  // element n of the column is read (inverse passes skip masked rows)
+  bool kept = true;
   if (MODE != 0) {
     ky = col / PP;
     kx = col - ky * PP;
     valid = valid && kx <= N / 2;   // padding columns carry no data
+    kept = keep < 0 || (kx <= keep && mabs(ky, N) <= keep);
   }
+  const bool ld_ok = valid && kept;
+  auto rowin = [&](int n) { return DIR < 0 || keep < 0 || mabs(n, N) <= keep; };
+  // rows outside the kept cube: the load is redirected to row 0 of the same
+  // column (always kept, in bounds) and its value replaced by zero, so the E
+  // loads stay unconditional (no per-element branch)
+  auto ldrow = [&](const c2<T>* base, int n) {
+    const bool r = rowin(n);
+    const c2<T> x = base[(long long)(r ? n : 0) * estride];
+    return r ? x : cmk<T>((T)0, (T)0);
+  };
   if (MODE == 1) {
     const int bb = plane;
     const SpecOp op = a.op[bb];
     const int ns = a.nsrc[bb];
-    c2<T> ld[E];
-    // all E loads of a source are issued before any arithmetic on them
-    {
-      const c2<T>* s0 = a.src[bb][0] + col;
-      const double c0 = a.coef[bb][0];
+    const c2<T>* s0 = a.src[bb][0] + col;
+    if (ld_ok) {
 #pragma unroll
-      for (int m = 0; m < E; ++m)
-        ld[m] = valid ? s0[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+      for (int m = 0; m < E; ++m) v[m] = ldrow(s0, t + m * Tf);
+    } else {
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cmk<double>(c0 * (double)ld[m].x, c0 * (double)ld[m].y);
+      for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
     }
+    // c0 stays pending until the first combination (one rounding per add) or
+    // is folded into the spectral factor
+    double c0 = a.coef[bb][0];
     for (int i = 1; i < ns; ++i) {
       const c2<T>* si = a.src[bb][i] + col;
       const double ci = a.coef[bb][i];
+      c2<T> ld[E];
+      if (ld_ok) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) ld[m] = ldrow(si, t + m * Tf);
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) ld[m] = cmk<T>((T)0, (T)0);
+      }
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        ld[m] = valid ? si[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        v[m].x += ci * (double)ld[m].x;
-        v[m].y += ci * (double)ld[m].y;
-      }
+        v[m] = cmk<T>((T)(c0 * (double)v[m].x + ci * (double)ld[m].x),
+                      (T)(c0 * (double)v[m].y + ci * (double)ld[m].y));
+      c0 = 1.0;
     }
-    if (op.flags != 0 || op.scale != 1.0) {
+    // Spectral factor (uniform branch: op and c0 are per plane).  The tile
+    // goes through the thread's own LDS slots so that the factor code runs
+    // in a rolled loop, one element at a time, and does not multiply the
+    // register footprint by E; the barrier orders these slot accesses before
+    // the first FFT pass writes other threads' slots.
+    if (op.flags != 0 || op.scale != 1.0 || c0 != 1.0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], spec_factor(op, kx, ky, t + m * Tf, N));
+      for (int m = 0; m < E; ++m) sm[(t + m * Tf) * C + c] = v[m];
+#pragma unroll 1
+      for (int m = 0; m < E; ++m) {
+        const int kz = t + m * Tf;
+        const double2 f = spec_factor(op, kx, ky, kz, N);
+        const c2<double> r = cmul(ccvt<double>(sm[kz * C + c]), make_double2(c0 * f.x, c0 * f.y));
+        sm[kz * C + c] = cmk<T>((T)r.x, (T)r.y);
+      }
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = sm[(t + m * Tf) * C + c];
+      __syncthreads();
     }
   } else {
     const c2<T>* s = a.spec + plane * pstride + col;
-    c2<T> ld[E];
+    if (ld_ok) {
 #pragma unroll
-    for (int m = 0; m < E; ++m)
-      ld[m] = valid ? s[(long long)(t + m * Tf) * estride] : cmk<T>((T)0, (T)0);
+      for (int m = 0; m < E; ++m) v[m] = ldrow(s, t + m * Tf);
+    } else {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)ld[m].x, (double)ld[m].y);
+      for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
+    }
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<double, N, E, DIR>(v, t, a.tw, L);
+  fft_core<T, N, E, DIR>(v, t, a.tw, L);
   if (!valid) return;
+  if (MODE == 1 && !kept) return;   // the pruned inverse y pass never reads it
   if (MODE == 2) {
     const int bb = plane;
     const SpecOp op = a.op[bb];
     c2<T>* d = a.dst[bb] + col;
-    const bool mul = op.flags != 0 || op.scale != 1.0;
+    if (!kept) {   // outside the kept cube: the masked spectrum is zero
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int kz = t + m * Tf;
-      const c2<double> r = mul ? cmul(v[m], spec_factor(op, kx, ky, kz, N)) : v[m];
-      d[(long long)kz * estride] = cmk<T>((T)r.x, (T)r.y);
+      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+    } else if (op.flags != 0 || op.scale != 1.0) {
+      // rolled epilogue through the thread's own LDS slots (every read of the
+      // last FFT pass precedes its barrier, so the slots are free)
+#pragma unroll
+      for (int m = 0; m < E; ++m) sm[(t + m * Tf) * C + c] = v[m];
+#pragma unroll 1
+      for (int m = 0; m < E; ++m) {
+        const int kz = t + m * Tf;
+        const c2<double> p = cmul(ccvt<double>(sm[kz * C + c]), spec_factor(op, kx, ky, kz, N));
+        d[(long long)kz * estride] = cmk<T>((T)p.x, (T)p.y);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = v[m];
     }
   } else {
+    // y passes and the inverse z pass: the forward y pass stores only the
+    // kept ky rows (the pruned z pass reads nothing else)
     c2<T>* d = a.spec + plane * pstride + col;
 #pragma unroll
-    for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)v[m].x, (T)v[m].y);
+    for (int m = 0; m < E; ++m) {
+      const int n = t + m * Tf;
+      if (MODE == 0 && DIR < 0 && keep >= 0 && mabs(n, N) > keep) continue;
+      d[(long long)n * estride] = v[m];
+    }
   }
 }
 

@@ -341,13 +341,8 @@ template <typename T> void Engine<T>::divAvg_free(int f_vec) {
 template <typename T> void Engine<T>::HilbertGradient(int f_vec, double order) {
   ST uh[3] = {stmp(), stmp(), stmp()};
   C* uhp[3] = {uh[0], uh[1], uh[2]};
-  SpecOp h;
-  h.flags = SP_HILBERT;
-  h.hil_a = p_.lambda1;
-  h.hil_e = 2.0 * order;
-  SpecOp ops[3] = {h, h, h};
-  const T* src[3] = {fphys(f_vec, 0), fphys(f_vec, 1), fphys(f_vec, 2)};
-  fwd(3, src, uhp, ops);
+  spec3(fphys(f_vec), uhp);
+  smap(S_hilbert3<T>{uhp[0], uhp[1], uhp[2], n_, pp_, p_.lambda1, 2.0 * order});
   InvSrc is[3];
   SpecOp none[3] = {sp_none(), sp_none(), sp_none()};
   T* dst[3];

@@ -232,6 +232,26 @@ template <typename T> struct S_leray {
     u2[i] = cmk<T>((T)(c.x - e3 * dr), (T)(c.y - e3 * di));
   }
 };
+
+// Sobolev weight of HilbertGradient (function_ops.f90:3605-3615) on the three
+// spectra of a vector: u_j / (1 + (lam |K|)^e); the k = 0 mode keeps weight 1.
+template <typename T> struct S_hilbert3 {
+  c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px; double lam, e;
+  __device__ void operator()(long long i) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
+    const double w = 1.0 / (1.0 + pow(lam * sqrt(K1 * K1 + K2 * K2 + K3 * K3), e));
+    const c2<T> a = u0[i], b = u1[i], c = u2[i];
+    u0[i] = cmk<T>((T)(w * a.x), (T)(w * a.y));
+    u1[i] = cmk<T>((T)(w * b.x), (T)(w * b.y));
+    u2[i] = cmk<T>((T)(w * c.x), (T)(w * c.y));
+  }
+};
 // divergence spectrum: sum_l i K'_l w_l  (function_ops.f90:3231)
 template <typename T> struct S_div {
   const c2<T>* w0; const c2<T>* w1; const c2<T>* w2; c2<T>* o; int n, px;

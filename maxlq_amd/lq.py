@@ -291,11 +291,12 @@ class MaxLq:
         return out
 
     def autotune(self, batch=3):
-        """Time the strided-pass tile variants; returns (times[4][4] ms, choice[4])."""
-        t = np.zeros(16)
+        """Time the strided-pass tile candidates; returns (times[4][8] ms,
+        choice[4]); candidate = shape (0-3) + 4 * (XCD-contiguous order)."""
+        t = np.zeros(32)
         ch = (C.c_int * 4)()
         self._chk(self.lib.mlq_autotune(self.ctx, batch, _p(t), ch))
-        return t.reshape(4, 4), list(ch)
+        return t.reshape(4, 8), list(ch)
 
     def time_copy(self, nfields=3, reps=5):
         r = C.c_double()
