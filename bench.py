@@ -116,6 +116,101 @@ def cpu_baseline(q, B, threads):
     return tJ
 
 
+def bench_linesearch8(args):
+    """8 trial step sizes tau_i = 10^(-4 + 3i/7) at 512^3, q = 4, mixed: one
+    batched launch vs 8 one-at-a-time launches after the same coefficient
+    precompute, and the reference-structured evaluation (94 transforms per J)."""
+    from maxlq_amd.lq import MaxLq
+    n, q, B = args.n, args.q, args.B
+    g = MaxLq(n, q=q, B=B, precision="mixed")
+    hu, hd = g.field(3), g.field(3)
+    g.seeded_field(hu, 4, "gauss", 256.0)
+    g.seeded_field(hd, 14, "gauss", 256.0)
+    taus = 10.0 ** (-4.0 + 3.0 * np.arange(8) / 7.0)
+    g.time_linesearch(hu, hd, taus, 8)                      # warm-up
+    Jb, tp, tb = g.time_linesearch(hu, hd, taus, 8)
+    Js, tp1, ts = g.time_linesearch(hu, hd, taus, 1)
+    Jr, _, tr = g.time_linesearch(hu, hd, taus, 0)
+    used, peak, _, _ = g.mem_info()
+    out = {"metric": "8-step line search at 512^3 q=4 mixed: batched vs sequential (ms)",
+           "value": tb, "unit": "ms", "higher_is_better": False,
+           "config": {"workload": f"J_LineMin at 8 step sizes, {n}^3 q={q:g} B={B:g}, u seed 4, "
+                                  "d seed 14, exp(-|m|^2/256)",
+                      "taus": list(taus), "precompute_ms": tp, "batched_ms": tb,
+                      "sequential_ms": ts, "reference_structured_ms": tr,
+                      "sequential_over_batched": ts / tb, "reference_over_batched": tr / tb,
+                      "max_rel_dJ_batched_vs_sequential": float(np.max(np.abs(Jb - Js) / np.abs(Js))),
+                      "max_rel_dJ_batched_vs_reference": float(np.max(np.abs(Jb - Jr) / np.abs(Jr))),
+                      "mem_peak_GB": peak / 1e9}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_continuation(args):
+    """256^3, q = 6, seed 3, |m|^2 exp(-|m|^2/64): B_j = 4^(j/9), j = 0..9,
+    30 RCG iterations per B (OPTIM_TOL 0), mixed and FP64 each continuing
+    from its own result; R error of the mixed path per step."""
+    import ctypes as C
+    from maxlq_amd import capi
+    from maxlq_amd.lq import MaxLq
+    n, q = 256, 6.0
+    Bs = [4.0 ** (j / 9.0) for j in range(10)]
+    ctx = {}
+    for prec in ("mixed", "fp64"):
+        g = MaxLq(n, q=q, B=Bs[0], precision=prec)
+        h = g.field(3)
+        g.seeded_field(h, 3, "k2gauss", 64.0)
+        ctx[prec] = (g, h)
+    steps = []
+    for j, B in enumerate(Bs):
+        row = {"B": B}
+        for prec in ("mixed", "fp64"):
+            g, h = ctx[prec]
+            g.set_params(B=B)
+            cfg = capi.RcgCfg(30, 0.0, 0, 3 if prec == "mixed" else 0, 25, 10, 0)
+            res = capi.RcgResult()
+            t0 = time.perf_counter()
+            g._chk(g.lib.mlq_rcg_solve(g.ctx, h, C.byref(cfg), C.byref(res),
+                                       C.cast(None, capi.LOG_CB), None))
+            row[prec] = {"J": res.J1, "iters": res.iters, "s": time.perf_counter() - t0,
+                         "n_J": res.n_J, "stop": res.stop_reason}
+        g64, h64 = ctx["fp64"]
+        gm, hm = ctx["mixed"]
+        um = gm.download(hm)
+        Rm = gm.calc_local_dLqdt_inclParts(um)[0]
+        R64 = g64.calc_local_dLqdt_inclParts(um)[0]
+        row["dR_over_R_same_field"] = abs(Rm - R64) / abs(R64)
+        row["dJ_over_J_trajectories"] = abs(row["mixed"]["J"] - row["fp64"]["J"]) / abs(row["fp64"]["J"])
+        log(f"B={B:.4f} J mixed {row['mixed']['J']:.10e} fp64 {row['fp64']['J']:.10e} "
+            f"same-field dR/R {row['dR_over_R_same_field']:.2e}")
+        steps.append(row)
+    tm = sum(r["mixed"]["s"] for r in steps)
+    t64 = sum(r["fp64"]["s"] for r in steps)
+    im = sum(r["mixed"]["iters"] for r in steps)
+    i64 = sum(r["fp64"]["iters"] for r in steps)
+    out = {"metric": "B-continuation 256^3 q=6, 10 B x 30 RCG iterations: mixed vs FP64",
+           "value": im / tm, "unit": "iter/s", "higher_is_better": True,
+           "config": {"workload": "256^3 q=6 seed 3 |m|^2 exp(-|m|^2/64), B_j = 4^(j/9)",
+                      "mixed_s": tm, "fp64_s": t64, "mixed_iters": im, "fp64_iters": i64,
+                      "fp64_iter_per_s": i64 / t64,
+                      "max_dR_over_R_same_field": max(r["dR_over_R_same_field"] for r in steps),
+                      "steps": steps}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_n1024(args):
+    """1024^3, q = 9, seed 5, exp(-|m|^2/1024), B = 2.2, mixed: one full RCG
+    iteration; allocator high-water mark and device memory against 288 GB."""
+    n, q, B = 1024, 9.0, 2.2
+    g, h, res, t_it, _ = run_rcg(n, q, B, "mixed", 3, 0, 1, seed=5, a=1024.0)
+    used, peak, free, total = g.mem_info()
+    out = {"metric": "RCG iteration 1024^3 q=9 mixed: s/iter and memory peak",
+           "value": t_it, "unit": "s/iter", "higher_is_better": False,
+           "config": {"workload": "1024^3 q=9 B=2.2 seed 5 exp(-|m|^2/1024)", "J1": res.J1,
+                      "n_J": res.n_J, "mem_peak_GB": peak / 1e9, "mem_in_use_GB": used / 1e9,
+                      "device_total_GB": total / 1e9, "device_free_GB": free / 1e9}}
+    print(json.dumps(out), flush=True)
+
+
 def capi_flag_reference():
     from maxlq_amd import capi
     return capi.MLQ_RCG_REFERENCE
@@ -133,12 +228,21 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--fft-only", choices=["mixed", "fp64"], default=None,
                     help="only the batched transform sweep and a copy kernel (for profilers)")
+    ap.add_argument("--case", choices=["iteration", "linesearch8", "continuation", "n1024"],
+                    default="iteration",
+                    help="iteration: the headline line (default); linesearch8: 8 step sizes "
+                         "batched vs sequential at 512^3; continuation: 256^3 q=6 B-continuation "
+                         "mixed vs FP64; n1024: one 1024^3 q=9 iteration and its memory peak")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     n, q, B = args.n, args.q, args.B
+    if args.case != "iteration":
+        {"linesearch8": bench_linesearch8, "continuation": bench_continuation,
+         "n1024": bench_n1024}[args.case](args)
+        return
     if args.fft_only:
         from maxlq_amd.lq import MaxLq
         g = MaxLq(n, q=q, B=B, precision=args.fft_only)
@@ -153,6 +257,7 @@ def main():
     log(f"mixed precision, batched line search: {n}^3 q={q} B={B}")
     gm, hm, resm, t_mixed, _ = run_rcg(n, q, B, "mixed", 3, args.warmup, args.steps)
     nJm = resm.n_J / max(resm.iters, 1)
+    mem_used, mem_peak, _, _ = gm.mem_info()
     fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
 
@@ -211,7 +316,7 @@ def main():
                    "mixed_over_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
                    "fp64_reference_structured_iter_per_s": (1.0 / t_ref) if t_ref else None,
                    "mixed_over_fp64_reference_structured": (t_ref / t_mixed) if t_ref else None,
-                   "dR_over_R": dR, "n_J_per_iter_mixed": nJm,
+                   "dR_over_R": dR, "n_J_per_iter_mixed": nJm, "mem_peak_GB_mixed": mem_peak / 1e9,
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()}},
         "roofline": roof,

@@ -140,6 +140,26 @@ int mlq_seeded_field(mlq_ctx* ctx, mlq_field f, unsigned long long seed, int kin
 int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
                      double* J);
 
+/* Measurement of the batched line search: one coefficient precompute for
+ * phi = u + tau d, then J_LineMin at the ntau step sizes in launches of
+ * `batch` step sizes each (1: one after the other; ntau: all in one batch;
+ * 0: the reference-structured evaluation, every J recomputed from the field
+ * as eval_J does).  t_ms[0] = precompute, t_ms[1] = evaluations (ms).  No
+ * counterpart in the reference (its mnbrak/brent evaluate J one at a time,
+ * optimization.f90:666-752). */
+int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
+                        int batch, double* J, double* t_ms);
+
+/* kappa_test (optimization.f90:957-1129): J0 = eval_J(phi), then for
+ * ii = 1..16, eps = 10^(ii-15): kappa = (eval_J(phi + eps pert) - J0) /
+ * (eps <grad, pert>_kind).  The perturbation is the caller's (the reference
+ * loads or draws it in kappa_test_pert).  table: 16 rows {eps, <grad, pert>,
+ * J1 - J0, kappa, kappa adjusted} as save_kappa_test writes them; *frac the
+ * adjustment factor; best kappa and its 1-based row by the reference's
+ * criterion.  mysystem: MLQ_J_MAXDLQDT or MLQ_J_LINEMIN. */
+int mlq_kappa_test(mlq_ctx* ctx, mlq_field phi, mlq_field grad, mlq_field pert, int mysystem,
+                   int inner_kind, double* table, double* frac, double* best_kappa, int* best_row);
+
 /* ---- the whole RCG solve for one B: maxdLqdt, optimization.f90:7-514 ---- */
 typedef struct mlq_rcg_cfg {
   int max_iter;          /* MAX_ITER */

@@ -336,6 +336,23 @@ int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus,
       if (std::isnan(J[i])) throw Error(MLQ_ENAN, "nan in J");
   });
 }
+int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
+                        int batch, double* J, double* t_ms) {
+  return guarded(ctx, [&] {
+    if (ntau < 1 || !taus || !J || !t_ms || batch < 0) throw Error(MLQ_EINVAL, "bad arguments");
+    ctx->eng->time_linesearch(u, d, taus, ntau, batch, J, t_ms);
+  });
+}
+int mlq_kappa_test(mlq_ctx* ctx, mlq_field phi, mlq_field grad, mlq_field pert, int mysystem,
+                   int inner_kind, double* table, double* frac, double* best_kappa, int* best_row) {
+  return guarded(ctx, [&] {
+    if (!table || !frac || !best_kappa || !best_row) throw Error(MLQ_EINVAL, "null output");
+    if (mysystem != MLQ_J_MAXDLQDT && mysystem != MLQ_J_LINEMIN)
+      throw Error(MLQ_EBADKIND, "kappa test system must be maxdLqdt or LineMin");
+    if (inner_kind < MLQ_IP_L2 || inner_kind > MLQ_IP_H) throw Error(MLQ_EBADKIND, "bad inner product");
+    ctx->eng->kappa_test(phi, grad, pert, mysystem, inner_kind, table, frac, best_kappa, best_row);
+  });
+}
 int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_result* res,
                   mlq_log_cb cb, void* user) {
   return guarded(ctx, [&] {

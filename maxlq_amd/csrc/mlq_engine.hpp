@@ -73,6 +73,14 @@ class IEngine {
   virtual void grad_L2(int u_vec, int out_vec) = 0;
   virtual double eval_J(int f_vec, int system) = 0;
   virtual void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) = 0;
+  // one coefficient precompute, then the step sizes in launches of `batch`
+  // (0: reference-structured evaluations); t_ms = {precompute, evaluation}
+  virtual void time_linesearch(int u, int d, const double* taus, int ntau, int batch, double* J,
+                               double* t_ms) = 0;
+  // kappa test (optimization.f90:957-1129) with a given perturbation: table
+  // rows {eps, <grad, pert>, J1 - J0, kappa, kappa adjusted}, 16 rows
+  virtual void kappa_test(int phi, int grad, int pert, int system, int kind, double* table,
+                          double* frac, double* best_kappa, int* best_row) = 0;
   virtual void seeded(int f_vec, unsigned long long seed, int kind, double a) = 0;
 
   // the RCG solve (optimization.f90:7-514)
@@ -82,8 +90,8 @@ class IEngine {
   // measurement
   virtual void time_fft(int batch, int reps, double* t_ms) = 0;
   virtual void time_copy(int nfields, int reps, double* t_ms) = 0;
-  // time the tile variants of the strided passes and keep the fastest;
-  // times[pass*4+var] in ms per launch, choice[pass]
+  // time the tile candidates of the strided passes and keep the fastest;
+  // times[pass*8+cand] in ms per launch, choice[pass]
   virtual void autotune(int batch, double* times, int* choice) = 0;
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;

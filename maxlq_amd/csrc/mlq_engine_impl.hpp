@@ -88,6 +88,10 @@ template <typename T> class Engine final : public IEngine {
   void grad_L2(int u_vec, int out_vec) override;
   double eval_J(int f_vec, int system) override;
   void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) override;
+  void time_linesearch(int u, int d, const double* taus, int ntau, int batch, double* J,
+                       double* t_ms) override;
+  void kappa_test(int phi, int grad, int pert, int system, int kind, double* table, double* frac,
+                  double* best_kappa, int* best_row) override;
   void seeded(int f_vec, unsigned long long seed, int kind, double a) override;
   void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) override;
   void time_fft(int batch, int reps, double* t_ms) override;

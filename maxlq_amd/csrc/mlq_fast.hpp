@@ -17,6 +17,7 @@
 //    retraction B phi/||phi||_q enters through R(s phi) = s^q (R1+R2)(phi) +
 //    s^(q+1) R3(phi).
 #pragma once
+#include <chrono>
 #include <map>
 #include <vector>
 
@@ -679,6 +680,36 @@ void Engine<T>::eval_J_batch(int u, int d, const double* taus, int ntau, double*
   F.set_direction_phys(fphys(d));
   F.precompute_ls();
   F.eval_batch(taus, ntau, J);
+}
+
+template <typename T>
+void Engine<T>::time_linesearch(int u, int d, const double* taus, int ntau, int batch, double* J,
+                                double* t_ms) {
+  using clk = std::chrono::steady_clock;
+  auto ms = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  sync();
+  if (batch <= 0 || !fast_path_ok()) {
+    const auto t0 = clk::now();
+    eval_J_batch_ref(u, d, taus, ntau, J);
+    sync();
+    t_ms[0] = 0.0;
+    t_ms[1] = ms(t0, clk::now());
+    return;
+  }
+  FastRcg<T> F(*this);
+  F.init_state(fphys(u));
+  F.set_direction_phys(fphys(d));
+  sync();
+  const auto t0 = clk::now();
+  F.precompute_ls();
+  sync();
+  const auto t1 = clk::now();
+  for (int i = 0; i < ntau; i += batch) F.eval_batch(taus + i, std::min(batch, ntau - i), J + i);
+  sync();
+  t_ms[0] = ms(t0, t1);
+  t_ms[1] = ms(t1, clk::now());
 }
 
 // ---------------------------------------------------------------------------

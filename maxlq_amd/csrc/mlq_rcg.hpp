@@ -3,6 +3,7 @@
 // kappa tests of the reference are diagnostics); every record the reference
 // writes from inside the loop is delivered to the log sink.
 #pragma once
+#include <limits>
 #include <chrono>
 #include <vector>
 
@@ -167,6 +168,59 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   res.n_J = (int)nJ;
   res.n_J_computed = (int)nJc;
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// kappa test (optimization.f90:957-1129) with the perturbation given by the
+// caller (the reference loads or draws its own, :1001-1011): 16 step sizes
+// eps = 10^(ii - 15), kappa = (J(phi + eps pert) - J(phi)) / (eps <grad, pert>),
+// the best row by the reference's criterion (:1035, including its use of the
+// current kappa on both sides) and the 4-point adjusted normalisation
+// (:1075-1097).  table[ii*5 + c] = {eps, <grad, pert>, J1 - J0, kappa,
+// kappa adjusted}; best_row is 1-based.
+template <typename T>
+void Engine<T>::kappa_test(int phi, int grad, int pert, int system, int kind, double* table,
+                           double* frac_out, double* best_kappa, int* best_row) {
+  constexpr int NK = 16, ADJ = 4;
+  const double innerProd = inner(grad, pert, kind);
+  const double J0 = eval_J(phi, system);
+  const int tmp = field_alloc(3);
+  double bestKappa = std::numeric_limits<double>::max();
+  int bestii = 0;
+  try {
+    for (int ii = 1; ii <= NK; ++ii) {
+      const double myexp = -7.0 - (double)NK / 2.0 + (double)ii;
+      const double eps = std::pow(10.0, myexp);
+      map(F_axpby<T>{fphys(phi), fphys(pert), fphys(tmp), (T)1, (T)eps}, nr_ * 3);
+      const double J1 = eval_J(tmp, system);
+      const double kappa = (J1 - J0) / (eps * innerProd);
+      if (std::max(std::fabs(kappa - 1.0), std::fabs(1.0 / kappa - 1.0)) <
+          std::max(std::fabs(bestKappa - 1.0), std::fabs(1.0 / kappa - 1.0))) {
+        bestKappa = kappa;
+        bestii = ii;
+      }
+      double* row = table + (ii - 1) * 5;
+      row[0] = eps; row[1] = innerProd; row[2] = J1 - J0; row[3] = kappa; row[4] = kappa;
+    }
+  } catch (...) {
+    field_free(tmp);
+    throw;
+  }
+  field_free(tmp);
+  int iAdjMin = 1;
+  double fracAdjMin = 9999.0;
+  for (int ii = 1; ii <= NK - (ADJ - 1) - 1; ++ii) {
+    double frac = 0.0;
+    for (int jj = 0; jj < ADJ; ++jj)
+      frac += std::fabs(1.0 - table[(ii + jj - 1) * 5 + 4] / table[(ii + jj) * 5 + 4]);
+    if (frac < fracAdjMin) { fracAdjMin = frac; iAdjMin = ii; }
+  }
+  double frac = 1.0;
+  for (int jj = 0; jj < ADJ; ++jj) frac *= std::fabs(table[(iAdjMin + jj - 1) * 5 + 4]);
+  frac = std::copysign(std::pow(frac, 1.0 / ADJ), table[(iAdjMin - 1) * 5 + 4]);
+  for (int ii = 0; ii < NK; ++ii) table[ii * 5 + 4] /= frac;
+  *frac_out = frac;
+  *best_kappa = bestKappa;
+  *best_row = bestii;
 }
 
 }  // namespace mlq

@@ -257,6 +257,28 @@ class MaxLq:
         return out
 
     # -- optimization: maxdLqdt ------------------------------------------------------------
+    def time_linesearch(self, hu, hd, taus, batch):
+        """J_LineMin at `taus` after one coefficient precompute, in launches of
+        `batch` step sizes (0: reference-structured); returns (J, t_precompute_ms,
+        t_eval_ms)."""
+        taus = np.ascontiguousarray(taus, dtype=np.float64)
+        J = np.empty(len(taus))
+        t = np.zeros(2)
+        self._chk(self.lib.mlq_time_linesearch(self.ctx, hu, hd, _p(taus), len(taus), int(batch),
+                                               _p(J), _p(t)))
+        return J, t[0], t[1]
+
+    def kappa_test(self, hphi, hgrad, hpert, mysystem="maxdLqdt", inner_product_space="L2"):
+        """kappa_test (optimization.f90:957-1129) on device fields; returns
+        (table[16][5], frac, best_kappa, best_row)."""
+        tab = np.zeros(80)
+        frac, best = C.c_double(), C.c_double()
+        row = C.c_int()
+        self._chk(self.lib.mlq_kappa_test(self.ctx, hphi, hgrad, hpert, _SYS[mysystem],
+                                          _IP[inner_product_space], _p(tab), C.byref(frac),
+                                          C.byref(best), C.byref(row)))
+        return tab.reshape(16, 5), frac.value, best.value, row.value
+
     def maxdLqdt(self, hu, max_iter=99999, optim_tol=None, start_iter=0, batch_linesearch=0,
                  reset_every=25, rolling=10, trace=True):
         """One RCG solve for the current B on the device-resident field ``hu``

@@ -836,6 +836,55 @@ def kappa_fd_check(orc: Oracle, u, grad, directions, h=1e-6):
     return out
 
 
+def kappa_best(kappas):
+    """Best row by the reference's criterion (optimization.f90:1035): row ii
+    replaces the best when max(|k-1|, |1/k-1|) < max(|best-1|, |1/k-1|),
+    both sides using the current row's 1/k; best starts at huge(pr).
+    Returns (best kappa, 1-based row)."""
+    best, bestii = np.finfo(np.float64).max, 0
+    for ii, k in enumerate(kappas, start=1):
+        if max(abs(k - 1.0), abs(1.0 / k - 1.0)) < max(abs(best - 1.0), abs(1.0 / k - 1.0)):
+            best, bestii = k, ii
+    return best, bestii
+
+
+def kappa_adjust(kappas, adj=4):
+    """Adjusted kappa (optimization.f90:1075-1097): the window of `adj`
+    consecutive rows (start ii = 1..n-adj) with the smallest sum of
+    |1 - k[i]/k[i+1]| gives frac = sign(k[start]) * (prod |k|)^(1/adj);
+    returns (kappas / frac, frac)."""
+    k = np.asarray(kappas, dtype=np.float64)
+    n = len(k)
+    imin, fmin = 0, 9999.0
+    for i in range(n - (adj - 1) - 1):
+        fr = sum(abs(1.0 - k[i + j] / k[i + j + 1]) for j in range(adj))
+        if fr < fmin:
+            fmin, imin = fr, i
+    frac = 1.0
+    for j in range(adj):
+        frac *= abs(k[imin + j])
+    frac = math.copysign(frac ** (1.0 / adj), k[imin])
+    return k / frac, frac
+
+
+def kappa_test(orc: Oracle, phi, gradJ, pert, mysystem="maxdLqdt", space=INNER_L2):
+    """kappa_test (optimization.f90:957-1129) with a given perturbation.
+    Returns (table[16][5] = {eps, <grad, pert>, J1 - J0, kappa, kappa adj},
+    frac, best kappa, best row)."""
+    nk = 16
+    ip = orc.global_summed_field_inner_product(gradJ, pert, space)
+    J0 = orc.eval_J(phi, mysystem)
+    tab = np.zeros((nk, 5))
+    for ii in range(1, nk + 1):
+        eps = 10.0 ** (-7.0 - nk / 2.0 + ii)
+        J1 = orc.eval_J(phi + eps * pert, mysystem)
+        kap = (J1 - J0) / (eps * ip)
+        tab[ii - 1] = (eps, ip, J1 - J0, kap, kap)
+    best, bestii = kappa_best(tab[:, 3])
+    tab[:, 4], frac = kappa_adjust(tab[:, 4])
+    return tab, frac, best, bestii
+
+
 # ---------------------------------------------------------------------------
 # Parity pins: oracle function -> the tests that check it against something
 # other than this file (golden vectors, closed forms, brute force, identities).
@@ -858,5 +907,6 @@ PINS = {
     "calcConstraintDerivativeL2": "tests/test_oracle_pins.py::test_constraint_derivative_fd",
     "vectorTransport": "tests/test_oracle_pins.py::test_vector_transport_tangency",
     "mnbrak/brent": "tests/test_oracle_linesearch.py",
+    "kappa_test/kappa_best/kappa_adjust": "tests/test_oracle_pins.py::test_kappa_*",
     "maxdLqdt": "tests/test_oracle_linesearch.py::test_rcg_monotone_and_sphere",
 }

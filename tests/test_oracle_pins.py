@@ -314,3 +314,46 @@ def test_vector_transport_tangency():
     n0 = (u * u).sum(axis=0) * u
     c = ip(n0, xi1) / np.mean(((u * u).sum(axis=0)) ** 2)
     assert rel(T0, xi1 - c * u) < 1e-11
+
+
+# ---- kappa test (optimization.f90:957-1129) ---------------------------------------
+def test_kappa_adjust_hand_case():
+    """The flattest 4-ratio window (rows 7..11, 1-based) is constant 2 -> frac 2;
+    a constant negative window gives a negative frac (sign of its first row)."""
+    from oracle.lq_oracle import kappa_adjust
+    k = np.array([50., -3., 9., 0.1, 7., 30., 2., 2., 2., 2., 2., 0.5, -8., 4., 60., 0.01])
+    adj, frac = kappa_adjust(k)
+    assert frac == 2.0
+    assert np.array_equal(adj, k / 2.0)
+    k2 = k.copy()
+    k2[6:11] = -3.0
+    adj2, frac2 = kappa_adjust(k2)
+    assert abs(frac2 + 3.0) < 1e-15
+    assert np.allclose(adj2[6:11], 1.0, rtol=1e-15, atol=0)
+
+
+def test_kappa_best_criterion_quirk():
+    """The reference compares max(|k-1|, |1/k-1|) against max(|best-1|, |1/k-1|)
+    (the current row's 1/k on both sides): 0.5 is kept over 1.8, which a
+    symmetric criterion would have preferred."""
+    from oracle.lq_oracle import kappa_best
+    assert kappa_best([0.5, 1.8]) == (0.5, 1)
+    assert kappa_best([10.0, 0.5, 1.2, 0.9, 1.05]) == (1.05, 5)
+
+
+def test_kappa_tends_to_one_for_exact_gradient():
+    """For band-limited fields GradL2ForLq is the exact L2 gradient of R, so
+    kappa -> 1 at moderate eps (a wrong sign or scale of the table fails)."""
+    from oracle.lq_oracle import kappa_test
+    n, q = 32, 4.0
+    orc = Oracle(Params(n=n, q=q))
+    u = orc.divAvg_free(bandlimit(seeded_field(n, 21, "gauss", 16.0, q=q), 5))
+    g = orc.GradL2ForLq(u)
+    pert = orc.divAvg_free(bandlimit(seeded_field(n, 1001, "gauss", 16.0, q=q), 5))
+    tab, frac, best, row = kappa_test(orc, u, g, pert, "maxdLqdt")
+    assert np.allclose(tab[:, 0], 10.0 ** (np.arange(1, 17) - 15.0), rtol=1e-15, atol=0)
+    for ii in range(5, 9):           # eps = 1e-9 .. 1e-6
+        assert abs(tab[ii, 3] - 1.0) < 1e-4, (tab[ii, 0], tab[ii, 3])
+    assert tab[13, 3] < 0.0          # eps = 0.1: far outside the linear range
+    assert abs(best - 1.0) < 1e-4 and 1 <= row <= 16
+    assert abs(frac - 1.0) < 1e-3
