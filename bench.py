@@ -200,14 +200,21 @@ def bench_continuation(args):
 def bench_n1024(args):
     """1024^3, q = 9, seed 5, exp(-|m|^2/1024), B = 2.2, mixed: one full RCG
     iteration; allocator high-water mark and device memory against 288 GB."""
+    from maxlq_amd.lq import MaxLq
     n, q, B = 1024, 9.0, 2.2
+    probe = MaxLq(n, q=q, B=B, precision="mixed")
+    resident = probe.uses_resident_state()
+    probe.close()
+    log(f"1024^3 q=9: {'resident-state' if resident else 'reference-structured'} iteration")
     g, h, res, t_it, _ = run_rcg(n, q, B, "mixed", 3, 0, 1, seed=5, a=1024.0)
     used, peak, free, total = g.mem_info()
     out = {"metric": "RCG iteration 1024^3 q=9 mixed: s/iter and memory peak",
            "value": t_it, "unit": "s/iter", "higher_is_better": False,
            "config": {"workload": "1024^3 q=9 B=2.2 seed 5 exp(-|m|^2/1024)", "J1": res.J1,
-                      "n_J": res.n_J, "mem_peak_GB": peak / 1e9, "mem_in_use_GB": used / 1e9,
-                      "device_total_GB": total / 1e9, "device_free_GB": free / 1e9}}
+                      "iteration": "resident-state" if resident else "reference-structured",
+                      "n_J": res.n_J, "mem_peak_GB": peak / 1e9, "mem_peak_GiB": peak / 2**30,
+                      "mem_in_use_GB": used / 1e9, "device_total_GiB": total / 2**30,
+                      "device_free_GB": free / 1e9}}
     print(json.dumps(out), flush=True)
 
 

@@ -160,6 +160,13 @@ int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* ta
 int mlq_kappa_test(mlq_ctx* ctx, mlq_field phi, mlq_field grad, mlq_field pert, int mysystem,
                    int inner_kind, double* table, double* frac, double* best_kappa, int* best_row);
 
+/* Which iteration mlq_rcg_solve and mlq_eval_J_batch will run with the
+ * current parameters: 1 the resident-state iteration (q >= 4 without the
+ * e_u variant, dealiased, and its buffers fit in the device memory left),
+ * 0 the reference-structured one (every operator recomputed from the field,
+ * a smaller footprint).  No counterpart in the reference. */
+int mlq_rcg_path(const mlq_ctx* ctx, int* resident_state);
+
 /* ---- the whole RCG solve for one B: maxdLqdt, optimization.f90:7-514 ---- */
 typedef struct mlq_rcg_cfg {
   int max_iter;          /* MAX_ITER */

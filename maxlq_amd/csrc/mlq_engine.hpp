@@ -83,6 +83,8 @@ class IEngine {
                           double* frac, double* best_kappa, int* best_row) = 0;
   virtual void seeded(int f_vec, unsigned long long seed, int kind, double a) = 0;
 
+  // whether rcg_solve / eval_J_batch take the resident-state iteration
+  virtual bool uses_fast_path() const = 0;
   // the RCG solve (optimization.f90:7-514)
   virtual void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res,
                          const LogSink& log) = 0;

@@ -24,7 +24,15 @@ template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
   }
   tw_ = (double2*)dmalloc(sizeof(double2) * n);
   MLQ_HIP(hipMemcpy(tw_, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
-  scratch_ = (C*)dmalloc(sizeof(C) * ns_ * MAXB);
+  // transform scratch: up to MAXB spectra, fewer when one spectrum is a
+  // sizeable share of the device (1024^3: 3 spectra instead of 12)
+  {
+    size_t fr = 0, tot = 0;
+    MLQ_HIP(hipMemGetInfo(&fr, &tot));
+    const double per = (double)sizeof(C) * (double)ns_;
+    sb_ = std::max(3, std::min(MAXB, (int)(0.06 * (double)tot / per)));
+  }
+  scratch_ = (C*)dmalloc(sizeof(C) * ns_ * sb_);
   part_ = (double*)dmalloc(sizeof(double) * RED_BLOCKS * RED_MAXK);
   red_dev_ = (double*)dmalloc(sizeof(double) * 64);
   MLQ_HIP(hipHostMalloc((void**)&red_host_, sizeof(double) * 64, hipHostMallocDefault));
@@ -49,6 +57,14 @@ template <typename T> Engine<T>::~Engine() {
   hipStreamDestroy(st_);
 }
 
+template <typename T> bool Engine<T>::fast_path_fits() const {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) { (void)hipGetLastError(); return true; }
+  double pooled = 0.0;
+  for (const auto& kv : pool_) pooled += (double)kv.first * (double)kv.second.size();
+  return fast_path_bytes() <= 0.95 * ((double)fr + pooled);
+}
+
 template <typename T> void Engine<T>::resolve_params() {
   if (p_.use_e_u < 0) use_e_u_ = p_.q < 4.0 - 2.0e-16;   // initialize.f90:79-85
   else use_e_u_ = p_.use_e_u != 0;
@@ -57,6 +73,15 @@ template <typename T> void Engine<T>::resolve_params() {
 template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // the pool caches freed temporaries by size; give them back and retry
+    (void)hipGetLastError();
+    MLQ_HIP(hipStreamSynchronize(st_));
+    for (auto& kv : pool_)
+      for (void* q : kv.second) dfree(q, kv.first);
+    pool_.clear();
+    e = hipMalloc(&p, bytes);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     throw Error(MLQ_EOOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
@@ -412,15 +437,15 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
 
 template <typename T>
 void Engine<T>::fwd(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {
-  for (int b0 = 0; b0 < nb; b0 += MAXB) {
-    const int k = std::min(MAXB, nb - b0);
+  for (int b0 = 0; b0 < nb; b0 += sb_) {
+    const int k = std::min(sb_, nb - b0);
     MLQ_DISPATCH_N(fwd_n, k, src + b0, dst + b0, ops + b0);
   }
 }
 template <typename T>
 void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {
-  for (int b0 = 0; b0 < nb; b0 += MAXB) {
-    const int k = std::min(MAXB, nb - b0);
+  for (int b0 = 0; b0 < nb; b0 += sb_) {
+    const int k = std::min(sb_, nb - b0);
     MLQ_DISPATCH_N(inv_n, k, src + b0, ops + b0, dst + b0, keep);
   }
 }
@@ -428,6 +453,11 @@ void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst,
 template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double order) {
   const int mc = mcut_of(order);
   if (mask_is_identity(mc)) return;   // identity mask: the round trip only adds round-off
+  // at most 3 spectra live at a time (bounded footprint at 1024^3)
+  if (nb > 3) {
+    for (int b0 = 0; b0 < nb; b0 += 3) dealias_dev(f + b0, std::min(3, nb - b0), order);
+    return;
+  }
   std::vector<ST> sp;
   std::vector<C*> dp;
   std::vector<SpecOp> ops(nb, sp_mask(mc));
@@ -443,7 +473,7 @@ template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double or
 }
 
 template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms) {
-  batch = std::max(1, std::min(batch, MAXB));
+  batch = std::max(1, std::min(batch, sb_));
   std::vector<RT> r;
   std::vector<ST> s;
   std::vector<T*> rp;
@@ -509,7 +539,7 @@ double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
 // candidates per pass: 4 tile shapes x {dispatch order, XCD-contiguous order}
 constexpr int NTILE = 8;
 template <typename T> void Engine<T>::autotune(int batch, double* times, int* choice) {
-  batch = std::max(1, std::min(batch, MAXB));
+  batch = std::max(1, std::min(batch, sb_));
   MLQ_HIP(hipMemsetAsync(scratch_, 0, sizeof(C) * ns_ * batch, st_));
   for (int pass = 0; pass < 4; ++pass) {
     double best = 1e300;

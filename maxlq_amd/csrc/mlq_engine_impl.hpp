@@ -170,6 +170,7 @@ template <typename T> class Engine final : public IEngine {
 
   // ---- state ----
   int n_, px_, pp_;   // grid size, n/2+1 modes along x, spectral row pitch
+  int sb_ = MAXB;     // spectra in the transform scratch (largest transform batch)
   // tile variant of the strided passes {y fwd, z fwd, z inv, y inv} (autotuned)
   int svar_[4] = {0, 0, 0, 0};
   template <int N> double time_pass_n(int pass, int var, int batch, int reps);
@@ -200,7 +201,21 @@ template <typename T> class Engine final : public IEngine {
   // the reference-structured solve (every operator recomputed from the field)
   void rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log);
   void eval_J_batch_ref(int u, int d, const double* taus, int ntau, double* J);
-  bool fast_path_ok() const { return !use_e_u_ && p_.q >= 4.0 - 1e-12 && p_.toDealias; }
+  // the resident-state iteration applies (q >= 4 without e_u, dealiased) and
+  // its buffers fit in the device memory left (free plus pooled temporaries)
+  bool fast_path_ok() const { return fast_path_applies() && fast_path_fits(); }
+  bool fast_path_applies() const { return !use_e_u_ && p_.q >= 4.0 - 1e-12 && p_.toDealias; }
+  bool fast_path_fits() const;
+  // device bytes of the resident-state iteration at its peak (inside the
+  // line-search precompute: 46 real fields and 27 half spectra at q = 4,
+  // a few more real fields for the |phi|^k chain at other q)
+  double fast_path_bytes() const {
+    const bool q4 = std::fabs(p_.q - 4.0) < 1e-14;
+    return (q4 ? 48.0 : 56.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+  }
+ public:
+  bool uses_fast_path() const override { return fast_path_ok(); }
+ private:
 
   template <typename U> friend class FastRcg;
   template <typename U>

@@ -265,8 +265,8 @@ template <typename T> class FastRcg {
 
   // ---- J_LineMin(tau) = -R(B phi/||phi||_q) for a batch of step sizes ---------------
   void eval_batch(const double* taus, int nb, double* J, double* Lout = nullptr) {
-    for (int b0 = 0; b0 < nb; b0 += MAXB) {
-      const int k = std::min(MAXB, nb - b0);
+    for (int b0 = 0; b0 < nb; b0 += e_.sb_) {
+      const int k = std::min(e_.sb_, nb - b0);
       if (q4_) eval_q4(taus + b0, k, J + b0, Lout ? Lout + b0 : nullptr);
       else
         for (int i = 0; i < k; ++i) eval_generic(taus[b0 + i], J + b0 + i, Lout ? Lout + b0 + i : nullptr);
@@ -560,7 +560,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   double J0 = e.eval_J(u, MLQ_J_MAXDLQDT);                         // :78
   double J1 = J0;
   FastRcg<T> F(e);
-  F.batch_ = cfg.batch_linesearch > 0 ? std::min(MAXB, std::max(1, cfg.batch_linesearch)) : 1;
+  F.batch_ = cfg.batch_linesearch > 0 ? std::min(e.sb_, std::max(1, cfg.batch_linesearch)) : 1;
   F.init_state(e.fphys(u));
   int iter = cfg.start_iter;
   const int start = cfg.start_iter;
@@ -789,7 +789,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
 template <typename T>
 void Engine<T>::ls_q4_sums(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1,
                            const C* Q2, const double* taus, int nb, int keep, double* out) {
-  if (nb < 1 || nb > MAXB || keep < 0 || keep >= n_ / 2)
+  if (nb < 1 || nb > sb_ || keep < 0 || keep >= n_ / 2)
     throw Error(MLQ_EINVAL, "ls_q4_sums: batch or mask out of range");
   MLQ_DISPATCH_N(ls_q4_sums_n, S0, S1, S2, Q0, Q1, Q2, taus, nb, keep, out);
 }

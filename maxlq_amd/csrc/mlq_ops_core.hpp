@@ -147,18 +147,16 @@ template <typename T> void Engine<T>::dLqdt_dev(const T* u, double q, double out
   out[2] = R3;
 }
 
-// L2 gradient of R: GradL2ForLq, function_ops.f90:1066-1211
+// L2 gradient of R: GradL2ForLq, function_ops.f90:1066-1211.  Written for a
+// small footprint: the velocity gradient (9 fields) lives only while the
+// pressure source is formed; the transposed-gradient term recomputes it three
+// columns at a time, and the dealiased products overwrite their inputs.
 template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
   const double q = p_.q, nu = p_.visc, vc = p_.viscCoefficient, pc = p_.pressureCoefficient;
   const bool D = p_.toDealias != 0;
   ST uh[3] = {stmp(), stmp(), stmp()};
   C* uhp[3] = {uh[0], uh[1], uh[2]};
   spec3(u, uhp);
-  RT Gb = rtmp(9);
-  T* Gp = Gb.get();
-  T* G[9];
-  for (int a = 0; a < 9; ++a) G[a] = Gp + a * nr_;
-  grad9(uhp, G);
   RT s = rtmp();
   map(F_dot3self<T>{u, nr_, s}, nr_);
   if (D) { T* f = s; dealias_dev(&f, 1, 2.0); }
@@ -166,25 +164,21 @@ template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
   chain_init(ch, u, s);
   const T* A = chain_pow(ch, q - 2.0);                          // :1102
   // pressure and its gradient                                       :1105-1109
-  RT Z = rtmp();
-  map(F_gradgradT<T>{Gp, nr_, Z}, nr_);
   ST ph = stmp();
   {
+    RT Gb = rtmp(9);
+    T* G[9];
+    for (int a = 0; a < 9; ++a) G[a] = Gb.get() + a * nr_;
+    grad9(uhp, G);
+    RT Z = rtmp();
+    map(F_gradgradT<T>{Gb.get(), nr_, Z}, nr_);
     SpecOp op;
     op.flags = (D ? SP_MASK : 0) | SP_INVKSQ;
     op.mcut = mcut_of(2.0);
     op.scale = -1.0;
     fwd1(Z, ph, op);
   }
-  RT wb = rtmp(21);   // gradP 3, DeltaU 3, aux3 3, gphi 3, aux5 3, aux7 3, e 3
-  T* work = wb.get();
-  T* gradP = work;
-  T* DeltaU = work + 3 * nr_;
-  T* aux3v = work + 6 * nr_;
-  T* gphi = work + 9 * nr_;
-  T* aux5 = work + 12 * nr_;
-  T* aux7 = work + 15 * nr_;
-  T* ev = work + 18 * nr_;
+  RT gradP = rtmp(3), DeltaU = rtmp(3);
   {
     InvSrc is[6];
     SpecOp ops[6];
@@ -199,41 +193,47 @@ template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
     }
     inv(6, is, ops, dst);
   }
-  RT aux2 = rtmp();
-  if (use_e_u_) {
-    map(F_unit<T>{u, s, nr_, ev}, nr_);                          // :1117
-    map(F_udotforce<T>{ev, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);
-    if (D && p_.dealiase_if_mult_by_e_u) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-    map(F_mul<T>{A, aux2, aux2}, nr_);
-    if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-    for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, ev + i * nr_, aux3v + i * nr_}, nr_);
-    if (D && p_.dealiase_if_mult_by_e_u) {
-      T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_};
-      dealias_dev(f, 3, 2.0);
+  ph = ST();
+  RT aux3v = rtmp(3);
+  {
+    RT aux2 = rtmp();
+    if (use_e_u_) {
+      RT eb = rtmp(3);
+      T* ev = eb;
+      map(F_unit<T>{u, s, nr_, ev}, nr_);                          // :1117
+      map(F_udotforce<T>{ev, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);
+      if (D && p_.dealiase_if_mult_by_e_u) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
+      map(F_mul<T>{A, aux2, aux2}, nr_);
+      if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
+      for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, ev + i * nr_, aux3v + i * nr_}, nr_);
+      if (D && p_.dealiase_if_mult_by_e_u) {
+        T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_};
+        dealias_dev(f, 3, 2.0);
+      }
+    } else {
+      map(F_udotforce<T>{u, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);  // :1131-1133
+      if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
+      const T* Bq = chain_pow(ch, q - 4.0);                        // :1136
+      map(F_mul<T>{Bq, aux2, aux2}, nr_);
+      if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
+      for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, u + i * nr_, aux3v + i * nr_}, nr_);
+      if (D) { T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_}; dealias_dev(f, 3, 2.0); }
     }
-  } else {
-    map(F_udotforce<T>{u, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);  // :1131-1133
-    if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-    const T* Bq = chain_pow(ch, q - 4.0);                        // :1136
-    map(F_mul<T>{Bq, aux2, aux2}, nr_);
-    if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-    for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, u + i * nr_, aux3v + i * nr_}, nr_);
-    if (D) { T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_}; dealias_dev(f, 3, 2.0); }
   }
-  // W_i = spectrum of D(|u|^(q-2) u_i)                               :1150-1153
-  ST W[3] = {stmp(), stmp(), stmp()};
-  C* Wp[3] = {W[0], W[1], W[2]};
+  // grad of Delta^-1 div(|u|^(q-2)u) and Delta of D(|u|^(q-2)u)       :1150-1158, 1172-1173
+  RT gphi = rtmp(3), aux7 = rtmp(3);
   {
-    RT t0 = rtmp(), t1 = rtmp(), t2 = rtmp();
-    T* tp[3] = {t0, t1, t2};
-    for (int i = 0; i < 3; ++i) map(F_mul<T>{A, u + i * nr_, tp[i]}, nr_);
-    SpecOp m = D ? sp_mask(mcut_of(2.0)) : sp_none();
-    SpecOp ops[3] = {m, m, m};
-    const T* src[3] = {tp[0], tp[1], tp[2]};
-    fwd(3, src, Wp, ops);
-  }
-  // grad of Delta^-1 div(|u|^(q-2)u)                                   :1155-1158
-  {
+    ST W[3] = {stmp(), stmp(), stmp()};
+    C* Wp[3] = {W[0], W[1], W[2]};
+    {
+      RT t0 = rtmp(3);
+      T* tp[3] = {t0, t0 + nr_, t0 + 2 * nr_};
+      for (int i = 0; i < 3; ++i) map(F_mul<T>{A, u + i * nr_, tp[i]}, nr_);
+      SpecOp m = D ? sp_mask(mcut_of(2.0)) : sp_none();
+      SpecOp ops[3] = {m, m, m};
+      const T* src[3] = {tp[0], tp[1], tp[2]};
+      fwd(3, src, Wp, ops);
+    }
     ST dv = stmp();
     smap(S_div<T>{Wp[0], Wp[1], Wp[2], dv, n_, pp_});
     InvSrc is[6];
@@ -244,28 +244,44 @@ template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
       SpecOp o; o.flags = SP_IK | SP_INVKSQ; o.dim = j; o.scale = 1.0;
       ops[j] = o;
       dst[j] = gphi + j * nr_;
-      is[3 + j] = InvSrc{{Wp[j], nullptr, nullptr}, {1.0, 0, 0}, 1};   // :1172-1173
+      is[3 + j] = InvSrc{{Wp[j], nullptr, nullptr}, {1.0, 0, 0}, 1};
       SpecOp l; l.flags = SP_NEGKSQ; ops[3 + j] = l;
       dst[3 + j] = aux7 + j * nr_;
     }
     inv(6, is, ops, dst);
   }
-  map(F_gradT_dot<T>{Gp, gphi, nr_, aux5}, nr_);                   // :1165-1169
+  // aux5_c = sum_a d_c u_a gphi_a (:1165-1169): column c of the gradient
+  // recomputed from the spectrum of u
+  RT aux5 = rtmp(3);
+  {
+    RT col = rtmp(3);
+    for (int c = 0; c < 3; ++c) {
+      InvSrc is[3];
+      SpecOp ops[3];
+      T* dst[3];
+      for (int a = 0; a < 3; ++a) {
+        is[a] = InvSrc{{uhp[a], nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[a] = sp_ik(c);
+        dst[a] = col + a * nr_;
+      }
+      inv(3, is, ops, dst);
+      map(F_dot3<T>{col, nr_, gphi, gphi + nr_, gphi + 2 * nr_, aux5 + c * nr_}, nr_);
+    }
+  }
   if (D) { T* f[3] = {aux5, aux5 + nr_, aux5 + 2 * nr_}; dealias_dev(f, 3, 2.0); }
   // aux4 = D(|u|^(q-2) grad p), aux6 = D(|u|^(q-2) Delta u)          :1178-1185
-  RT a4[3] = {rtmp(), rtmp(), rtmp()};
-  RT a6[3] = {rtmp(), rtmp(), rtmp()};
+  // (formed in place of grad p and Delta u)
   for (int i = 0; i < 3; ++i) {
-    map(F_mul<T>{A, gradP + i * nr_, a4[i]}, nr_);
-    map(F_mul<T>{A, DeltaU + i * nr_, a6[i]}, nr_);
+    map(F_mul<T>{A, gradP + i * nr_, gradP + i * nr_}, nr_);
+    map(F_mul<T>{A, DeltaU + i * nr_, DeltaU + i * nr_}, nr_);
   }
   if (D) {
-    T* f[6] = {a4[0], a4[1], a4[2], a6[0], a6[1], a6[2]};
+    T* f[6] = {gradP, gradP + nr_, gradP + 2 * nr_, DeltaU, DeltaU + nr_, DeltaU + 2 * nr_};
     dealias_dev(f, 6, 2.0);
   }
   for (int i = 0; i < 3; ++i)
-    map(F_gradcombine<T>{aux3v + i * nr_, a4[i], aux5 + i * nr_, a6[i], aux7 + i * nr_, Vout[i],
-                         (T)(q - 2.0), (T)pc, (T)(vc * nu)}, nr_);
+    map(F_gradcombine<T>{aux3v + i * nr_, gradP + i * nr_, aux5 + i * nr_, DeltaU + i * nr_,
+                         aux7 + i * nr_, Vout[i], (T)(q - 2.0), (T)pc, (T)(vc * nu)}, nr_);
 }
 
 // ---------------------------------------------------------------------------

@@ -19,14 +19,19 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   const double q = p_.q, B = p_.B;
   const long long n3 = nr_ * 3;
   // state fields (optimization.f90:15, 28-50); gradJ0, diff_gradJ and uvec0 are
-  // never read by the algorithm and are not kept
-  const int g1 = field_alloc(3), nH = field_alloc(3), gproj = field_alloc(3), gu1 = field_alloc(3),
-            gu0 = field_alloc(3), d1 = field_alloc(3), vg0 = field_alloc(3), vd0 = field_alloc(3),
-            tmp = field_alloc(3), eta = field_alloc(3), fs = field_alloc(1);
+  // never read by the algorithm and are not kept.  One buffer carries the
+  // gradient, then its tangential part, then the normalised gradient
+  // (gradJ1 -> gradJ1projected -> gradJused1, each the only later reader of
+  // the one before); gradJused0 enters only through its H_l norm, kept as a
+  // number; the step eta shares the scratch vector.
+  const int g1 = field_alloc(3), nH = field_alloc(3), d1 = field_alloc(3), vg0 = field_alloc(3),
+            vd0 = field_alloc(3), tmp = field_alloc(3), fs = field_alloc(1);
+  const int gproj = g1, gu1 = g1, eta = tmp;
+  double gu0_norm = 0.0;                                           // <gradJused0, gradJused0>_H_l
   struct Cleanup {
     Engine* e; std::vector<int> h;
     ~Cleanup() { for (int x : h) try { e->field_free(x); } catch (...) {} }
-  } cleanup{this, {g1, nH, gproj, gu1, gu0, d1, vg0, vd0, tmp, eta, fs}};
+  } cleanup{this, {g1, nH, d1, vg0, vd0, tmp, fs}};
   auto axpby = [&](int x, double a, int y, double b, int o) {
     map(F_axpby<T>{fphys(x), fphys(y), fphys(o), (T)a, (T)b}, n3);
   };
@@ -72,7 +77,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     // Polak-Ribiere with transported vectors (:226-240)
     axpby(gu1, 1.0, vg0, -1.0, tmp);
     double beta = inner(gu1, tmp, HL);
-    const double g0n = inner(gu0, gu0, HL);
+    const double g0n = gu0_norm;                                    // :227
     if (g0n < MACH_EPSILON || beta < MACH_EPSILON) beta = 0.0;
     else beta = beta / g0n;
     if (cfg.reset_every > 0 && iter % cfg.reset_every == 0) beta = 0.0;
@@ -149,7 +154,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :375
     }
     J0 = J1;                                                        // :432-436
-    axpby(gu1, 1.0, gu1, 0.0, gu0);
+    gu0_norm = inner(gu1, gu1, HL);
     tau0 = tau1;
     for (int i = nroll - 1; i > 0; --i) rel[i] = rel[i - 1];        // :439-445
     rel[0] = deltaJ;

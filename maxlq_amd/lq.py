@@ -257,6 +257,12 @@ class MaxLq:
         return out
 
     # -- optimization: maxdLqdt ------------------------------------------------------------
+    def uses_resident_state(self):
+        """True when mlq_rcg_solve runs the resident-state iteration."""
+        r = C.c_int()
+        self._chk(self.lib.mlq_rcg_path(self.ctx, C.byref(r)))
+        return bool(r.value)
+
     def time_linesearch(self, hu, hd, taus, batch):
         """J_LineMin at `taus` after one coefficient precompute, in launches of
         `batch` step sizes (0: reference-structured); returns (J, t_precompute_ms,
