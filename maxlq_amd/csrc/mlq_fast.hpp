@@ -101,18 +101,21 @@ template <typename T> class FastRcg {
     // grad p and Laplacian of u
     RT gp = e_.rtmp(3), lap = e_.rtmp(3);
     {
-      InvSrc is[6];
-      SpecOp ops[6];
-      T* dst[6];
+      // the pressure spectrum is masked (band-limited pass); u is not
+      InvSrc is[3], il[3];
+      SpecOp ops[3], opl[3];
+      T* dst[3];
+      T* dsl[3];
       for (int j = 0; j < 3; ++j) {
         is[j] = InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1};
         ops[j] = sp_ik(j);
         dst[j] = gp + j * nr_;
-        is[3 + j] = InvSrc{{uh_ + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
-        SpecOp l; l.flags = SP_NEGKSQ; ops[3 + j] = l;
-        dst[3 + j] = lap + j * nr_;
+        il[j] = InvSrc{{uh_ + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        SpecOp l; l.flags = SP_NEGKSQ; opl[j] = l;
+        dsl[j] = lap + j * nr_;
       }
-      e_.inv(6, is, ops, dst);
+      e_.inv(3, is, ops, dst, M_);
+      e_.inv(3, il, opl, dsl);
     }
     // aux2 = D(|u|^(q-4) D(u.(nu Lap u - grad p)))           (:1131-1138)
     RT aux2 = e_.rtmp();
@@ -135,58 +138,51 @@ template <typename T> class FastRcg {
         SpecOp o; o.flags = SP_IK | SP_INVKSQ; o.dim = j; ops[j] = o;
         dst[j] = gphi + j * nr_;
       }
-      e_.inv(3, is, ops, dst);
+      e_.inv(3, is, ops, dst, M_);   // div of the masked W: band-limited
     }
-    // V_hat = M FFT(Y) - nu |K|^2 W  (the final sum of GradL2ForLq, :1186-1190)
+    // V_hat = M FFT(Y); then, in one spectral pass, V_hat - nu |K|^2 W (the
+    // final sum of GradL2ForLq, :1186-1190), Hilbert + Leray of it (:150-153)
+    // and of the normal ||u||_q^(1-q) W (:162-169)
     ST Vh = e_.stmp(3);
     {
       RT Y = e_.rtmp(3);
       e_.map(F_grad_terms<T>{aux2, u_, A_, gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
                              (T)nuvc_, Y}, nr_);
       fwd3(Y, Vh, sp_mask(M_));
-      e_.smap(S_add_lap<T>{Vh, W_, ns_, n_, pp_, nuvc_});
     }
-    // Hilbert gradient + Leray (:150-153)
-    e_.smap(S_hilbert_leray<T>{Vh, Vh + ns_, Vh + 2 * ns_, n_, pp_, p_.lambda1, 2.0 * s, 1.0});
-    // normal: Leray(Hilbert(||u||_q^(1-q) W))                  (:162-169)
     ST nH = e_.stmp(3);
-    e_.smap(S_axpby3<T>{W_, nullptr, nH, ns_, std::pow(L_, 1.0 - q_), 0.0});
-    e_.smap(S_hilbert_leray<T>{nH, nH + ns_, nH + 2 * ns_, n_, pp_, p_.lambda1, 2.0 * s, 1.0});
-    const double nn = hl(nH, nH);
-    const double alpha = hl(nH, Vh) / nn;                        // :170-171
-    ST gproj = e_.stmp(3);
-    e_.smap(S_axpby3<T>{Vh, nH, gproj, ns_, 1.0, -alpha});       // :177
-    {
-      ST sum = e_.stmp(3);
-      e_.smap(S_axpby3<T>{gproj, nH, sum, ns_, 1.0, 1.0});
-      const double t1 = std::fabs(hl(gproj, nH) / hl(sum, sum)); // :180
-      if (t1 > 9.999999960041972e-13) out.msgs.push_back(42);
-    }
-    const double gg = hl(gproj, gproj);
-    {
-      double dn;
-      e_.template reduce<1>(R_divnorm<T>{gproj, ns_, n_, pp_}, ns_, &dn);
-      const double t2 = (dn / N6_) / gg;                          // :188-189
-      if (t2 > 1.0000000036274937e-15) out.msgs.push_back(43);
-    }
-    // normalised tangential gradient (:201-214)
-    const double c = std::sqrt(hl(uh_, uh_) / gg);
-    gu1_ = e_.stmp(3);
-    e_.smap(S_axpby3<T>{gproj, nullptr, gu1_, ns_, c, 0.0});
-    // Polak-Ribiere with transported vectors (:226-240)
+    e_.smap(S_dir_prep<T>{Vh, W_, nH, ns_, n_, pp_, p_.lambda1, 2.0 * s, nuvc_,
+                          std::pow(L_, 1.0 - q_)});
+    // every H_l product of the step in one pass; the tangential projection
+    // gproj = Vh - alpha nH (:177), the normalisation (:211-214) and the
+    // Polak-Ribiere numerator (:226) follow by bilinearity
+    double r[9];
+    e_.template reduce<9>(R_dir<T>{Vh, nH, uh_, have_cg_ ? vg0_.get() : nullptr, ns_, n_, pp_,
+                                   p_.lambda1, s}, ns_, r);
+    for (double& x : r) x /= N6_;
+    const double vv = r[0], vn = r[1], nn = r[2], uu = r[3], vg = r[4], ng = r[5];
+    const double alpha = vn / nn;                                 // :170-171
+    const double a1 = 1.0 - alpha;
+    const double t1 = std::fabs((vn - alpha * nn) / (vv + 2.0 * a1 * vn + a1 * a1 * nn));   // :180
+    if (t1 > 9.999999960041972e-13) out.msgs.push_back(42);
+    const double gg = vv - 2.0 * alpha * vn + alpha * alpha * nn;   // <gproj, gproj>
+    const double dn = r[6] - 2.0 * alpha * r[7] + alpha * alpha * r[8];
+    if (dn / gg > 1.0000000036274937e-15) out.msgs.push_back(43);   // :188-189
+    const double c = std::sqrt(uu / gg);                          // :211-214
+    const double gu1n = c * c * gg;                               // <gu1, gu1>
     double beta = 0.0;
-    if (have_cg_) {
-      ST diff = e_.stmp(3);
-      e_.smap(S_axpby3<T>{gu1_, vg0_, diff, ns_, 1.0, -1.0});
-      beta = hl(gu1_, diff);
-      const double g0n = hl(gu0_, gu0_);
+    if (have_cg_) {                                               // :226-232
+      beta = gu1n - c * (vg - alpha * ng);                        // <gu1, gu1 - vT_g0>
+      const double g0n = gu0n_;
       if (g0n < MACH_EPSILON || beta < MACH_EPSILON) beta = 0.0;
       else beta = beta / g0n;
     }
-    if (reset_every > 0 && iter % reset_every == 0) beta = 0.0;
+    if (reset_every > 0 && iter % reset_every == 0) beta = 0.0;   // :235
+    gu1_ = e_.stmp(3);
     dh_ = e_.stmp(3);
-    if (have_cg_) e_.smap(S_axpby3<T>{gu1_, vd0_, dh_, ns_, 1.0, beta});
-    else e_.smap(S_axpby3<T>{gu1_, nullptr, dh_, ns_, 1.0, 0.0});
+    e_.smap(S_dir_update<T>{Vh, nH, have_cg_ ? vd0_.get() : nullptr, gu1_, dh_, ns_, c, alpha,
+                            beta});                               // :240
+    gu1n_ = gu1n;
     // zero-mode checks (:251-261): the k = 0 coefficient is the unnormalised mean
     {
       double m0 = k0_abs_sum(dh_);
@@ -243,7 +239,7 @@ template <typename T> class FastRcg {
                       InvSrc{{P12h_ + ns_, nullptr, nullptr}, {1.0, 0, 0}, 1}};
       SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
       T* dst[3] = {P_, P_ + nr_, P_ + 2 * nr_};
-      e_.inv(3, is, ops, dst);
+      e_.inv(3, is, ops, dst, M_);   // pressure spectra are masked
     }
     // |grad phi|^2 coefficient spectra
     Q12h_ = e_.stmp(2);
@@ -335,7 +331,8 @@ template <typename T> class FastRcg {
     vd0_ = e_.stmp(3);
     e_.smap(S_axpby3<T>{gu1_, phih, vg0_, ns_, sc, -sc * Lmq * ir[0]});
     e_.smap(S_axpby3<T>{dh_, phih, vd0_, ns_, sc, -sc * Lmq * ir[1]});
-    gu0_ = std::move(gu1_);
+    gu0n_ = gu1n_;
+    gu1_ = ST();
     have_cg_ = true;
     // retraction and state update by homogeneity
     e_.map(F_scale<T>{phi, (T)sc}, nr_ * 3);
@@ -537,7 +534,8 @@ template <typename T> class FastRcg {
   ST uh_, ph_, W_, Q0h_;
   double L_ = 1.0;
   // CG state
-  ST gu0_, vg0_, vd0_, gu1_, dh_;
+  ST vg0_, vd0_, gu1_, dh_;
+  double gu0n_ = 0.0, gu1n_ = 0.0;   // <gradJused, gradJused>_H_l of the last two steps
   bool have_cg_ = false;
   // line-search coefficients
   RT d_, Gd_, S1_, S2_, b_, ee_, Q1_, Q2_, P_;

@@ -165,6 +165,119 @@ template <typename T> struct S_hilbert_leray {
   }
 };
 
+// Spectral part of the gradient step, fused: with P the Leray projection
+// (Nyquist-zeroed K', k = 0 removed, divAvg_free :2879-2909) and H the
+// Sobolev weight 1/(1 + (lam |K|)^e2s) (HilbertGradient :3611),
+//   V  <- P H (V - nu |K|^2 W)      (the Laplacian term of GradL2ForLq :1186-1190)
+//   nH <- P H (sW W)                (the normal, optimization.f90:162-169)
+// V, W, nH: three spectra each (stride ns).
+template <typename T> struct S_dir_prep {
+  c2<T>* V; const c2<T>* W; c2<T>* nH; long long ns; int n, px; double lam, e2s, nu, sW;
+  __device__ void operator()(long long i) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
+    const double ksq = K1 * K1 + K2 * K2 + K3 * K3;
+    if (ksq <= 2.0e-16) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { V[c * ns + i] = cmk<T>(0, 0); nH[c * ns + i] = cmk<T>(0, 0); }
+      return;
+    }
+    const double hw = 1.0 / (1.0 + pow(lam * sqrt(ksq), e2s));
+    const double e[3] = {mx == h ? 0.0 : K1, my == h ? 0.0 : K2, mz == h ? 0.0 : K3};
+    const double f = -nu * ksq;
+    double vr[3], vi[3], wr[3], wi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> v = V[c * ns + i], w = W[c * ns + i];
+      vr[c] = hw * ((double)v.x + f * (double)w.x);
+      vi[c] = hw * ((double)v.y + f * (double)w.y);
+      wr[c] = hw * (sW * (double)w.x);
+      wi[c] = hw * (sW * (double)w.y);
+    }
+    const double dvr = (e[0] * vr[0] + e[1] * vr[1] + e[2] * vr[2]) / ksq;
+    const double dvi = (e[0] * vi[0] + e[1] * vi[1] + e[2] * vi[2]) / ksq;
+    const double dwr = (e[0] * wr[0] + e[1] * wr[1] + e[2] * wr[2]) / ksq;
+    const double dwi = (e[0] * wi[0] + e[1] * wi[1] + e[2] * wi[2]) / ksq;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      V[c * ns + i] = cmk<T>((T)(vr[c] - e[c] * dvr), (T)(vi[c] - e[c] * dvi));
+      nH[c * ns + i] = cmk<T>((T)(wr[c] - e[c] * dwr), (T)(wi[c] - e[c] * dwi));
+    }
+  }
+};
+
+// Every inner product of the CG step in one pass over three-component
+// spectra (H_l weight m (1 + (lam |K|)^s)^2, half-spectrum multiplicity m):
+//   acc[0..5] = <V,V>, <V,N>, <N,N>, <U,U>, <V,G0>, <N,G0>  (H_l; G0 may be null)
+//   acc[6..8] = sum m |K'.V|^2, m Re(K'.V conj K'.N), m |K'.N|^2  (divergence)
+template <typename T> struct R_dir {
+  const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px;
+  double lam, s;
+  __device__ void operator()(long long i, double (&acc)[9]) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const double K = TWO_PI * sqrt((double)mx * mx + (double)my * my + (double)mz * mz);
+    const double w = 1.0 + pow(lam * K, s);
+    const double m = (kx == 0 || kx == h) ? 1.0 : 2.0;
+    const double mw = m * w * w;
+    const double e[3] = {mx == h ? 0.0 : TWO_PI * mx, my == h ? 0.0 : TWO_PI * my,
+                         mz == h ? 0.0 : TWO_PI * mz};
+    double vv = 0, vn = 0, nn = 0, uu = 0, vg = 0, ng = 0;
+    double dvr = 0, dvi = 0, dnr = 0, dni = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> v = V[c * ns + i], a = N[c * ns + i], u = U[c * ns + i];
+      const double vx = v.x, vy = v.y, ax = a.x, ay = a.y, ux = u.x, uy = u.y;
+      vv += vx * vx + vy * vy;
+      vn += vx * ax + vy * ay;
+      nn += ax * ax + ay * ay;
+      uu += ux * ux + uy * uy;
+      if (G0) {
+        const c2<T> g = G0[c * ns + i];
+        vg += vx * (double)g.x + vy * (double)g.y;
+        ng += ax * (double)g.x + ay * (double)g.y;
+      }
+      dvr += e[c] * vx; dvi += e[c] * vy;
+      dnr += e[c] * ax; dni += e[c] * ay;
+    }
+    acc[0] += mw * vv; acc[1] += mw * vn; acc[2] += mw * nn; acc[3] += mw * uu;
+    acc[4] += mw * vg; acc[5] += mw * ng;
+    acc[6] += m * (dvr * dvr + dvi * dvi);
+    acc[7] += m * (dvr * dnr + dvi * dni);
+    acc[8] += m * (dnr * dnr + dni * dni);
+  }
+};
+
+// gu1 = cg (V - alpha N); d = gu1 + beta D0 (D0 null: d = gu1)  (optimization.f90:177, 211-214, 240)
+template <typename T> struct S_dir_update {
+  const c2<T>* V; const c2<T>* N; const c2<T>* D0; c2<T>* gu1; c2<T>* d; long long ns;
+  double cg, alpha, beta;
+  __device__ void operator()(long long i) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> v = V[c * ns + i], a = N[c * ns + i];
+      const double gr = cg * ((double)v.x - alpha * (double)a.x);
+      const double gi = cg * ((double)v.y - alpha * (double)a.y);
+      gu1[c * ns + i] = cmk<T>((T)gr, (T)gi);
+      if (D0) {
+        const c2<T> q = D0[c * ns + i];
+        d[c * ns + i] = cmk<T>((T)(gr + beta * (double)q.x), (T)(gi + beta * (double)q.y));
+      } else {
+        d[c * ns + i] = cmk<T>((T)gr, (T)gi);
+      }
+    }
+  }
+};
+
 // o = a*x + b*y on three spectra at once (x, y, o: 3 comps, stride ns)
 template <typename T> struct S_axpby3 {
   const c2<T>* x; const c2<T>* y; c2<T>* o; long long ns; double a, b;
