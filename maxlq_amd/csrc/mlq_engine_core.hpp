@@ -240,7 +240,7 @@ void Engine<T>::reduce(const F& f, long long n, double* out) {
   prof_begin(PC_REDUCE);
   hipLaunchKernelGGL((k_reduce<K, F>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, f, n, part_);
   MLQ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st_, part_, RED_BLOCKS, K,
+  hipLaunchKernelGGL(k_reduce_final, dim3(K), dim3(RED_THREADS), 0, st_, part_, RED_BLOCKS, K,
                      red_dev_);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_REDUCE, 0);

@@ -5,20 +5,19 @@
 
 namespace mlq {
 
+// out[k] = sum_i part[i*K + k]; one workgroup per k (grid K), fixed order
 __global__ void k_reduce_final(const double* part, int nblk, int K, double* out) {
   __shared__ double sh[RED_THREADS];
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(long long)i * K + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = sh[0];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(long long)i * K + k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 template class Engine<double>;

@@ -158,6 +158,7 @@ template <typename T> class Engine final : public IEngine {
   };
   void chain_init(Chain& ch, const T* u, const T* s_pre = nullptr);
   const T* chain_pow(Chain& ch, double k);
+  RT chain_take(Chain& ch, double k);   // |u|^k, ownership moved out of the chain
   void spec3(const T* u, C* const* uh);                      // 3 forward transforms
   void grad9(C* const* uh, T* const* G);                     // G[i*3+j] = d_j u_i
   void dLqdt_dev(const T* u, double q, double out[3]);

@@ -308,8 +308,6 @@ template <typename T> class FastRcg {
     // phi = u + tau1 d, its spectrum and the |phi|^k chain
     RT phi = e_.rtmp(3);
     e_.map(F_axpby<T>{u_, d_, phi, (T)1, (T)tau1}, nr_ * 3);
-    ST phih = e_.stmp(3);
-    e_.smap(S_axpby3<T>{uh_, dh_, phih, ns_, 1.0, tau1});
     RT s1 = e_.rtmp();
     e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
     typename E_::Chain ch;
@@ -317,27 +315,31 @@ template <typename T> class FastRcg {
     const T* A1 = e_.chain_pow(ch, q - 2.0);
     const T* L1h = e_.chain_pow(ch, q / 2.0);
     const double L1 = std::pow(e_.dot(L1h, L1h) / N3_, 1.0 / q);
+    const double sc = B_ / L1;
+    const double Lmq = std::pow(L1, -q);
+    // spectrum of D(|phi|^(q-2) phi), already scaled to the retracted state
+    // (the normal spectrum W of u_new = sc phi is sc^(q-1) times it)
+    const double wsc = std::pow(sc, q - 1.0);
     ST Wphi = e_.stmp(3);
-    fwd_Au(A1, phi, Wphi);
+    fwd_Au(A1, phi, Wphi, wsc);
     // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1
     double ir[2];
     e_.template reduce<2>(R_kept_dot2<T>{Wphi, gu1_, dh_, ns_, n_, pp_, M_},
                           (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), ir);
-    ir[0] /= N6_;
-    ir[1] /= N6_;
-    const double sc = B_ / L1;
-    const double Lmq = std::pow(L1, -q);
+    ir[0] /= N6_ * wsc;
+    ir[1] /= N6_ * wsc;
+    // vector transport of g_used1 and d1 and the spectrum of the retracted u,
+    // all from phi_hat = u_hat + tau1 d_hat in one pass
     vg0_ = e_.stmp(3);
     vd0_ = e_.stmp(3);
-    e_.smap(S_axpby3<T>{gu1_, phih, vg0_, ns_, sc, -sc * Lmq * ir[0]});
-    e_.smap(S_axpby3<T>{dh_, phih, vd0_, ns_, sc, -sc * Lmq * ir[1]});
+    e_.smap(S_transport_update<T>{uh_, dh_, gu1_, vg0_, vd0_, ns_, tau1, sc, sc * Lmq * ir[0],
+                                  sc * Lmq * ir[1]});
     gu0n_ = gu1n_;
     gu1_ = ST();
     have_cg_ = true;
     // retraction and state update by homogeneity
     e_.map(F_scale<T>{phi, (T)sc}, nr_ * 3);
     std::swap(u_, phi);
-    e_.smap(S_axpby3<T>{phih, nullptr, uh_, ns_, sc, 0.0});
     e_.map(F_axpby<T>{G_, Gd_, G_, (T)sc, (T)(sc * tau1)}, nr_ * 9);
     e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
@@ -347,10 +349,9 @@ template <typename T> class FastRcg {
     e_.smap(S_quad<T>{ph_, P12h_, P12h_ + ns_, ph_, tau1, sc * sc});
     e_.smap(S_quad<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, tau1, sc * sc});
     e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
-    e_.smap(S_axpby3<T>{Wphi, nullptr, W_, ns_, std::pow(sc, q - 1.0), 0.0});
+    W_ = std::move(Wphi);
     // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
-    A_ = e_.rtmp();
-    e_.map(F_copy<T>{A1, A_}, nr_);
+    A_ = e_.chain_take(ch, q - 2.0);
     e_.map(F_scale<T>{A_, (T)std::pow(sc, q - 2.0)}, nr_);
     if (!q4_) {
       const T* Bq1 = e_.chain_pow(ch, q - 4.0);
@@ -411,10 +412,11 @@ template <typename T> class FastRcg {
     e_.dealias_dev(g, 3, 2.0);
   }
   // W_i = M FFT(A u_i)
-  void fwd_Au(const T* A, const T* u, C* W) {
+  // W_i = scale * M FFT(A u_i)
+  void fwd_Au(const T* A, const T* u, C* W, double scale = 1.0) {
     RT t = e_.rtmp(3);
     for (int i = 0; i < 3; ++i) e_.map(F_mul<T>{A, u + i * nr_, t + i * nr_}, nr_);
-    fwd3(t, W, sp_mask(M_));
+    fwd3(t, W, sp_mask(M_, scale));
   }
   double hl(const C* a, const C* b) {
     double r;
@@ -775,7 +777,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
     default: launch_zls_v<T, N, 0>(st_, la, ncols, (long long)N * PP, nb); break;
   }
   MLQ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st_, lspart_, nbx, 2 * nb,
+  hipLaunchKernelGGL(k_reduce_final, dim3(2 * nb), dim3(RED_THREADS), 0, st_, lspart_, nbx, 2 * nb,
                      red_dev_);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZFWD, 2.0 * nb * spec_bytes());

@@ -278,6 +278,25 @@ template <typename T> struct S_dir_update {
   }
 };
 
+// After the step tau1 (optimization.f90:326-346), with phi_hat = u_hat + tau1 d_hat:
+//   vg0 = sc gu1 - c0 phi_hat,  vd0 = sc d - c1 phi_hat   (vectorTransport :1746-1747)
+//   u_hat <- sc phi_hat                                   (the retraction)
+template <typename T> struct S_transport_update {
+  c2<T>* uh; const c2<T>* dh; const c2<T>* gu1; c2<T>* vg0; c2<T>* vd0; long long ns;
+  double tau, sc, c0, c1;
+  __device__ void operator()(long long i) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long k = c * ns + i;
+      const c2<T> u = uh[k], d = dh[k], g = gu1[k];
+      const double pr = (double)u.x + tau * (double)d.x, pi = (double)u.y + tau * (double)d.y;
+      vg0[k] = cmk<T>((T)(sc * (double)g.x - c0 * pr), (T)(sc * (double)g.y - c0 * pi));
+      vd0[k] = cmk<T>((T)(sc * (double)d.x - c1 * pr), (T)(sc * (double)d.y - c1 * pi));
+      uh[k] = cmk<T>((T)(sc * pr), (T)(sc * pi));
+    }
+  }
+};
+
 // o = a*x + b*y on three spectra at once (x, y, o: 3 comps, stride ns)
 template <typename T> struct S_axpby3 {
   const c2<T>* x; const c2<T>* y; c2<T>* o; long long ns; double a, b;

@@ -350,7 +350,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
 // FP64 reductions: per-block partial sums (wavefront shuffles + LDS), then a
 // single-block final sum; fixed block count, so results are deterministic.
 // ---------------------------------------------------------------------------
-constexpr int RED_BLOCKS = 1024;
+constexpr int RED_BLOCKS = 4096;   // 16 workgroups per CU: enough loads in flight
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAXK = 48;   // widest reduction: 2 sums per step size x 12 (+ spare)
 

@@ -58,6 +58,14 @@ template <typename T> const T* Engine<T>::chain_pow(Chain& ch, double k) {
   return r;
 }
 
+template <typename T> typename Engine<T>::RT Engine<T>::chain_take(Chain& ch, double k) {
+  chain_pow(ch, k);
+  auto it = ch.memo.find(std::llround(k * 1.0e6));
+  RT r = std::move(it->second);
+  ch.memo.erase(it);
+  return r;
+}
+
 template <typename T> void Engine<T>::spec3(const T* u, C* const* uh) {
   const T* src[3] = {u, u + nr_, u + 2 * nr_};
   SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
