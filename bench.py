@@ -131,6 +131,7 @@ def bench_linesearch8(args):
     Jb, tp, tb = g.time_linesearch(hu, hd, taus, 8)
     Js, tp1, ts = g.time_linesearch(hu, hd, taus, 1)
     Jr, _, tr = g.time_linesearch(hu, hd, taus, 0)
+    Jp, tpp, tpe = g.time_linesearch(hu, hd, taus, -1)
     used, peak, _, _ = g.mem_info()
     out = {"metric": "8-step line search at 512^3 q=4 mixed: batched vs sequential (ms)",
            "value": tb, "unit": "ms", "higher_is_better": False,
@@ -139,6 +140,8 @@ def bench_linesearch8(args):
                       "taus": list(taus), "precompute_ms": tp, "batched_ms": tb,
                       "sequential_ms": ts, "reference_structured_ms": tr,
                       "sequential_over_batched": ts / tb, "reference_over_batched": tr / tb,
+                      "polynomial_precompute_ms": tpp, "polynomial_eval_ms": tpe,
+                      "max_rel_dJ_polynomial_vs_batched": float(np.max(np.abs(Jp - Jb) / np.abs(Jb))),
                       "max_rel_dJ_batched_vs_sequential": float(np.max(np.abs(Jb - Js) / np.abs(Js))),
                       "max_rel_dJ_batched_vs_reference": float(np.max(np.abs(Jb - Jr) / np.abs(Jr))),
                       "mem_peak_GB": peak / 1e9}}
@@ -264,6 +267,8 @@ def main():
     log(f"mixed precision, batched line search: {n}^3 q={q} B={B}")
     gm, hm, resm, t_mixed, _ = run_rcg(n, q, B, "mixed", 3, args.warmup, args.steps)
     nJm = resm.n_J / max(resm.iters, 1)
+    log(f"mixed: {resm.n_J} J requested, {resm.n_J_computed} computed, {resm.n_J_poly} by the "
+        f"line-search polynomials")
     mem_used, mem_peak, _, _ = gm.mem_info()
     fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
@@ -324,6 +329,7 @@ def main():
                    "fp64_reference_structured_iter_per_s": (1.0 / t_ref) if t_ref else None,
                    "mixed_over_fp64_reference_structured": (t_ref / t_mixed) if t_ref else None,
                    "dR_over_R": dR, "n_J_per_iter_mixed": nJm, "mem_peak_GB_mixed": mem_peak / 1e9,
+                   "n_J_poly_fraction_mixed": resm.n_J_poly / max(resm.n_J_computed, 1),
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()}},
         "roofline": roof,

@@ -142,11 +142,13 @@ int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus,
 
 /* Measurement of the batched line search: one coefficient precompute for
  * phi = u + tau d, then J_LineMin at the ntau step sizes in launches of
- * `batch` step sizes each (1: one after the other; ntau: all in one batch;
- * 0: the reference-structured evaluation, every J recomputed from the field
- * as eval_J does).  t_ms[0] = precompute, t_ms[1] = evaluations (ms).  No
- * counterpart in the reference (its mnbrak/brent evaluate J one at a time,
- * optimization.f90:666-752). */
+ * `batch` step sizes each through the fused batched transform (1: one after
+ * the other; ntau: all in one batch); -1: the solver's own evaluation (the
+ * exact tau-polynomials wherever no grid point clips, the batched transform
+ * elsewhere); 0: the reference-structured evaluation, every J recomputed from
+ * the field as eval_J does.  t_ms[0] = precompute, t_ms[1] = evaluations (ms).
+ * No counterpart in the reference (its mnbrak/brent evaluate J one at a
+ * time, optimization.f90:666-752). */
 int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
                         int batch, double* J, double* t_ms);
 
@@ -191,6 +193,9 @@ typedef struct mlq_rcg_result {
   int n_J;               /* J evaluations requested by mnbrak+brent (all iterations) */
   int n_J_computed;      /* J evaluations computed (>= n_J with speculation) */
   double seconds;
+  int n_J_poly;          /* of n_J_computed, those given by the exact line-search
+                            polynomials (q = 4, no clipping of D(phi.phi)) */
+  int pad_;
 } mlq_rcg_result;
 
 /* log record kinds delivered to the callback (data_ops.f90 writers) */

@@ -60,7 +60,8 @@ class RcgCfg(C.Structure):
 class RcgResult(C.Structure):
     _fields_ = [("J1", C.c_double), ("iters", C.c_int), ("successful", C.c_int),
                 ("while_flag", C.c_int), ("stop_reason", C.c_int), ("n_J", C.c_int),
-                ("n_J_computed", C.c_int), ("seconds", C.c_double)]
+                ("n_J_computed", C.c_int), ("seconds", C.c_double), ("n_J_poly", C.c_int),
+                ("pad_", C.c_int)]
 
 
 LOG_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -345,7 +345,7 @@ int mlq_rcg_path(const mlq_ctx* ctx, int* resident_state) {
 int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
                         int batch, double* J, double* t_ms) {
   return guarded(ctx, [&] {
-    if (ntau < 1 || !taus || !J || !t_ms || batch < 0) throw Error(MLQ_EINVAL, "bad arguments");
+    if (ntau < 1 || !taus || !J || !t_ms || batch < -1) throw Error(MLQ_EINVAL, "bad arguments");
     ctx->eng->time_linesearch(u, d, taus, ntau, batch, J, t_ms);
   });
 }

@@ -52,6 +52,7 @@ template <typename T> Engine<T>::~Engine() {
   hipFree(scratch_);
   hipFree(part_);
   if (lspart_) hipFree(lspart_);
+  if (lsrisky_) hipFree(lsrisky_);
   hipFree(red_dev_);
   hipHostFree(red_host_);
   hipStreamDestroy(st_);

@@ -144,6 +144,10 @@ template <typename T> class Engine final : public IEngine {
   // Re(conj(A_b) Q(tau_b)); requires 0 <= keep < n/2 and nb <= MAXB
   void ls_q4_sums(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1, const C* Q2,
                   const double* taus, int nb, int keep, double* out);
+  // the points where max(S0 + t S1 + t^2 S2, 0) can clip for some t
+  // (k_ls_risky): count, and the (S0, S1, S2) list when it fits its capacity
+  void ls_risky(const T* S0, const T* S1, const T* S2, std::vector<double>& list,
+                long long& count);
   template <int N>
   void ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0, const C* Q1, const C* Q2,
                     const double* taus, int nb, int keep, double* out);
@@ -184,6 +188,8 @@ template <typename T> class Engine final : public IEngine {
   double* part_ = nullptr;
   double* lspart_ = nullptr;   // partial sums of the fused line-search z pass
   size_t lspart_bytes_ = 0;
+  double* lsrisky_ = nullptr;  // clipping-candidate list of the line search
+  size_t lsrisky_bytes_ = 0;
   double* red_dev_ = nullptr;
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;

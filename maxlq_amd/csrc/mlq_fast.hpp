@@ -46,6 +46,7 @@ template <typename T> class FastRcg {
     ns_ = e.ns_;
     M_ = e.mcut_of(2.0);
     q4_ = std::fabs(q_ - 4.0) < 1e-14;
+    poly_ok_ = q4_ && M_ < n_ / 2;
     N3_ = (double)nr_;
     N6_ = N3_ * N3_;
   }
@@ -60,7 +61,17 @@ template <typename T> class FastRcg {
     grad9(uh_, G_);
     s_ = e_.rtmp();
     e_.map(F_dot3self<T>{u_, nr_, s_}, nr_);
-    dealias1(s_);
+    if (poly_ok_) {
+      // D(u.u) with its masked spectrum kept for the line-search polynomials
+      S0h_ = e_.stmp();
+      e_.fwd1(s_, S0h_, sp_mask(M_));
+      InvSrc is{{S0h_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1};
+      SpecOp none = sp_none();
+      T* dst = s_;
+      e_.inv(1, &is, &none, &dst, M_);
+    } else {
+      dealias1(s_);
+    }
     chain_state_from_s();
     // pressure spectrum: M FFT(grad u : grad u^T) / |K|^2  (solve_poisson(-aux))
     ph_ = e_.stmp();
@@ -219,8 +230,22 @@ template <typename T> class FastRcg {
     Q1_ = e_.rtmp(); Q2_ = e_.rtmp();
     e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_},
            nr_);
-    // dealias the quadratic coefficients (batch of 8)
-    {
+    // dealias the quadratic coefficients; the masked spectra of S1, S2 are
+    // kept for the line-search polynomials
+    if (poly_ok_) {
+      S12h_ = e_.stmp(2);
+      const T* src[2] = {S1_, S2_};
+      C* sh[2] = {S12h_, S12h_ + ns_};
+      SpecOp ops[2] = {sp_mask(M_), sp_mask(M_)};
+      e_.fwd(2, src, sh, ops);
+      InvSrc is[2] = {InvSrc{{sh[0], nullptr, nullptr}, {1.0, 0, 0}, 1},
+                      InvSrc{{sh[1], nullptr, nullptr}, {1.0, 0, 0}, 1}};
+      SpecOp none[2] = {sp_none(), sp_none()};
+      T* dst[2] = {S1_, S2_};
+      e_.inv(2, is, none, dst, M_);
+      T* f[6] = {b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_};
+      e_.dealias_dev(f, 6, 2.0);
+    } else {
       T* f[8] = {S1_, S2_, b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_};
       e_.dealias_dev(f, 8, 2.0);
     }
@@ -255,6 +280,21 @@ template <typename T> class FastRcg {
                              nr_, acc);
       for (int k = 0; k < 5; ++k) cpoly_[k] = acc[k];
       for (int k = 0; k < 6; ++k) rpoly_[k] = acc[5 + k];
+      if (poly_ok_) {
+        double pa[10];
+        e_.template reduce<10>(R_ls_poly4<T>{S0h_, S12h_, S12h_ + ns_, Q0h_, Q12h_, Q12h_ + ns_,
+                                              n_, pp_, M_},
+                               (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), pa);
+        for (int k = 0; k < 5; ++k) { lpoly_[k] = pa[k]; qpoly_[k] = pa[5 + k]; }
+        e_.ls_risky(s_, S1_, S2_, risky_, nrisky_);
+        // k = 0 coefficients: sum over the grid of S0, S1, S2
+        C z[3];
+        MLQ_HIP(hipMemcpyAsync(&z[0], S0h_.get(), sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+        MLQ_HIP(hipMemcpyAsync(&z[1], S12h_.get(), sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+        MLQ_HIP(hipMemcpyAsync(&z[2], S12h_ + ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+        e_.sync();
+        for (int k = 0; k < 3; ++k) ssum_[k] = (double)z[k].x;
+      }
     }
     cache_.clear();
   }
@@ -348,6 +388,7 @@ template <typename T> class FastRcg {
                        (T)(sc * sc)}, nr_);
     e_.smap(S_quad<T>{ph_, P12h_, P12h_ + ns_, ph_, tau1, sc * sc});
     e_.smap(S_quad<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, tau1, sc * sc});
+    if (poly_ok_) e_.smap(S_quad<T>{S0h_, S12h_, S12h_ + ns_, S0h_, tau1, sc * sc});
     e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
     W_ = std::move(Wphi);
     // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
@@ -366,12 +407,13 @@ template <typename T> class FastRcg {
 
   void release_ls() {
     d_ = RT(); Gd_ = RT(); S1_ = RT(); S2_ = RT(); b_ = RT(); ee_ = RT(); Q1_ = RT(); Q2_ = RT();
-    P_ = RT(); P12h_ = ST(); Q12h_ = ST(); dh_ = ST();
+    P_ = RT(); P12h_ = ST(); Q12h_ = ST(); dh_ = ST(); S12h_ = ST();
   }
 
   double L() const { return L_; }
-  long long nJ_ = 0, nJc_ = 0;
+  long long nJ_ = 0, nJc_ = 0, nJpoly_ = 0;
   int batch_ = 1;
+  bool poly_enabled_ = true;   // false: every J through the batched transform
 
  private:
   // ---- helpers ---------------------------------------------------------------------
@@ -454,8 +496,28 @@ template <typename T> class FastRcg {
 
   void eval_q4(const double* taus, int nb, double* J, double* Lout) {
     double acc[2 * MAXB];
-    if (M_ < n_ / 2) {
-      e_.ls_q4_sums(s_, S1_, S2_, Q0h_, Q12h_, Q12h_ + ns_, taus, nb, M_, acc);
+    // step sizes where no grid point clips: the exact polynomials; the others
+    // in one fused batched transform
+    int ex[MAXB], nex = 0;
+    double tex[MAXB];
+    for (int b = 0; b < nb; ++b) {
+      const double t = taus[b];
+      if (poly_ok_ && poly_enabled_ && poly_exact(t)) {
+        double la = 0.0, lb = 0.0;
+        for (int k = 4; k >= 0; --k) { la = la * t + lpoly_[k]; lb = lb * t + qpoly_[k]; }
+        acc[2 * b] = la;
+        acc[2 * b + 1] = lb;
+        ++nJpoly_;
+      } else {
+        ex[nex] = b;
+        tex[nex++] = t;
+      }
+    }
+    if (nex == 0) {
+    } else if (M_ < n_ / 2) {
+      double ae[2 * MAXB];
+      e_.ls_q4_sums(s_, S1_, S2_, Q0h_, Q12h_, Q12h_ + ns_, tex, nex, M_, ae);
+      for (int i = 0; i < nex; ++i) { acc[2 * ex[i]] = ae[2 * i]; acc[2 * ex[i] + 1] = ae[2 * i + 1]; }
     } else {
       // the mask keeps every mode: unfused path (g^2 fields, transform, sums)
       std::vector<RT> gg;
@@ -530,7 +592,7 @@ template <typename T> class FastRcg {
   double q_, B_, nuvc_, pc_, N3_, N6_;
   int n_, pp_, M_;
   long long nr_, ns_;
-  bool q4_;
+  bool q4_, poly_ok_;
   // state
   RT u_, G_, s_, A_, Bq_, a_, Q0_;
   ST uh_, ph_, W_, Q0h_;
@@ -543,6 +605,28 @@ template <typename T> class FastRcg {
   RT d_, Gd_, S1_, S2_, b_, ee_, Q1_, Q2_, P_;
   ST P12h_, Q12h_;
   double cpoly_[5] = {0}, rpoly_[6] = {0};
+  // line-search polynomials of ||phi||_q^q and the R1 numerator (valid where
+  // poly_exact); masked spectra of D(u.u) (state) and of S1, S2
+  ST S0h_, S12h_;
+  double lpoly_[5] = {0}, qpoly_[5] = {0};
+  // points that can clip (S0, S1, S2 triples; nrisky_ of them, the list is
+  // empty when more than its capacity) and the grid sums of S0, S1, S2
+  std::vector<double> risky_;
+  long long nrisky_ = 0;
+  double ssum_[3] = {0, 0, 0};
+  // the polynomials are used at tau when the mass clipped by max(s(tau), 0)
+  // is negligible against sum s(tau): 1e-14 (FP64) or 1e-9 (FP32) relative
+  bool poly_exact(double t) const {
+    if (nrisky_ > 0 && risky_.empty()) return false;   // too many points to account for
+    double m = 0.0;
+    for (size_t i = 0; i < risky_.size(); i += 3) {
+      const double v = risky_[i] + t * (risky_[i + 1] + t * risky_[i + 2]);
+      if (v < 0.0) m -= v;
+    }
+    const double tot = ssum_[0] + t * (ssum_[1] + t * ssum_[2]);
+    const double tol = sizeof(T) == sizeof(double) ? 1.0e-14 : 1.0e-9;
+    return tot > 0.0 && m <= tol * tot;
+  }
   std::map<double, double> cache_;
 };
 
@@ -657,6 +741,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   res.stop_reason = stop;
   res.n_J = (int)F.nJ_;
   res.n_J_computed = (int)F.nJc_;
+  res.n_J_poly = (int)F.nJpoly_;
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -690,7 +775,7 @@ void Engine<T>::time_linesearch(int u, int d, const double* taus, int ntau, int 
     return std::chrono::duration<double, std::milli>(b - a).count();
   };
   sync();
-  if (batch <= 0 || !fast_path_ok()) {
+  if (batch == 0 || !fast_path_ok()) {
     const auto t0 = clk::now();
     eval_J_batch_ref(u, d, taus, ntau, J);
     sync();
@@ -701,6 +786,8 @@ void Engine<T>::time_linesearch(int u, int d, const double* taus, int ntau, int 
   FastRcg<T> F(*this);
   F.init_state(fphys(u));
   F.set_direction_phys(fphys(d));
+  F.poly_enabled_ = batch < 0;
+  if (batch < 0) batch = ntau;
   sync();
   const auto t0 = clk::now();
   F.precompute_ls();
@@ -784,6 +871,34 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));
   for (int k = 0; k < 2 * nb; ++k) out[k] = red_host_[k];
+}
+
+template <typename T>
+void Engine<T>::ls_risky(const T* S0, const T* S1, const T* S2, std::vector<double>& list,
+                         long long& count) {
+  constexpr unsigned CAP = 1u << 16;
+  const size_t need = sizeof(double) * 3 * CAP + 256;
+  if (need > lsrisky_bytes_) {
+    if (lsrisky_) dfree(lsrisky_, lsrisky_bytes_);
+    lsrisky_ = (double*)dmalloc(need);
+    lsrisky_bytes_ = need;
+  }
+  unsigned* cnt = (unsigned*)(lsrisky_ + 3 * CAP);
+  MLQ_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), st_));
+  prof_begin(PC_REDUCE);
+  hipLaunchKernelGGL((k_ls_risky<T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, S0, S1, S2, nr_,
+                     cnt, lsrisky_, CAP);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_REDUCE, 0);
+  unsigned c = 0;
+  MLQ_HIP(hipMemcpyAsync(&c, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+  MLQ_HIP(hipStreamSynchronize(st_));
+  count = c;
+  list.clear();
+  if (c > 0 && c <= CAP) {
+    list.resize(3 * (size_t)c);
+    MLQ_HIP(hipMemcpy(list.data(), lsrisky_, sizeof(double) * 3 * c, hipMemcpyDeviceToHost));
+  }
 }
 
 template <typename T>

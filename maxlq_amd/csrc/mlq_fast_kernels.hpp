@@ -457,6 +457,65 @@ template <typename T> struct R_kept_dot2 {
 };
 
 // ---------------------------------------------------------------------------
+// Line-search polynomials (q = 4).  Where g^2 = max(s(tau), 0) does not clip,
+// s(tau) = S0 + tau S1 + tau^2 S2 is band-limited and A(tau) = S0h + tau S1h
+// + tau^2 S2h (masked spectra), so both line-search sums are degree-4
+// polynomials in tau:
+//   acc[j]     : sum_kept m |A|^2             = sum_j acc[j] tau^j
+//   acc[5 + j] : sum_kept m Re(conj(A) Q)     = sum_j acc[5+j] tau^j
+// with Q = Q0 + tau Q1 + tau^2 Q2.  Kept-cube index space as R_ls_spec.
+template <typename T> struct R_ls_poly4 {
+  const c2<T>* S0; const c2<T>* S1; const c2<T>* S2;
+  const c2<T>* Q0; const c2<T>* Q1; const c2<T>* Q2; int n, px, M;
+  __device__ void operator()(long long i, double (&acc)[10]) const {
+    const int W = 2 * M + 1;
+    const int kx = (int)(i % (M + 1));
+    const long long r = i / (M + 1);
+    const int jy = (int)(r % W), jz = (int)(r / W);
+    const int my = jy - M, mz = jz - M;
+    const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
+    const long long off = ((long long)kz * n + ky) * px + kx;
+    const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
+    const c2<T> a0 = S0[off], a1 = S1[off], a2 = S2[off];
+    const c2<T> q0 = Q0[off], q1 = Q1[off], q2 = Q2[off];
+    auto re = [](c2<T> x, c2<T> y) { return (double)x.x * (double)y.x + (double)x.y * (double)y.y; };
+    acc[0] += m * re(a0, a0);
+    acc[1] += m * 2.0 * re(a0, a1);
+    acc[2] += m * (re(a1, a1) + 2.0 * re(a0, a2));
+    acc[3] += m * 2.0 * re(a1, a2);
+    acc[4] += m * re(a2, a2);
+    acc[5] += m * re(a0, q0);
+    acc[6] += m * (re(a0, q1) + re(a1, q0));
+    acc[7] += m * (re(a0, q2) + re(a1, q1) + re(a2, q0));
+    acc[8] += m * (re(a1, q2) + re(a2, q1));
+    acc[9] += m * re(a2, q2);
+  }
+};
+
+// Points where g^2 = max(s(tau), 0) can clip for some tau.  Where S0 > 0,
+// S2 > 0 and S1^2 <= 4 S0 S2 the quadratic s(tau) is positive for every tau
+// (for the undealiased products this is Cauchy-Schwarz); every other point
+// is appended to `list` as (S0, S1, S2) (at most cap entries; *count counts
+// all of them).
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_ls_risky(const T* S0, const T* S1, const T* S2,
+                                                           long long n, unsigned* count,
+                                                           double* list, unsigned cap) {
+  for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * RED_THREADS) {
+    const double s0 = S0[i], s1 = S1[i], s2 = S2[i];
+    if (!(s0 > 0.0 && s2 > 0.0 && s1 * s1 <= 4.0 * s0 * s2)) {
+      const unsigned k = atomicAdd(count, 1u);
+      if (k < cap) {
+        list[3 * (size_t)k] = s0;
+        list[3 * (size_t)k + 1] = s1;
+        list[3 * (size_t)k + 2] = s2;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fused q = 4 line-search sums.  For a batch of step sizes tau_b:
 //   A_b = M FFT(g_b^2),  g_b = sqrt(max(S0 + tau_b S1 + tau_b^2 S2, 0))
 //   out[2b]   = sum_kept m |A_b|^2                            (||phi||_q^q)

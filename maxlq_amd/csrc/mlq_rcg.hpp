@@ -172,6 +172,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   res.stop_reason = stop;
   res.n_J = (int)nJ;
   res.n_J_computed = (int)nJc;
+  res.n_J_poly = 0;
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -33,6 +33,7 @@ module maxlq_c
       real(c_double) :: J1
       integer(c_int) :: iters, successful, while_flag, stop_reason, n_J, n_J_computed
       real(c_double) :: seconds
+      integer(c_int) :: n_J_poly, pad_
    end type mlq_rcg_result
 
    interface

@@ -70,9 +70,12 @@ def test_linesearch_batched_vs_sequential(precision, tol):
     Jb, tp, tb = g.time_linesearch(hu, hd, taus, 8)
     Js, _, ts = g.time_linesearch(hu, hd, taus, 1)
     Jr, _, tr = g.time_linesearch(hu, hd, taus, 0)
-    print(precision, "batched", tb, "ms, sequential", ts, "ms, reference", tr, "ms, precompute", tp)
+    Jp, tpp, tpe = g.time_linesearch(hu, hd, taus, -1)   # polynomials where no point clips
+    print(precision, "batched", tb, "ms, sequential", ts, "ms, reference", tr, "ms, precompute", tp,
+          "polynomial", tpe, "ms after", tpp)
     assert np.allclose(Jb, Js, rtol=1e-14, atol=0)
     assert np.allclose(Jb, Jr, rtol=tol, atol=0)
+    assert np.allclose(Jp, Jb, rtol=1e-12 if precision == "fp64" else 1e-6, atol=0)
     if precision == "fp64":
         orc = L.Oracle(L.Params(n=n, q=q, B=B))
         u, d = g.download(hu), g.download(hd)
