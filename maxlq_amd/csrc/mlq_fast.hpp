@@ -351,9 +351,20 @@ template <typename T> class FastRcg {
     RT s1 = e_.rtmp();
     e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
     typename E_::Chain ch;
-    e_.chain_init(ch, nullptr, s1);
-    const T* A1 = e_.chain_pow(ch, q - 2.0);
-    const T* L1h = e_.chain_pow(ch, q / 2.0);
+    // q = 4 without clipping at tau1: |phi|^2 = D(D(g g) 1), g = sqrt(max(s1, 0)),
+    // is s1 itself (s1 is band-limited and positive; each dealias round trip
+    // returns its input up to rounding)
+    const bool direct = poly_ok_ && poly_exact(tau1);
+    const T* A1;
+    const T* L1h;
+    if (direct) {
+      A1 = s1;
+      L1h = s1;
+    } else {
+      e_.chain_init(ch, nullptr, s1);
+      A1 = e_.chain_pow(ch, q - 2.0);
+      L1h = e_.chain_pow(ch, q / 2.0);
+    }
     const double L1 = std::pow(e_.dot(L1h, L1h) / N3_, 1.0 / q);
     const double sc = B_ / L1;
     const double Lmq = std::pow(L1, -q);
@@ -392,8 +403,13 @@ template <typename T> class FastRcg {
     e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
     W_ = std::move(Wphi);
     // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
-    A_ = e_.chain_take(ch, q - 2.0);
-    e_.map(F_scale<T>{A_, (T)std::pow(sc, q - 2.0)}, nr_);
+    if (direct) {
+      A_ = e_.rtmp();
+      e_.map(F_copy<T>{s_, A_}, nr_);   // s_ = sc^2 s1 = |u_new|^2 already
+    } else {
+      A_ = e_.chain_take(ch, q - 2.0);
+      e_.map(F_scale<T>{A_, (T)std::pow(sc, q - 2.0)}, nr_);
+    }
     if (!q4_) {
       const T* Bq1 = e_.chain_pow(ch, q - 4.0);
       Bq_ = e_.rtmp();
