@@ -99,8 +99,10 @@ template <typename T> class FastRcg {
   // direction d given as a physical field (for batched J evaluations through
   // the C ABI)
   void set_direction_phys(const T* d) {
-    dh_ = e_.stmp(3);
-    spec3(d, dh_);
+    dB_ = e_.stmp(3);
+    spec3(d, dB_);
+    gam_ = 0.0;
+    dB_limited_ = false;   // an arbitrary field: full inverse transforms
   }
 
   // ---- one RCG iteration up to the CG direction (optimization.f90:146-261) --------
@@ -167,11 +169,13 @@ template <typename T> class FastRcg {
     // every H_l product of the step in one pass; the tangential projection
     // gproj = Vh - alpha nH (:177), the normalisation (:211-214) and the
     // Polak-Ribiere numerator (:226) follow by bilinearity
-    double r[9];
-    e_.template reduce<9>(R_dir<T>{Vh, nH, uh_, have_cg_ ? vg0_.get() : nullptr, ns_, n_, pp_,
-                                   p_.lambda1, s}, ns_, r);
+    double r[11];
+    e_.template reduce<11>(R_dir<T>{Vh, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_, pp_,
+                                    p_.lambda1, s}, ns_, r);
     for (double& x : r) x /= N6_;
-    const double vv = r[0], vn = r[1], nn = r[2], uu = r[3], vg = r[4], ng = r[5];
+    // transported gradient vT_g0 = vg_s gradJused0 + vg_u u (see transport_and_update)
+    const double vv = r[0], vn = r[1], nn = r[2], uu = r[3];
+    const double vg = vg_s_ * r[4] + vg_u_ * r[9], ng = vg_s_ * r[5] + vg_u_ * r[10];
     const double alpha = vn / nn;                                 // :170-171
     const double a1 = 1.0 - alpha;
     const double t1 = std::fabs((vn - alpha * nn) / (vv + 2.0 * a1 * vn + a1 * a1 * nn));   // :180
@@ -189,14 +193,19 @@ template <typename T> class FastRcg {
       else beta = beta / g0n;
     }
     if (reset_every > 0 && iter % reset_every == 0) beta = 0.0;   // :235
+    // d1 = gradJused1 + beta vT_d0 with vT_d0 = vd_B dB + vd_u u (:240): the
+    // band-limited part and the multiple of u are kept apart
     gu1_ = e_.stmp(3);
-    dh_ = e_.stmp(3);
-    e_.smap(S_dir_update<T>{Vh, nH, have_cg_ ? vd0_.get() : nullptr, gu1_, dh_, ns_, c, alpha,
-                            beta});                               // :240
+    ST dBn = e_.stmp(3);
+    e_.smap(S_dir_update<T>{Vh, nH, have_cg_ ? dB_.get() : nullptr, gu1_, dBn, ns_, c, alpha,
+                            beta * vd_B_});
+    gam_ = have_cg_ ? beta * vd_u_ : 0.0;
+    dB_ = std::move(dBn);
+    dB_limited_ = true;
     gu1n_ = gu1n;
     // zero-mode checks (:251-261): the k = 0 coefficient is the unnormalised mean
     {
-      double m0 = k0_abs_sum(dh_);
+      double m0 = k0_abs_sum(dB_, gam_);
       if (m0 > 1.0000000474974513e-03 * B_) out.msgs.push_back(44);
       m0 = k0_abs_sum(uh_);
       if (m0 > 1.0000000474974513e-03 * B_) out.msgs.push_back(44);
@@ -214,7 +223,7 @@ template <typename T> class FastRcg {
       SpecOp ops[12];
       T* dst[12];
       for (int i = 0; i < 3; ++i) {
-        is[i] = InvSrc{{dh_ + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        is[i] = InvSrc{{dB_ + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
         ops[i] = sp_none();
         dst[i] = d_ + i * nr_;
         for (int j = 0; j < 3; ++j) {
@@ -223,13 +232,13 @@ template <typename T> class FastRcg {
           dst[3 + i * 3 + j] = Gd_ + (i * 3 + j) * nr_;
         }
       }
-      e_.inv(12, is, ops, dst);
+      e_.inv(12, is, ops, dst, dB_limited_ ? M_ : -1);   // band-limited part of d
     }
     S1_ = e_.rtmp(); S2_ = e_.rtmp(); b_ = e_.rtmp(3); ee_ = e_.rtmp(3);
     RT X12 = e_.rtmp(2);
     Q1_ = e_.rtmp(); Q2_ = e_.rtmp();
-    e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_},
-           nr_);
+    e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_,
+                            (T)gam_}, nr_);
     // dealias the quadratic coefficients; the masked spectra of S1, S2 are
     // kept for the line-search polynomials
     if (poly_ok_) {
@@ -276,8 +285,8 @@ template <typename T> class FastRcg {
     }
     if (q4_) {
       double acc[11];
-      e_.template reduce<11>(R_ls_poly<T>{a_, b_, ee_, u_, d_, P_, P_ + nr_, P_ + 2 * nr_, nr_},
-                             nr_, acc);
+      e_.template reduce<11>(R_ls_poly<T>{a_, b_, ee_, u_, d_, P_, P_ + nr_, P_ + 2 * nr_, nr_,
+                                          gam_}, nr_, acc);
       for (int k = 0; k < 5; ++k) cpoly_[k] = acc[k];
       for (int k = 0; k < 6; ++k) rpoly_[k] = acc[5 + k];
       if (poly_ok_) {
@@ -347,7 +356,8 @@ template <typename T> class FastRcg {
     const double q = q_;
     // phi = u + tau1 d, its spectrum and the |phi|^k chain
     RT phi = e_.rtmp(3);
-    e_.map(F_axpby<T>{u_, d_, phi, (T)1, (T)tau1}, nr_ * 3);
+    const double opg = 1.0 + tau1 * gam_;   // phi = (1 + tau1 gam) u + tau1 dB
+    e_.map(F_axpby<T>{u_, d_, phi, (T)opg, (T)tau1}, nr_ * 3);
     RT s1 = e_.rtmp();
     e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
     typename E_::Chain ch;
@@ -373,25 +383,31 @@ template <typename T> class FastRcg {
     const double wsc = std::pow(sc, q - 1.0);
     ST Wphi = e_.stmp(3);
     fwd_Au(A1, phi, Wphi, wsc);
-    // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1
-    double ir[2];
-    e_.template reduce<2>(R_kept_dot2<T>{Wphi, gu1_, dh_, ns_, n_, pp_, M_},
+    // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u
+    double ir[3];
+    e_.template reduce<3>(R_kept_dot3<T>{Wphi, gu1_, dB_, uh_, ns_, n_, pp_, M_},
                           (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), ir);
-    ir[0] /= N6_ * wsc;
-    ir[1] /= N6_ * wsc;
-    // vector transport of g_used1 and d1 and the spectrum of the retracted u,
-    // all from phi_hat = u_hat + tau1 d_hat in one pass
-    vg0_ = e_.stmp(3);
-    vd0_ = e_.stmp(3);
-    e_.smap(S_transport_update<T>{uh_, dh_, gu1_, vg0_, vd0_, ns_, tau1, sc, sc * Lmq * ir[0],
-                                  sc * Lmq * ir[1]});
+    const double ir0 = ir[0] / (N6_ * wsc), ir1 = (ir[1] + gam_ * ir[2]) / (N6_ * wsc);
+    // vector transport Gamma(u, tau1 d1, xi) = sc xi - c_xi phi_hat (:1746-1747)
+    // with phi_hat = u_hat_new / sc, and the retraction u_hat_new = sc phi_hat:
+    //   vT_g0 = sc gradJused1 - (c0/sc) u_new
+    //   vT_d0 = sc dB / (1 + tau1 gam) + (gam / (1 + tau1 gam) - c1/sc) u_new
+    // (u_old = (u_new/sc - tau1 dB) / (1 + tau1 gam)); only scalars are kept
+    const double c0 = sc * Lmq * ir0, c1 = sc * Lmq * ir1;
+    if (std::fabs(opg) < 1.0e-8)
+      throw Error(MLQ_EUNSUP, "step along a direction dominated by -u/tau (1 + tau gam = 0)");
+    vg_s_ = sc;
+    vg_u_ = -c0 / sc;
+    vd_B_ = sc / opg;
+    vd_u_ = gam_ / opg - c1 / sc;
+    e_.smap(S_axpby3<T>{uh_, dB_, uh_, ns_, sc * opg, sc * tau1});
     gu0n_ = gu1n_;
-    gu1_ = ST();
+    gu0_ = std::move(gu1_);
     have_cg_ = true;
     // retraction and state update by homogeneity
     e_.map(F_scale<T>{phi, (T)sc}, nr_ * 3);
     std::swap(u_, phi);
-    e_.map(F_axpby<T>{G_, Gd_, G_, (T)sc, (T)(sc * tau1)}, nr_ * 9);
+    e_.map(F_axpby<T>{G_, Gd_, G_, (T)(sc * opg), (T)(sc * tau1)}, nr_ * 9);
     e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
     for (int j = 0; j < 3; ++j)
@@ -423,7 +439,7 @@ template <typename T> class FastRcg {
 
   void release_ls() {
     d_ = RT(); Gd_ = RT(); S1_ = RT(); S2_ = RT(); b_ = RT(); ee_ = RT(); Q1_ = RT(); Q2_ = RT();
-    P_ = RT(); P12h_ = ST(); Q12h_ = ST(); dh_ = ST(); S12h_ = ST();
+    P_ = RT(); P12h_ = ST(); Q12h_ = ST(); S12h_ = ST();
   }
 
   double L() const { return L_; }
@@ -481,12 +497,17 @@ template <typename T> class FastRcg {
     e_.template reduce<1>(R_specdot3<T>{a, b, ns_, n_, pp_, 1, p_.lambda1, 1.5 - 1.0 / q_}, ns_, &r);
     return r / N6_;
   }
-  double k0_abs_sum(const C* h) {
-    C v[3];
-    for (int c = 0; c < 3; ++c)
+  // sum_c |Re (h + g uh)_c(k = 0)|
+  double k0_abs_sum(const C* h, double g = 0.0) {
+    C v[3], w[3];
+    for (int c = 0; c < 3; ++c) {
       MLQ_HIP(hipMemcpyAsync(&v[c], h + c * ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+      MLQ_HIP(hipMemcpyAsync(&w[c], uh_ + c * ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
+    }
     e_.sync();
-    return std::fabs((double)v[0].x) + std::fabs((double)v[1].x) + std::fabs((double)v[2].x);
+    double r = 0.0;
+    for (int c = 0; c < 3; ++c) r += std::fabs((double)v[c].x + g * (double)w[c].x);
+    return r;
   }
   void chain_state_from_s() {
     typename E_::Chain ch;
@@ -595,8 +616,8 @@ template <typename T> class FastRcg {
     dealias1(Pd);
     // physical |grad phi|^2 coefficients: Q0 (state), Q1, Q2
     double acc[4];
-    e_.template reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, nr_, t},
-                          nr_, acc);
+    e_.template reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, nr_, t,
+                                          gam_}, nr_, acc);
     const double R1 = -nuvc_ * acc[0] / N3_;
     const double R2 = -nuvc_ * (q_ - 2.0) * acc[1] / N3_;
     const double R3 = pc_ * (q_ - 2.0) * acc[2] / N3_;
@@ -614,7 +635,13 @@ template <typename T> class FastRcg {
   ST uh_, ph_, W_, Q0h_;
   double L_ = 1.0;
   // CG state
-  ST vg0_, vd0_, gu1_, dh_;
+  // CG state: gradJused of this and the last step; the direction's band-limited
+  // part dB and its multiple of u (d1 = dB + gam u); the transported vectors
+  // as combinations vT_g0 = vg_s gu0 + vg_u u, vT_d0 = vd_B dB + vd_u u
+  ST gu0_, gu1_, dB_;
+  double gam_ = 0.0;
+  bool dB_limited_ = true;
+  double vg_s_ = 0.0, vg_u_ = 0.0, vd_B_ = 0.0, vd_u_ = 0.0;
   double gu0n_ = 0.0, gu1n_ = 0.0;   // <gradJused, gradJused>_H_l of the last two steps
   bool have_cg_ = false;
   // line-search coefficients

@@ -15,15 +15,18 @@ constexpr int RED_MAXK_BIG = 48;
 //   w_j(tau) = D(sum_i phi_i d_j phi_i)       -> b'_j, e'_j
 //   X(tau)   = sum_ij d_i phi_j d_j phi_i     -> X1, X2 (pressure source)
 //   Q(tau)   = |grad phi|^2                   -> Q1, Q2
+// The direction is d = dB + gam u (dB band-limited, gam a scalar) and its
+// gradient Gd = grad dB + gam G: d and Gd here hold the dB parts.
 template <typename T> struct F_ls_products {
   const T* u; const T* d; const T* G; const T* Gd; long long nr;
   T* S1; T* S2; T* b; T* e; T* X1; T* X2; T* Q1; T* Q2;   // b, e: 3 comps (stride nr)
+  T gam;
   __device__ void operator()(long long i) const {
     T uu[3], dd[3], g[9], h[9];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i]; }
+    for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i] + gam * uu[c]; }
 #pragma unroll
-    for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i]; }
+    for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i] + gam * g[a]; }
     S1[i] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
     S2[i] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
 #pragma unroll
@@ -59,13 +62,13 @@ template <typename T> struct F_ls_products {
 //   p(tau) * sum_j phi_j w_j, p = P0 + tau P1 + tau^2 P2   -> r0..r5
 template <typename T> struct R_ls_poly {
   const T* a; const T* b; const T* e; const T* u; const T* d;
-  const T* P0; const T* P1; const T* P2; long long nr;
+  const T* P0; const T* P1; const T* P2; long long nr; double gam;   // d + gam u
   __device__ void operator()(long long i, double (&acc)[11]) const {
     double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const double aj = a[j * nr + i], bj = b[j * nr + i], ej = e[j * nr + i];
-      const double uj = u[j * nr + i], dj = d[j * nr + i];
+      const double uj = u[j * nr + i], dj = (double)d[j * nr + i] + gam * uj;
       c0 += aj * aj;
       c1 += 2.0 * aj * bj;
       c2 += bj * bj + 2.0 * aj * ej;
@@ -215,10 +218,11 @@ template <typename T> struct S_dir_prep {
 // spectra (H_l weight m (1 + (lam |K|)^s)^2, half-spectrum multiplicity m):
 //   acc[0..5] = <V,V>, <V,N>, <N,N>, <U,U>, <V,G0>, <N,G0>  (H_l; G0 may be null)
 //   acc[6..8] = sum m |K'.V|^2, m Re(K'.V conj K'.N), m |K'.N|^2  (divergence)
+//   acc[9..10] = <V,U>, <N,U>  (H_l)
 template <typename T> struct R_dir {
   const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px;
   double lam, s;
-  __device__ void operator()(long long i, double (&acc)[9]) const {
+  __device__ void operator()(long long i, double (&acc)[11]) const {
     const long long row = i / px;
     const int kx = (int)(i - row * px);   // px: row pitch
     if (kx > (n >> 1)) return;             // row padding
@@ -231,7 +235,7 @@ template <typename T> struct R_dir {
     const double mw = m * w * w;
     const double e[3] = {mx == h ? 0.0 : TWO_PI * mx, my == h ? 0.0 : TWO_PI * my,
                          mz == h ? 0.0 : TWO_PI * mz};
-    double vv = 0, vn = 0, nn = 0, uu = 0, vg = 0, ng = 0;
+    double vv = 0, vn = 0, nn = 0, uu = 0, vg = 0, ng = 0, vu = 0, nu = 0;
     double dvr = 0, dvi = 0, dnr = 0, dni = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -241,6 +245,8 @@ template <typename T> struct R_dir {
       vn += vx * ax + vy * ay;
       nn += ax * ax + ay * ay;
       uu += ux * ux + uy * uy;
+      vu += vx * ux + vy * uy;
+      nu += ax * ux + ay * uy;
       if (G0) {
         const c2<T> g = G0[c * ns + i];
         vg += vx * (double)g.x + vy * (double)g.y;
@@ -254,6 +260,8 @@ template <typename T> struct R_dir {
     acc[6] += m * (dvr * dvr + dvi * dvi);
     acc[7] += m * (dvr * dnr + dvi * dni);
     acc[8] += m * (dnr * dnr + dni * dni);
+    acc[9] += mw * vu;
+    acc[10] += mw * nu;
   }
 };
 
@@ -274,25 +282,6 @@ template <typename T> struct S_dir_update {
       } else {
         d[c * ns + i] = cmk<T>((T)gr, (T)gi);
       }
-    }
-  }
-};
-
-// After the step tau1 (optimization.f90:326-346), with phi_hat = u_hat + tau1 d_hat:
-//   vg0 = sc gu1 - c0 phi_hat,  vd0 = sc d - c1 phi_hat   (vectorTransport :1746-1747)
-//   u_hat <- sc phi_hat                                   (the retraction)
-template <typename T> struct S_transport_update {
-  c2<T>* uh; const c2<T>* dh; const c2<T>* gu1; c2<T>* vg0; c2<T>* vd0; long long ns;
-  double tau, sc, c0, c1;
-  __device__ void operator()(long long i) const {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long k = c * ns + i;
-      const c2<T> u = uh[k], d = dh[k], g = gu1[k];
-      const double pr = (double)u.x + tau * (double)d.x, pi = (double)u.y + tau * (double)d.y;
-      vg0[k] = cmk<T>((T)(sc * (double)g.x - c0 * pr), (T)(sc * (double)g.y - c0 * pi));
-      vd0[k] = cmk<T>((T)(sc * (double)d.x - c1 * pr), (T)(sc * (double)d.y - c1 * pi));
-      uh[k] = cmk<T>((T)(sc * pr), (T)(sc * pi));
     }
   }
 };
@@ -409,7 +398,7 @@ template <typename T> struct R_ls_spec {
 template <typename T> struct R_ls_generic {
   const T* A; const T* Bw; const T* Pd; const T* Lh;
   const T* Q0; const T* Q1; const T* Q2; const T* a; const T* b; const T* e;
-  const T* u; const T* d; long long nr; double t;
+  const T* u; const T* d; long long nr; double t; double gam;   // d + gam u
   __device__ void operator()(long long i, double (&acc)[4]) const {
     const double q = (double)Q0[i] + t * ((double)Q1[i] + t * (double)Q2[i]);
     acc[0] += (double)A[i] * q;
@@ -418,7 +407,8 @@ template <typename T> struct R_ls_generic {
     for (int j = 0; j < 3; ++j) {
       const double wj = (double)a[j * nr + i] + t * ((double)b[j * nr + i] + t * (double)e[j * nr + i]);
       w2 += wj * wj;
-      pw += ((double)u[j * nr + i] + t * (double)d[j * nr + i]) * wj;
+      const double uj = u[j * nr + i];
+      pw += (uj + t * ((double)d[j * nr + i] + gam * uj)) * wj;
     }
     acc[1] += (Bw ? (double)Bw[i] : 1.0) * w2;
     acc[2] += (double)Pd[i] * pw;
@@ -435,10 +425,10 @@ template <typename T> struct F_pB {
 };
 
 // sum_c sum_k m Re(conj(W_c) xi_c) over three-component spectra restricted to
-// the kept cube (W masked), for two right-hand sides at once.
-template <typename T> struct R_kept_dot2 {
-  const c2<T>* W; const c2<T>* x; const c2<T>* y; long long ns; int n, px, M;
-  __device__ void operator()(long long i, double (&acc)[2]) const {
+// the kept cube (W masked), for three right-hand sides at once.
+template <typename T> struct R_kept_dot3 {
+  const c2<T>* W; const c2<T>* x; const c2<T>* y; const c2<T>* z; long long ns; int n, px, M;
+  __device__ void operator()(long long i, double (&acc)[3]) const {
     const int Wd = 2 * M + 1;
     const int kx = (int)(i % (M + 1));
     const long long r = i / (M + 1);
@@ -449,9 +439,11 @@ template <typename T> struct R_kept_dot2 {
     const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const c2<T> w = W[c * ns + off], a = x[c * ns + off], b = y[c * ns + off];
+      const c2<T> w = W[c * ns + off], a = x[c * ns + off], b = y[c * ns + off],
+                  zz = z[c * ns + off];
       acc[0] += m * ((double)w.x * a.x + (double)w.y * a.y);
       acc[1] += m * ((double)w.x * b.x + (double)w.y * b.y);
+      acc[2] += m * ((double)w.x * zz.x + (double)w.y * zz.y);
     }
   }
 };
