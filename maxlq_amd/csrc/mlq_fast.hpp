@@ -95,6 +95,8 @@ template <typename T> class FastRcg {
   }
 
   void download_u(T* dst) { e_.map(F_copy<T>{u_, dst}, nr_ * 3); }
+  // |u|^(q-2) of the state (at q = 4 without clipping it is D(u.u) = s_)
+  const T* Acur() const { return A_.get() ? A_.get() : s_.get(); }
 
   // direction d given as a physical field (for batched J evaluations through
   // the C ABI)
@@ -159,7 +161,7 @@ template <typename T> class FastRcg {
     ST Vh = e_.stmp(3);
     {
       RT Y = e_.rtmp(3);
-      e_.map(F_grad_terms<T>{aux2, u_, A_, gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+      e_.map(F_grad_terms<T>{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
                              (T)nuvc_, Y}, nr_);
       fwd3(Y, Vh, sp_mask(M_));
     }
@@ -354,10 +356,8 @@ template <typename T> class FastRcg {
   // Returns ||u + tau1 d||_q.
   double transport_and_update(double tau1) {
     const double q = q_;
-    // phi = u + tau1 d, its spectrum and the |phi|^k chain
-    RT phi = e_.rtmp(3);
-    const double opg = 1.0 + tau1 * gam_;   // phi = (1 + tau1 gam) u + tau1 dB
-    e_.map(F_axpby<T>{u_, d_, phi, (T)opg, (T)tau1}, nr_ * 3);
+    // phi = u + tau1 d1 = (1 + tau1 gam) u + tau1 dB and the |phi|^k chain
+    const double opg = 1.0 + tau1 * gam_;
     RT s1 = e_.rtmp();
     e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
     typename E_::Chain ch;
@@ -381,8 +381,15 @@ template <typename T> class FastRcg {
     // spectrum of D(|phi|^(q-2) phi), already scaled to the retracted state
     // (the normal spectrum W of u_new = sc phi is sc^(q-1) times it)
     const double wsc = std::pow(sc, q - 1.0);
+    // one pass over u, d: the transform input |phi|^(q-2) phi and the
+    // retracted field sc phi
     ST Wphi = e_.stmp(3);
-    fwd_Au(A1, phi, Wphi, wsc);
+    RT unew = e_.rtmp(3);
+    {
+      RT Au = e_.rtmp(3);
+      e_.map(F_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, Au, unew}, nr_);
+      fwd3(Au, Wphi, sp_mask(M_, wsc));
+    }
     // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u
     double ir[3];
     e_.template reduce<3>(R_kept_dot3<T>{Wphi, gu1_, dB_, uh_, ns_, n_, pp_, M_},
@@ -405,8 +412,7 @@ template <typename T> class FastRcg {
     gu0_ = std::move(gu1_);
     have_cg_ = true;
     // retraction and state update by homogeneity
-    e_.map(F_scale<T>{phi, (T)sc}, nr_ * 3);
-    std::swap(u_, phi);
+    u_ = std::move(unew);
     e_.map(F_axpby<T>{G_, Gd_, G_, (T)(sc * opg), (T)(sc * tau1)}, nr_ * 9);
     e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
@@ -416,12 +422,11 @@ template <typename T> class FastRcg {
     e_.smap(S_quad<T>{ph_, P12h_, P12h_ + ns_, ph_, tau1, sc * sc});
     e_.smap(S_quad<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, tau1, sc * sc});
     if (poly_ok_) e_.smap(S_quad<T>{S0h_, S12h_, S12h_ + ns_, S0h_, tau1, sc * sc});
-    e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
+    if (!q4_) e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
     W_ = std::move(Wphi);
     // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
     if (direct) {
-      A_ = e_.rtmp();
-      e_.map(F_copy<T>{s_, A_}, nr_);   // s_ = sc^2 s1 = |u_new|^2 already
+      A_ = RT();   // |u_new|^2 = sc^2 s1 is s_ itself
     } else {
       A_ = e_.chain_take(ch, q - 2.0);
       e_.map(F_scale<T>{A_, (T)std::pow(sc, q - 2.0)}, nr_);

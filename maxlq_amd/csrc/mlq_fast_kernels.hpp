@@ -61,6 +61,7 @@ template <typename T> struct F_ls_products {
 //   sum_j w_j(tau)^2 with w = a + tau b + tau^2 e          -> c0..c4
 //   p(tau) * sum_j phi_j w_j, p = P0 + tau P1 + tau^2 P2   -> r0..r5
 template <typename T> struct R_ls_poly {
+  static constexpr int unroll = 1;   // 18 input streams: enough loads in flight
   const T* a; const T* b; const T* e; const T* u; const T* d;
   const T* P0; const T* P1; const T* P2; long long nr; double gam;   // d + gam u
   __device__ void operator()(long long i, double (&acc)[11]) const {
@@ -103,6 +104,21 @@ template <typename T> struct F_ls_g {
       const T gv = s <= (T)0 ? (T)0 : sqrt(s);
       if (g[b]) g[b][i] = gv;
       gg[b][i] = gv * gv;
+    }
+  }
+};
+
+// phi_c = a u_c + b d_c: Au_c = A phi_c (the normal transform's input) and
+// un_c = sc phi_c (the retracted field), three components
+template <typename T> struct F_step {
+  const T* u; const T* d; const T* A; long long nr; T a, b, sc; T* Au; T* un;
+  __device__ void operator()(long long i) const {
+    const T w = A[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T phi = a * u[c * nr + i] + b * d[c * nr + i];
+      Au[c * nr + i] = w * phi;
+      un[c * nr + i] = sc * phi;
     }
   }
 };
@@ -220,6 +236,7 @@ template <typename T> struct S_dir_prep {
 //   acc[6..8] = sum m |K'.V|^2, m Re(K'.V conj K'.N), m |K'.N|^2  (divergence)
 //   acc[9..10] = <V,U>, <N,U>  (H_l)
 template <typename T> struct R_dir {
+  static constexpr int unroll = 1;   // one pow per mode: heavy in registers
   const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px;
   double lam, s;
   __device__ void operator()(long long i, double (&acc)[11]) const {
@@ -320,6 +337,7 @@ template <typename T> struct S_add_lap {
 // Weighted Parseval sums over three-component spectra:
 //   sum_c sum_k m(kx) w(k)^2 Re(a_c conj b_c), kind as R_specdot.
 template <typename T> struct R_specdot3 {
+  static constexpr int unroll = 1;
   const c2<T>* a; const c2<T>* b; long long ns; int n, px, kind; double lam, s;
   __device__ void operator()(long long i, double (&acc)[1]) const {
     const long long row = i / px;

@@ -9,6 +9,8 @@
 // (epilogue of the forward, prologue of the inverse), the 1/N^3 of fftbwd
 // (fftwfunction.f90:109) into the inverse x pass.
 #pragma once
+#include <type_traits>
+
 #include "mlq_fft.hpp"
 
 namespace mlq {
@@ -188,9 +190,9 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
     // load, no branch) and discard
     const bool in = kk <= kmax;
     const int ks = in ? kk : 0;
-    const c2<T> As = sp[ks], Bs = sp[PX + ks];
+    c2<T> As = sp[ks], Bs = sp[PX + ks];
+    if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
     double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
-    if (!in) { ax = 0.0; ay = 0.0; bx = 0.0; by = 0.0; }
     if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
     if (k > N / 2) { ay = -ay; by = -by; }
     v[m] = cmk<double>(ax - by, ay + bx);
@@ -376,14 +378,29 @@ template <int K> __device__ __forceinline__ void block_reduce_store(double (&acc
   }
 }
 
+// elements per grid-stride trip: 2 unless the functor says otherwise
+// (`static constexpr int unroll`, for functors heavy in registers)
+template <class F, class = void> struct RedUnroll { static constexpr int v = 2; };
+template <class F> struct RedUnroll<F, std::void_t<decltype(F::unroll)>> {
+  static constexpr int v = F::unroll;
+};
+
 template <int K, class F>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce(F f, long long n, double* part) {
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
-  for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n;
-       i += (long long)gridDim.x * RED_THREADS)
-    f(i, acc);
+  // two grid-stride elements per trip: their loads are independent and
+  // overlap (twice the bytes in flight per thread)
+  const long long st = (long long)gridDim.x * RED_THREADS;
+  long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x;
+  if constexpr (RedUnroll<F>::v >= 2) {
+    for (; i + st < n; i += 2 * st) {
+      f(i, acc);
+      f(i + st, acc);
+    }
+  }
+  for (; i < n; i += st) f(i, acc);
   block_reduce_store<K>(acc, part + (long long)blockIdx.x * K);
 }
 

@@ -185,6 +185,7 @@ template <typename T> struct R_nan {
 // with m = 1 on the kx = 0 and kx = N/2 planes and 2 elsewhere.
 // kind 0: w = 1; 1: w = 1 + (lam|K|)^s (H_l); 2: w = (1 + |K|)^s (H).
 template <typename T> struct R_specdot {
+  static constexpr int unroll = 1;
   const c2<T>* a; const c2<T>* b; int n, px, kind; double lam, s;
   __device__ void operator()(long long i, double (&acc)[1]) const {
     const long long row = i / px;
