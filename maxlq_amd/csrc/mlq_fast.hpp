@@ -49,7 +49,33 @@ template <typename T> class FastRcg {
     poly_ok_ = q4_ && M_ < n_ / 2;
     N3_ = (double)nr_;
     N6_ = N3_ * N3_;
+    nk_ = (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1);
+    // weights of |K| = 2 pi |m| by |m|^2: Hilbert 1/(1 + (lam |K|)^(2s)) and the
+    // squared H_l weight (1 + (lam |K|)^s)^2, s = 3/2 - 1/q (function_ops.f90:3611, 3449)
+    const int m2max = 3 * (n_ / 2) * (n_ / 2);
+    std::vector<double> hw(m2max + 1), hl(m2max + 1);
+    const double sx = 1.5 - 1.0 / q_, lam = p_.lambda1;
+    for (int m2 = 0; m2 <= m2max; ++m2) {
+      const double K = TWO_PI * std::sqrt((double)m2);
+      hw[m2] = 1.0 / (1.0 + std::pow(lam * K, 2.0 * sx));
+      const double w = 1.0 + std::pow(lam * K, sx);
+      hl[m2] = w * w;
+    }
+    tab_bytes_ = sizeof(double) * (size_t)(m2max + 1);
+    hwt_ = (double*)e_.dmalloc(tab_bytes_);
+    hlt_ = (double*)e_.dmalloc(tab_bytes_);
+    MLQ_HIP(hipMemcpy(hwt_, hw.data(), tab_bytes_, hipMemcpyHostToDevice));
+    MLQ_HIP(hipMemcpy(hlt_, hl.data(), tab_bytes_, hipMemcpyHostToDevice));
   }
+  ~FastRcg() {
+    try {
+      e_.dfree(hwt_, tab_bytes_);
+      e_.dfree(hlt_, tab_bytes_);
+    } catch (...) {
+    }
+  }
+  FastRcg(const FastRcg&) = delete;
+  FastRcg& operator=(const FastRcg&) = delete;
 
   // ---- state from a physical field ----------------------------------------------
   void init_state(const T* u_in) {
@@ -57,6 +83,12 @@ template <typename T> class FastRcg {
     e_.map(F_copy<T>{u_in, u_}, nr_ * 3);
     uh_ = e_.stmp(3);
     spec3(u_, uh_);
+    {
+      double r;   // <u_hat, u_hat>_H_l (the update with a = 1, b = 0 rewrites it unchanged)
+      e_.template reduce<1>(R_uh_update<T>{uh_, nullptr, ns_, n_, pp_, -1, true, 1.0, 0.0, hlt_},
+                            ns_, &r);
+      uu_ = r / N6_;
+    }
     G_ = e_.rtmp(9);
     grad9(uh_, G_);
     s_ = e_.rtmp();
@@ -144,7 +176,7 @@ template <typename T> class FastRcg {
     RT gphi = e_.rtmp(3);
     {
       ST dv = e_.stmp();
-      e_.smap(S_div<T>{W_, W_ + ns_, W_ + 2 * ns_, dv, n_, pp_});
+      e_.map(S_div_k<T>{W_, dv, ns_, n_, pp_, M_}, nk_, PC_SPECTRAL);
       InvSrc is[3];
       SpecOp ops[3];
       T* dst[3];
@@ -166,24 +198,26 @@ template <typename T> class FastRcg {
       fwd3(Y, Vh, sp_mask(M_));
     }
     ST nH = e_.stmp(3);
-    e_.smap(S_dir_prep<T>{Vh, W_, nH, ns_, n_, pp_, p_.lambda1, 2.0 * s, nuvc_,
-                          std::pow(L_, 1.0 - q_)});
+    e_.map(S_dir_prep_k<T>{Vh, W_, nH, ns_, n_, pp_, M_, hwt_, nuvc_, std::pow(L_, 1.0 - q_)},
+           nk_, PC_SPECTRAL);
     // every H_l product of the step in one pass; the tangential projection
     // gproj = Vh - alpha nH (:177), the normalisation (:211-214) and the
     // Polak-Ribiere numerator (:226) follow by bilinearity
-    double r[11];
-    e_.template reduce<11>(R_dir<T>{Vh, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_, pp_,
-                                    p_.lambda1, s}, ns_, r);
+    // (V, nH and gradJused0 are band-limited: the kept cube carries them; <u,u>
+    // comes from the last update of u_hat)
+    double r[10];
+    e_.template reduce<10>(R_dir_k<T>{Vh, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_, pp_,
+                                      M_, hlt_}, nk_, r);
     for (double& x : r) x /= N6_;
     // transported gradient vT_g0 = vg_s gradJused0 + vg_u u (see transport_and_update)
-    const double vv = r[0], vn = r[1], nn = r[2], uu = r[3];
-    const double vg = vg_s_ * r[4] + vg_u_ * r[9], ng = vg_s_ * r[5] + vg_u_ * r[10];
+    const double vv = r[0], vn = r[1], nn = r[2], uu = uu_;
+    const double vg = vg_s_ * r[3] + vg_u_ * r[5], ng = vg_s_ * r[4] + vg_u_ * r[6];
     const double alpha = vn / nn;                                 // :170-171
     const double a1 = 1.0 - alpha;
     const double t1 = std::fabs((vn - alpha * nn) / (vv + 2.0 * a1 * vn + a1 * a1 * nn));   // :180
     if (t1 > 9.999999960041972e-13) out.msgs.push_back(42);
     const double gg = vv - 2.0 * alpha * vn + alpha * alpha * nn;   // <gproj, gproj>
-    const double dn = r[6] - 2.0 * alpha * r[7] + alpha * alpha * r[8];
+    const double dn = r[7] - 2.0 * alpha * r[8] + alpha * alpha * r[9];
     if (dn / gg > 1.0000000036274937e-15) out.msgs.push_back(43);   // :188-189
     const double c = std::sqrt(uu / gg);                          // :211-214
     const double gu1n = c * c * gg;                               // <gu1, gu1>
@@ -199,8 +233,8 @@ template <typename T> class FastRcg {
     // band-limited part and the multiple of u are kept apart
     gu1_ = e_.stmp(3);
     ST dBn = e_.stmp(3);
-    e_.smap(S_dir_update<T>{Vh, nH, have_cg_ ? dB_.get() : nullptr, gu1_, dBn, ns_, c, alpha,
-                            beta * vd_B_});
+    e_.map(S_dir_update_k<T>{Vh, nH, have_cg_ ? dB_.get() : nullptr, gu1_, dBn, ns_, n_, pp_, M_,
+                             c, alpha, beta * vd_B_}, nk_, PC_SPECTRAL);
     gam_ = have_cg_ ? beta * vd_u_ : 0.0;
     dB_ = std::move(dBn);
     dB_limited_ = true;
@@ -407,7 +441,12 @@ template <typename T> class FastRcg {
     vg_u_ = -c0 / sc;
     vd_B_ = sc / opg;
     vd_u_ = gam_ / opg - c1 / sc;
-    e_.smap(S_axpby3<T>{uh_, dB_, uh_, ns_, sc * opg, sc * tau1});
+    {
+      double r;
+      e_.template reduce<1>(R_uh_update<T>{uh_, dB_, ns_, n_, pp_, M_, dB_limited_, sc * opg,
+                                           sc * tau1, hlt_}, ns_, &r);
+      uu_ = r / N6_;
+    }
     gu0n_ = gu1n_;
     gu0_ = std::move(gu1_);
     have_cg_ = true;
@@ -419,9 +458,13 @@ template <typename T> class FastRcg {
     for (int j = 0; j < 3; ++j)
       e_.map(F_quad<T>{a_ + j * nr_, b_ + j * nr_, ee_ + j * nr_, a_ + j * nr_, (T)tau1,
                        (T)(sc * sc)}, nr_);
-    e_.smap(S_quad<T>{ph_, P12h_, P12h_ + ns_, ph_, tau1, sc * sc});
-    e_.smap(S_quad<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, tau1, sc * sc});
-    if (poly_ok_) e_.smap(S_quad<T>{S0h_, S12h_, S12h_ + ns_, S0h_, tau1, sc * sc});
+    // masked spectra: only the kept cube changes
+    e_.map(S_quad_k<T>{ph_, P12h_, P12h_ + ns_, ph_, n_, pp_, M_, tau1, sc * sc}, nk_, PC_SPECTRAL);
+    e_.map(S_quad_k<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, n_, pp_, M_, tau1, sc * sc}, nk_,
+           PC_SPECTRAL);
+    if (poly_ok_)
+      e_.map(S_quad_k<T>{S0h_, S12h_, S12h_ + ns_, S0h_, n_, pp_, M_, tau1, sc * sc}, nk_,
+             PC_SPECTRAL);
     if (!q4_) e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
     W_ = std::move(Wphi);
     // |u|^(q-2), |u|^(q-4) and ||u||_q of the new state from the chain at tau1
@@ -635,6 +678,11 @@ template <typename T> class FastRcg {
   int n_, pp_, M_;
   long long nr_, ns_;
   bool q4_, poly_ok_;
+  long long nk_;                      // modes in the kept cube
+  double* hwt_ = nullptr;             // Hilbert weight by |m|^2
+  double* hlt_ = nullptr;             // squared H_l weight by |m|^2
+  size_t tab_bytes_ = 0;
+  double uu_ = 0.0;                   // <u_hat, u_hat>_H_l of the state
   // state
   RT u_, G_, s_, A_, Bq_, a_, Q0_;
   ST uh_, ph_, W_, Q0h_;

@@ -282,6 +282,182 @@ template <typename T> struct R_dir {
   }
 };
 
+// ---- kept-cube spectral kernels ----------------------------------------------------
+// Index space: the modes the 2/3 mask keeps, kx in [0, M], my, mz in [-M, M]
+// (kx fastest; M < n/2, so no Nyquist mode is inside).  Spectra produced by
+// masked transforms are zero outside this cube, and these kernels leave the
+// outside of their outputs untouched: their consumers read only the cube
+// (kept-cube kernels and the band-limited transform passes).  Weights of
+// |K| = 2 pi |m| come from tables indexed by |m|^2.
+struct KeptIdx { int kx, my, mz, m2; long long off; };
+__device__ __forceinline__ KeptIdx kept_decode(long long i, int n, int px, int M) {
+  const int W = 2 * M + 1;
+  KeptIdx k;
+  k.kx = (int)(i % (M + 1));
+  const long long r = i / (M + 1);
+  k.my = (int)(r % W) - M;
+  k.mz = (int)(r / W) - M;
+  const int ky = k.my < 0 ? k.my + n : k.my, kz = k.mz < 0 ? k.mz + n : k.mz;
+  k.off = ((long long)kz * n + ky) * px + k.kx;
+  k.m2 = k.kx * k.kx + k.my * k.my + k.mz * k.mz;
+  return k;
+}
+
+// S_dir_prep on the kept cube; hw[m2] = 1 / (1 + (lam |K|)^e2s)
+template <typename T> struct S_dir_prep_k {
+  c2<T>* V; const c2<T>* W; c2<T>* nH; long long ns; int n, px, M; const double* hw;
+  double nu, sW;
+  __device__ void operator()(long long i) const {
+    const KeptIdx k = kept_decode(i, n, px, M);
+    const long long o = k.off;
+    if (k.m2 == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { V[c * ns + o] = cmk<T>(0, 0); nH[c * ns + o] = cmk<T>(0, 0); }
+      return;
+    }
+    const double e[3] = {TWO_PI * k.kx, TWO_PI * k.my, TWO_PI * k.mz};
+    const double ksq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    const double h = hw[k.m2], f = -nu * ksq;
+    double vr[3], vi[3], wr[3], wi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> v = V[c * ns + o], w = W[c * ns + o];
+      vr[c] = h * ((double)v.x + f * (double)w.x);
+      vi[c] = h * ((double)v.y + f * (double)w.y);
+      wr[c] = h * (sW * (double)w.x);
+      wi[c] = h * (sW * (double)w.y);
+    }
+    const double dvr = (e[0] * vr[0] + e[1] * vr[1] + e[2] * vr[2]) / ksq;
+    const double dvi = (e[0] * vi[0] + e[1] * vi[1] + e[2] * vi[2]) / ksq;
+    const double dwr = (e[0] * wr[0] + e[1] * wr[1] + e[2] * wr[2]) / ksq;
+    const double dwi = (e[0] * wi[0] + e[1] * wi[1] + e[2] * wi[2]) / ksq;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      V[c * ns + o] = cmk<T>((T)(vr[c] - e[c] * dvr), (T)(vi[c] - e[c] * dvi));
+      nH[c * ns + o] = cmk<T>((T)(wr[c] - e[c] * dwr), (T)(wi[c] - e[c] * dwi));
+    }
+  }
+};
+
+// R_dir on the kept cube (V, N, G0 band-limited; U's kept part suffices for
+// <V,U>, <N,U>); hl[m2] = (1 + (lam |K|)^s)^2.  acc = {<V,V>, <V,N>, <N,N>,
+// <V,G0>, <N,G0>, <V,U>, <N,U>, |K.V|^2, Re(K.V conj K.N), |K.N|^2}
+template <typename T> struct R_dir_k {
+  const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px, M;
+  const double* hl;
+  __device__ void operator()(long long i, double (&acc)[10]) const {
+    const KeptIdx k = kept_decode(i, n, px, M);
+    const long long o = k.off;
+    const double m = k.kx == 0 ? 1.0 : 2.0;
+    const double mw = m * hl[k.m2];
+    const double e[3] = {TWO_PI * k.kx, TWO_PI * k.my, TWO_PI * k.mz};
+    double vv = 0, vn = 0, nn = 0, vg = 0, ng = 0, vu = 0, nu = 0;
+    double dvr = 0, dvi = 0, dnr = 0, dni = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> v = V[c * ns + o], a = N[c * ns + o], u = U[c * ns + o];
+      const double vx = v.x, vy = v.y, ax = a.x, ay = a.y, ux = u.x, uy = u.y;
+      vv += vx * vx + vy * vy;
+      vn += vx * ax + vy * ay;
+      nn += ax * ax + ay * ay;
+      vu += vx * ux + vy * uy;
+      nu += ax * ux + ay * uy;
+      if (G0) {
+        const c2<T> g = G0[c * ns + o];
+        vg += vx * (double)g.x + vy * (double)g.y;
+        ng += ax * (double)g.x + ay * (double)g.y;
+      }
+      dvr += e[c] * vx; dvi += e[c] * vy;
+      dnr += e[c] * ax; dni += e[c] * ay;
+    }
+    acc[0] += mw * vv; acc[1] += mw * vn; acc[2] += mw * nn;
+    acc[3] += mw * vg; acc[4] += mw * ng; acc[5] += mw * vu; acc[6] += mw * nu;
+    acc[7] += m * (dvr * dvr + dvi * dvi);
+    acc[8] += m * (dvr * dnr + dvi * dni);
+    acc[9] += m * (dnr * dnr + dni * dni);
+  }
+};
+
+// gu1 = cg (V - alpha N); d = gu1 + beta D0 on the kept cube
+template <typename T> struct S_dir_update_k {
+  const c2<T>* V; const c2<T>* N; const c2<T>* D0; c2<T>* gu1; c2<T>* d; long long ns; int n, px, M;
+  double cg, alpha, beta;
+  __device__ void operator()(long long i) const {
+    const long long o = kept_decode(i, n, px, M).off;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long k = c * ns + o;
+      const c2<T> v = V[k], a = N[k];
+      const double gr = cg * ((double)v.x - alpha * (double)a.x);
+      const double gi = cg * ((double)v.y - alpha * (double)a.y);
+      gu1[k] = cmk<T>((T)gr, (T)gi);
+      if (D0) {
+        const c2<T> q = D0[k];
+        d[k] = cmk<T>((T)(gr + beta * (double)q.x), (T)(gi + beta * (double)q.y));
+      } else {
+        d[k] = cmk<T>((T)gr, (T)gi);
+      }
+    }
+  }
+};
+
+// o = sc (x0 + t x1 + t^2 x2) on the kept cube (one spectrum)
+template <typename T> struct S_quad_k {
+  const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; int n, px, M; double t, sc;
+  __device__ void operator()(long long i) const {
+    const long long k = kept_decode(i, n, px, M).off;
+    const c2<T> a = x0[k], b = x1[k], c = x2[k];
+    o[k] = cmk<T>((T)(sc * (a.x + t * (b.x + t * c.x))), (T)(sc * (a.y + t * (b.y + t * c.y))));
+  }
+};
+
+// i K . W (divergence of a band-limited vector spectrum) on the kept cube
+template <typename T> struct S_div_k {
+  const c2<T>* W; c2<T>* o; long long ns; int n, px, M;
+  __device__ void operator()(long long i) const {
+    const KeptIdx k = kept_decode(i, n, px, M);
+    const long long q = k.off;
+    const double e1 = TWO_PI * k.kx, e2 = TWO_PI * k.my, e3 = TWO_PI * k.mz;
+    const c2<T> a = W[q], b = W[ns + q], c = W[2 * ns + q];
+    const double re = e1 * a.x + e2 * b.x + e3 * c.x;
+    const double im = e1 * a.y + e2 * b.y + e3 * c.y;
+    o[q] = cmk<T>((T)(-im), (T)re);
+  }
+};
+
+// u_hat <- a u_hat + b dB over the whole half spectrum (dB read only on the
+// kept cube when band-limited) and <u_hat, u_hat>_H_l of the result
+template <typename T> struct R_uh_update {
+  c2<T>* U; const c2<T>* D; long long ns; int n, px, M; bool dlim; double a, b;
+  const double* hl;
+  __device__ void operator()(long long i, double (&acc)[1]) const {
+    const long long row = i / px;
+    const int kx = (int)(i - row * px);   // px: row pitch
+    if (kx > (n >> 1)) return;             // row padding
+    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int h = n >> 1;
+    const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
+    const bool in = !dlim || (kx <= M && my <= M && my >= -M && mz <= M && mz >= -M);
+    const double m = (kx == 0 || kx == h) ? 1.0 : 2.0;
+    const double w = m * hl[kx * kx + my * my + mz * mz];
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> u = U[c * ns + i];
+      double ur = a * (double)u.x, ui = a * (double)u.y;
+      if (in) {
+        const c2<T> d = D[c * ns + i];
+        ur += b * (double)d.x;
+        ui += b * (double)d.y;
+      }
+      const c2<T> r = cmk<T>((T)ur, (T)ui);
+      U[c * ns + i] = r;
+      s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+    }
+    acc[0] += w * s;
+  }
+};
+
 // gu1 = cg (V - alpha N); d = gu1 + beta D0 (D0 null: d = gu1)  (optimization.f90:177, 211-214, 240)
 template <typename T> struct S_dir_update {
   const c2<T>* V; const c2<T>* N; const c2<T>* D0; c2<T>* gu1; c2<T>* d; long long ns;
