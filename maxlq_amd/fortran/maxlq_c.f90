@@ -254,6 +254,118 @@ module maxlq_c
          type(c_funptr), value :: cb
          type(c_ptr), value :: user
       end function
+      type(c_ptr) function mlq_last_error(ctx) bind(C, name="mlq_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mlq_sync(ctx) bind(C, name="mlq_sync")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mlq_calc_gk_order2(ctx, g, k, gk) bind(C, name="mlq_calc_gk_order2")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: g(*)
+         real(c_double), value :: k
+         real(c_double), intent(out) :: gk(*)
+      end function
+      integer(c_int) function mlq_inner_product(ctx, f, g, kind, res) bind(C, name="mlq_inner_product")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: f(*), g(*)
+         integer(c_int), value :: kind
+         real(c_double), intent(out) :: res
+      end function
+      integer(c_int) function mlq_calc_local_dLqdt_inclParts(ctx, u, q, res) &
+            bind(C, name="mlq_calc_local_dLqdt_inclParts")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), value :: q
+         real(c_double), intent(out) :: res(3)
+      end function
+      integer(c_int) function mlq_seeded_field(ctx, h, seed, kind, a) bind(C, name="mlq_seeded_field")
+         import :: c_ptr, c_int, c_double, c_long_long
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: h
+         integer(c_long_long), value :: seed
+         integer(c_int), value :: kind
+         real(c_double), value :: a
+      end function
+      integer(c_int) function mlq_eval_J_batch(ctx, u, d, taus, ntau, J) bind(C, name="mlq_eval_J_batch")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: u, d, ntau
+         real(c_double), intent(in) :: taus(*)
+         real(c_double), intent(out) :: J(*)
+      end function
+      integer(c_int) function mlq_time_linesearch(ctx, u, d, taus, ntau, batch, J, t_ms) &
+            bind(C, name="mlq_time_linesearch")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: u, d, ntau, batch
+         real(c_double), intent(in) :: taus(*)
+         real(c_double), intent(out) :: J(*), t_ms(2)
+      end function
+      integer(c_int) function mlq_kappa_test(ctx, phi, grad, pert, mysystem, inner_kind, table, &
+            frac, best_kappa, best_row) bind(C, name="mlq_kappa_test")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: phi, grad, pert, mysystem, inner_kind
+         real(c_double), intent(out) :: table(5, 16), frac, best_kappa
+         integer(c_int), intent(out) :: best_row
+      end function
+      integer(c_int) function mlq_rcg_path(ctx, resident_state) bind(C, name="mlq_rcg_path")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: resident_state
+      end function
+      integer(c_int) function mlq_time_fft(ctx, batch, reps, t_ms) bind(C, name="mlq_time_fft")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: batch, reps
+         real(c_double), intent(out) :: t_ms(6)
+      end function
+      integer(c_int) function mlq_time_copy(ctx, nfields, reps, t_ms) bind(C, name="mlq_time_copy")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nfields, reps
+         real(c_double), intent(out) :: t_ms
+      end function
+      integer(c_int) function mlq_autotune(ctx, batch, times, choice) bind(C, name="mlq_autotune")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: batch
+         real(c_double), intent(out) :: times(32)
+         integer(c_int), intent(out) :: choice(4)
+      end function
+      integer(c_int) function mlq_sizes(ctx, real_bytes, spec_bytes) bind(C, name="mlq_sizes")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: real_bytes, spec_bytes
+      end function
+      integer(c_int) function mlq_prof_enable(ctx, on) bind(C, name="mlq_prof_enable")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function
+      integer(c_int) function mlq_prof_get(ctx, cls, count, total_ms, bytes) bind(C, name="mlq_prof_get")
+         import :: c_ptr, c_int, c_double, c_long_long
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: cls
+         integer(c_long_long), intent(out) :: count
+         real(c_double), intent(out) :: total_ms, bytes
+      end function
+      type(c_ptr) function mlq_prof_name(cls) bind(C, name="mlq_prof_name")
+         import :: c_ptr, c_int
+         integer(c_int), value :: cls
+      end function
+      integer(c_int) function mlq_mem_info(ctx, used_bytes, peak_bytes, device_free, device_total) &
+            bind(C, name="mlq_mem_info")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: used_bytes, peak_bytes, device_free, device_total
+      end function
    end interface
 
 end module maxlq_c
