@@ -239,6 +239,18 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     kept = keep < 0 || (kx <= keep && mabs(ky, N) <= keep);
   }
   const bool ld_ok = valid && kept;
+  // z passes of a pruned launch: a tile with no kept column skips its
+  // transform (block-uniform test).  The inverse pass leaves such columns
+  // unwritten (the pruned y pass reads kept columns only); the forward pass
+  // stores the zeros of the masked spectrum.
+  if (MODE != 0 && keep >= 0 && !__syncthreads_or(ld_ok)) {
+    if (MODE == 2 && valid) {
+      c2<T>* d = a.dst[plane] + col;
+#pragma unroll
+      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+    }
+    return;
+  }
   auto rowin = [&](int n) { return DIR < 0 || keep < 0 || mabs(n, N) <= keep; };
   // rows outside the kept cube: the load is redirected to row 0 of the same
   // column (always kept, in bounds) and its value replaced by zero, so the E
