@@ -383,30 +383,99 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   }
   if (km >= 0 && (keep < 0 || km < keep)) keep = km;
   if (keep >= N / 2) keep = -1;
+  // Factored plan.  The i K_x and i K_y parts of a spectral factor commute
+  // with the z pass (i K_x also with the y pass), so they are applied in the
+  // x and y passes instead, and outputs that differ only in them share their
+  // z (and y) transforms: d and its 3 derivatives need 2 z and 3 y
+  // transforms instead of 4 each.
+  SpecOp zop[MAXB];
+  int fdim[MAXB];
+  for (int b = 0; b < nb; ++b) {
+    zop[b] = ops[b];
+    fdim[b] = -1;
+    if ((ops[b].flags & SP_IK) && ops[b].dim < 2) {
+      fdim[b] = ops[b].dim;
+      zop[b].flags &= ~SP_IK;
+      zop[b].dim = 0;
+    }
+  }
+  auto same_z = [&](int a, int b) {
+    const SpecOp &p = zop[a], &q = zop[b];
+    if (p.flags != q.flags || p.scale != q.scale) return false;
+    if ((p.flags & SP_MASK) && p.mcut != q.mcut) return false;
+    if ((p.flags & SP_IK) && p.dim != q.dim) return false;
+    if (src[a].ns != src[b].ns) return false;
+    for (int i = 0; i < src[a].ns; ++i)
+      if (src[a].s[i] != src[b].s[i] || src[a].c[i] != src[b].c[i]) return false;
+    return true;
+  };
+  int zrep[MAXB], zid[MAXB], nz = 0;
+  for (int b = 0; b < nb; ++b) {
+    int j = 0;
+    while (j < nz && !same_z(zrep[j], b)) ++j;
+    if (j == nz) zrep[nz++] = b;
+    zid[b] = j;
+  }
+  // y transforms: one per (z transform, i K_y or not); the first of each z
+  // transform runs in place, the others (launched first) to new spectra
+  int yz[MAXB], yk[MAXB], yslot[MAXB], yid[MAXB], ny = 0, next = nz;
+  for (int b = 0; b < nb; ++b) {
+    const int k = fdim[b] == 1;
+    int p = 0;
+    while (p < ny && !(yz[p] == zid[b] && yk[p] == k)) ++p;
+    if (p == ny) {
+      bool first = true;
+      for (int r = 0; r < ny; ++r) first = first && yz[r] != zid[b];
+      yz[ny] = zid[b];
+      yk[ny] = k;
+      yslot[ny] = first ? zid[b] : next++;
+      ++ny;
+    }
+    yid[b] = p;
+  }
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
   sa.sstride = ns_;
   sa.keep = keep;
   double zin = 0;
-  for (int b = 0; b < nb; ++b) {
-    sa.nsrc[b] = src[b].ns;
-    for (int i = 0; i < src[b].ns; ++i) { sa.src[b][i] = src[b].s[i]; sa.coef[b][i] = src[b].c[i]; }
-    sa.op[b] = ops[b];
+  for (int j = 0; j < nz; ++j) {
+    const int b = zrep[j];
+    sa.nsrc[j] = src[b].ns;
+    for (int i = 0; i < src[b].ns; ++i) { sa.src[j][i] = src[b].s[i]; sa.coef[j][i] = src[b].c[i]; }
+    sa.op[j] = zop[b];
     zin += src[b].ns;
   }
-  const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
+  const double rb = (double)nb * real_bytes(), sb = (double)spec_bytes();
   prof_begin(PC_ZINV);
-  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nb);
+  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nz);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_ZINV, sb + zin * spec_bytes());
+  prof_end(PC_ZINV, nz * sb + zin * spec_bytes());
   prof_begin(PC_YINV);
-  launch_strided<T, N, 1, 0>(svar_[3], st_, sa, keep >= 0 ? keep + 1 : PX, (long long)PP,
-                             (long long)N * PP, N * nb);
-  MLQ_HIP(hipGetLastError());
-  prof_end(PC_YINV, 2 * sb);
+  for (int pass = 0; pass < 2; ++pass) {   // out of place first, then in place
+    SArgs<T> ya = sa;
+    ya.ymap = 1;
+    int nf = 0;
+    for (int p = 0; p < ny; ++p) {
+      if ((yslot[p] == yz[p]) != (pass == 1)) continue;
+      ya.yin[nf] = yz[p];
+      ya.yout[nf] = yslot[p];
+      ya.yik[nf] = yk[p];
+      ++nf;
+    }
+    if (nf == 0) continue;
+    launch_strided<T, N, 1, 0>(svar_[3], st_, ya, keep >= 0 ? keep + 1 : PX, (long long)PP,
+                               (long long)N * PP, N * nf);
+    MLQ_HIP(hipGetLastError());
+  }
+  prof_end(PC_YINV, 2 * ny * sb);
   XArgs<T> xa{};
-  for (int b = 0; b < nb; ++b) xa.rdst[b] = dst[b];
+  for (int b = 0; b < nb; ++b) {
+    xa.rdst[b] = dst[b];
+    xa.xin[b] = yslot[yid[b]];
+    xa.xik[b] = fdim[b] == 0;
+  }
+  xa.xmap = 1;
   xa.spec = scratch_;
   xa.tw = tw_;
   xa.sstride = ns_;
@@ -420,7 +489,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   else
     hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_XINV, rb + sb);
+  prof_end(PC_XINV, rb + nb * sb);
 }
 
 #define MLQ_DISPATCH_N(FN, ...)                                              \

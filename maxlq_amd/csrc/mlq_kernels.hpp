@@ -59,6 +59,11 @@ template <typename T> struct XArgs {
   int nb;
   int keep = -1;
   double scale;
+  // xmap: output b reads scratch spectrum xin[b] (else b) and, with xik[b],
+  // multiplies it by i K_x first (the x part of a factored derivative)
+  int xmap = 0;
+  int xin[MAXB];
+  int xik[MAXB];
 };
 
 template <typename T> struct SArgs {
@@ -72,6 +77,13 @@ template <typename T> struct SArgs {
   long long sstride;
   int keep = -1;
   int remap = 0;   // 1: XCD-contiguous tile order (xcd_tile)
+  // ymap (y passes): field f of the launch reads scratch spectrum yin[f],
+  // writes yout[f] and, with yik[f], multiplies its input by i K_y (the y
+  // part of a factored derivative); else in place on spectrum f
+  int ymap = 0;
+  int yin[MAXB];
+  int yout[MAXB];
+  int yik[MAXB];
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -172,32 +184,62 @@ template <typename T, int N, int NF, bool PR>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
-  __shared__ c2<T> sm[NF * LP];
+  constexpr int LS = LP > N + 2 ? LP : N + 2;   // row stride: FFT rows, or the staged input
+  __shared__ c2<T> sm[NF * LS];
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const long long pairs_per = (long long)N * N / 2;
   const long long gp = (long long)blockIdx.x * NF + f;
   const int b = (int)(gp / pairs_per);
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
-  const c2<T>* sp = a.spec + b * a.sstride + row0 * PX;
+  const int slot = a.xmap ? a.xin[b] : b;
+  const bool xik = a.xmap && a.xik[b];
+  const c2<T>* sp = a.spec + slot * a.sstride + row0 * PX;
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   c2<double> v[E];
+  {
+    // each mode is loaded once: the kx <= N/2 entries of both rows go to the
+    // row's LDS slots (A at [0, N/2], B at [N/2+1, N+1]) and the Hermitian
+    // extension k > N/2 reads its mirror N-k from there.  Modes beyond the
+    // kept cube are zero: their loads are redirected to kx = 0 (no branch)
+    // and discarded.
+    c2<T>* st = sm + f * LS;
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k = t + m * Tf;
-    const int kk = k <= N / 2 ? k : N - k;
-    // modes beyond the kept cube are zero: load from kx = 0 (unconditional
-    // load, no branch) and discard
-    const bool in = kk <= kmax;
-    const int ks = in ? kk : 0;
-    c2<T> As = sp[ks], Bs = sp[PX + ks];
-    if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
-    double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
-    if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
-    if (k > N / 2) { ay = -ay; by = -by; }
-    v[m] = cmk<double>(ax - by, ay + bx);
+    for (int m = 0; m < E / 2; ++m) {
+      const int k = t + m * Tf;
+      const bool in = k <= kmax;
+      const int ks = in ? k : 0;
+      c2<T> As = sp[ks], Bs = sp[PX + ks];
+      if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
+      if (xik) {   // i K_x, K_x = 2 pi k (k < N/2 here)
+        const double K = TWO_PI * k;
+        As = cmk<T>((T)(-K * (double)As.y), (T)(K * (double)As.x));
+        Bs = cmk<T>((T)(-K * (double)Bs.y), (T)(K * (double)Bs.x));
+      }
+      st[k] = As;
+      st[N / 2 + 1 + k] = Bs;
+    }
+    if (t == 0) {   // Nyquist kx = N/2: zero under i K_x (REAL(ifft) rule)
+      const bool in = N / 2 <= kmax && !xik;
+      c2<T> As = sp[in ? N / 2 : 0], Bs = sp[PX + (in ? N / 2 : 0)];
+      if (!in) { As = cmk<T>((T)0, (T)0); Bs = As; }
+      st[N / 2] = As;
+      st[N + 1] = Bs;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * Tf;
+      const int kk = k <= N / 2 ? k : N - k;
+      const c2<T> As = st[kk], Bs = st[N / 2 + 1 + kk];
+      double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
+      if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
+      if (k > N / 2) { ay = -ay; by = -by; }
+      v[m] = cmk<double>(ax - by, ay + bx);
+    }
+    __syncthreads();   // the first FFT pass overwrites the staged rows
   }
-  LdsRow<T> L{sm + f * LP};
+  LdsRow<T> L{sm + f * LS};
   fft_core<double, N, E, +1>(v, t, a.tw, L);
   T* o = a.rdst[b] + row0 * N;
   const double sc = a.scale;
@@ -313,12 +355,26 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     }
   } else {
     const c2<T>* s = a.spec + plane * pstride + col;
+    bool yik = false;
+    if (MODE == 0 && a.ymap) {
+      const int f = plane / N, z = plane - f * N;
+      s = a.spec + a.yin[f] * a.sstride + z * pstride + col;
+      yik = a.yik[f] != 0;
+    }
     if (ld_ok) {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = ldrow(s, t + m * Tf);
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
+    }
+    if (yik) {   // i K_y, zero at the y Nyquist (REAL(ifft) rule)
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = t + m * Tf;
+        const double K = n == N / 2 ? 0.0 : TWO_PI * (n < N / 2 ? n : n - N);
+        v[m] = cmk<T>((T)(-K * (double)v[m].y), (T)(K * (double)v[m].x));
+      }
     }
   }
   LdsCol<T, C> L{sm, c};
@@ -351,6 +407,10 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     // y passes and the inverse z pass: the forward y pass stores only the
     // kept ky rows (the pruned z pass reads nothing else)
     c2<T>* d = a.spec + plane * pstride + col;
+    if (MODE == 0 && a.ymap) {
+      const int f = plane / N, z = plane - f * N;
+      d = a.spec + a.yout[f] * a.sstride + z * pstride + col;
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = t + m * Tf;
