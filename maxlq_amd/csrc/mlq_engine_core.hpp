@@ -324,11 +324,20 @@ template <int N> constexpr int xpass_NF() {
   return nf < N * N / 2 ? nf : N * N / 2;
 }
 
+// pruned when every output of the batch is masked to a cube smaller than the grid
+template <typename T> int Engine<T>::fwd_keep(int nb, const SpecOp* ops) const {
+  int keep = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= n_ / 2) return -1;
+    keep = std::max(keep, ops[b].mcut);
+  }
+  return keep;
+}
+
 template <typename T> template <int N>
 void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {
   constexpr int NF = xpass_NF<N>();
   constexpr int Tf = FftShape<N>::Tf;
-  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
   XArgs<T> xa{};
   for (int b = 0; b < nb; ++b) xa.rsrc[b] = src[b];
   xa.spec = scratch_;
@@ -336,22 +345,25 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   xa.sstride = ns_;
   xa.nb = nb;
   xa.scale = (T)1;
-  // pruned when every output of the batch is masked to a cube smaller than the grid
-  int keep = -1;
-  for (int b = 0; b < nb; ++b) {
-    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= N / 2) { keep = -1; break; }
-    keep = std::max(keep, ops[b].mcut);
-  }
+  const int keep = fwd_keep(nb, ops);
   xa.keep = keep;
   const long long pairs = (long long)nb * N * N / 2;
-  const double rb = (double)nb * real_bytes(), sb = (double)nb * spec_bytes();
   prof_begin(PC_XFWD);
   if (keep >= 0)
     hipLaunchKernelGGL((k_xfwd<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   else
     hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_XFWD, rb + sb);
+  prof_end(PC_XFWD, nb * (real_bytes() + spec_bytes()));
+  fwd_yz_n<N>(nb, dst, ops, keep);
+}
+
+// y and z passes of a forward transform whose x pass left nb spectra in the
+// scratch slots 0..nb-1
+template <typename T> template <int N>
+void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep) {
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  const double sb = (double)nb * spec_bytes();
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
@@ -367,6 +379,31 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   launch_strided<T, N, -1, 2>(svar_[1], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZFWD, 2 * sb);
+}
+
+template <typename T> template <int N, class P>
+void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool& done) {
+  if constexpr (xfwd_p_fits<P, T, N>()) {
+    constexpr int Tf = FftShape<N>::Tf;
+    XArgs<T> xa{};
+    xa.spec = scratch_;
+    xa.tw = tw_;
+    xa.sstride = ns_;
+    xa.nb = P::NB;
+    xa.scale = (T)1;
+    const int keep = fwd_keep(P::NB, ops);
+    xa.keep = keep;
+    const unsigned pairs = (unsigned)((long long)N * N / 2);
+    prof_begin(PC_XFWD);
+    if (keep >= 0)
+      hipLaunchKernelGGL((k_xfwd_p<T, N, true, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);
+    else
+      hipLaunchKernelGGL((k_xfwd_p<T, N, false, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_XFWD, P::NB * spec_bytes());
+    fwd_yz_n<N>(P::NB, dst, ops, keep);
+    done = true;
+  }
 }
 
 template <typename T> template <int N>
@@ -504,6 +541,17 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
     case 1024: FN<1024>(__VA_ARGS__); break;                                 \
     default: throw Error(MLQ_EINVAL, "unsupported n " + std::to_string(n_)); \
   }
+
+// forward transforms of the P::NB fields a producer computes pointwise
+// (k_xfwd_p); false (nothing launched) when the scratch or the LDS cannot
+// hold the batch, the caller then materialises the fields
+template <typename T> template <class P>
+bool Engine<T>::fwd_prod(const P& prod, C* const* dst, const SpecOp* ops) {
+  if (P::NB > sb_) return false;
+  bool done = false;
+  MLQ_DISPATCH_N(fwd_prod_n, prod, dst, ops, done);
+  return done;
+}
 
 template <typename T>
 void Engine<T>::fwd(int nb, const T* const* src, C* const* dst, const SpecOp* ops) {

@@ -271,53 +271,74 @@ template <typename T> class FastRcg {
       e_.inv(12, is, ops, dst, dB_limited_ ? M_ : -1);   // band-limited part of d
     }
     S1_ = e_.rtmp(); S2_ = e_.rtmp(); b_ = e_.rtmp(3); ee_ = e_.rtmp(3);
-    RT X12 = e_.rtmp(2);
-    Q1_ = e_.rtmp(); Q2_ = e_.rtmp();
-    e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_,
-                            (T)gam_}, nr_);
-    // dealias the quadratic coefficients; the masked spectra of S1, S2 are
-    // kept for the line-search polynomials
     if (poly_ok_) {
+      // the 12 coefficient fields go straight into their masked spectra (one
+      // fused forward pass, no intermediate fields): S1, S2 (kept for the
+      // line-search polynomials), b, e (dealiased below), X1, X2 (the
+      // Poisson solves P1, P2) and Q1, Q2 (|grad phi|^2 coefficients)
       S12h_ = e_.stmp(2);
-      const T* src[2] = {S1_, S2_};
-      C* sh[2] = {S12h_, S12h_ + ns_};
-      SpecOp ops[2] = {sp_mask(M_), sp_mask(M_)};
-      e_.fwd(2, src, sh, ops);
-      InvSrc is[2] = {InvSrc{{sh[0], nullptr, nullptr}, {1.0, 0, 0}, 1},
-                      InvSrc{{sh[1], nullptr, nullptr}, {1.0, 0, 0}, 1}};
-      SpecOp none[2] = {sp_none(), sp_none()};
-      T* dst[2] = {S1_, S2_};
-      e_.inv(2, is, none, dst, M_);
-      T* f[6] = {b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_};
-      e_.dealias_dev(f, 6, 2.0);
+      P12h_ = e_.stmp(2);
+      Q12h_ = e_.stmp(2);
+      ST beh = e_.stmp(6);
+      C* dst[12] = {S12h_, S12h_ + ns_, beh, beh + ns_, beh + 2 * ns_, beh + 3 * ns_,
+                    beh + 4 * ns_, beh + 5 * ns_, P12h_, P12h_ + ns_, Q12h_, Q12h_ + ns_};
+      SpecOp ops[12];
+      for (int k = 0; k < 12; ++k) ops[k] = (k == 8 || k == 9) ? poisson_op() : sp_mask(M_);
+      if (!e_.fwd_prod(P_ls_products<T>{u_, d_, G_, Gd_, nr_, (T)gam_}, dst, ops)) {
+        RT X = e_.rtmp(12);
+        e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, X, X + nr_, X + 2 * nr_, X + 5 * nr_,
+                                X + 8 * nr_, X + 9 * nr_, X + 10 * nr_, X + 11 * nr_, (T)gam_},
+               nr_);
+        const T* src[12];
+        for (int k = 0; k < 12; ++k) src[k] = X + k * nr_;
+        e_.fwd(12, src, dst, ops);
+      }
+      // dealiased S1, S2, b, e and the pressure coefficients P0, P1, P2 in
+      // one batch of band-limited inverses
+      P_ = e_.rtmp(3);
+      const C* in[11] = {S12h_, S12h_ + ns_, beh, beh + ns_, beh + 2 * ns_, beh + 3 * ns_,
+                         beh + 4 * ns_, beh + 5 * ns_, ph_, P12h_, P12h_ + ns_};
+      T* out[11] = {S1_, S2_, b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_,
+                    P_, P_ + nr_, P_ + 2 * nr_};
+      InvSrc is[11];
+      SpecOp none[11];
+      for (int k = 0; k < 11; ++k) {
+        is[k] = InvSrc{{in[k], nullptr, nullptr}, {1.0, 0, 0}, 1};
+        none[k] = sp_none();
+      }
+      e_.inv(11, is, none, out, M_);
     } else {
+      RT X12 = e_.rtmp(2);
+      Q1_ = e_.rtmp(); Q2_ = e_.rtmp();
+      e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, S1_, S2_, b_, ee_, X12, X12 + nr_, Q1_, Q2_,
+                              (T)gam_}, nr_);
       T* f[8] = {S1_, S2_, b_, b_ + nr_, b_ + 2 * nr_, ee_, ee_ + nr_, ee_ + 2 * nr_};
       e_.dealias_dev(f, 8, 2.0);
-    }
-    // pressure coefficients P1, P2 (spectra kept for the state update) and P0
-    P12h_ = e_.stmp(2);
-    {
-      const T* src[2] = {X12, X12 + nr_};
-      C* dst[2] = {P12h_, P12h_ + ns_};
-      SpecOp ops[2] = {poisson_op(), poisson_op()};
-      e_.fwd(2, src, dst, ops);
-    }
-    P_ = e_.rtmp(3);
-    {
-      InvSrc is[3] = {InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
-                      InvSrc{{P12h_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
-                      InvSrc{{P12h_ + ns_, nullptr, nullptr}, {1.0, 0, 0}, 1}};
-      SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
-      T* dst[3] = {P_, P_ + nr_, P_ + 2 * nr_};
-      e_.inv(3, is, ops, dst, M_);   // pressure spectra are masked
-    }
-    // |grad phi|^2 coefficient spectra
-    Q12h_ = e_.stmp(2);
-    {
-      const T* src[2] = {Q1_, Q2_};
-      C* dst[2] = {Q12h_, Q12h_ + ns_};
-      SpecOp ops[2] = {sp_mask(M_), sp_mask(M_)};
-      e_.fwd(2, src, dst, ops);
+      // pressure coefficients P1, P2 (spectra kept for the state update) and P0
+      P12h_ = e_.stmp(2);
+      {
+        const T* src[2] = {X12, X12 + nr_};
+        C* dst[2] = {P12h_, P12h_ + ns_};
+        SpecOp ops[2] = {poisson_op(), poisson_op()};
+        e_.fwd(2, src, dst, ops);
+      }
+      P_ = e_.rtmp(3);
+      {
+        InvSrc is[3] = {InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
+                        InvSrc{{P12h_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1},
+                        InvSrc{{P12h_ + ns_, nullptr, nullptr}, {1.0, 0, 0}, 1}};
+        SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
+        T* dst[3] = {P_, P_ + nr_, P_ + 2 * nr_};
+        e_.inv(3, is, ops, dst, M_);   // pressure spectra are masked
+      }
+      // |grad phi|^2 coefficient spectra
+      Q12h_ = e_.stmp(2);
+      {
+        const T* src[2] = {Q1_, Q2_};
+        C* dst[2] = {Q12h_, Q12h_ + ns_};
+        SpecOp ops[2] = {sp_mask(M_), sp_mask(M_)};
+        e_.fwd(2, src, dst, ops);
+      }
     }
     if (q4_) {
       double acc[11];

@@ -17,43 +17,70 @@ constexpr int RED_MAXK_BIG = 48;
 //   Q(tau)   = |grad phi|^2                   -> Q1, Q2
 // The direction is d = dB + gam u (dB band-limited, gam a scalar) and its
 // gradient Gd = grad dB + gam G: d and Gd here hold the dB parts.
+template <typename T>
+__device__ __forceinline__ void ls_products_at(const T* u, const T* d, const T* G, const T* Gd,
+                                               long long nr, T gam, long long i, T (&o)[12]) {
+  T uu[3], dd[3], g[9], h[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i] + gam * uu[c]; }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i] + gam * g[a]; }
+  o[0] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
+  o[1] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    T bj = 0, ej = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      bj += uu[a] * h[a * 3 + j] + dd[a] * g[a * 3 + j];
+      ej += dd[a] * h[a * 3 + j];
+    }
+    o[2 + j] = bj;
+    o[5 + j] = ej;
+  }
+  T x1 = 0, x2 = 0, q1 = 0, q2 = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x1 += g[c * 3 + a] * h[a * 3 + c] + h[c * 3 + a] * g[a * 3 + c];
+      x2 += h[c * 3 + a] * h[a * 3 + c];
+      q1 += g[a * 3 + c] * h[a * 3 + c];
+      q2 += h[a * 3 + c] * h[a * 3 + c];
+    }
+  o[8] = x1;
+  o[9] = x2;
+  o[10] = (T)2 * q1;
+  o[11] = q2;
+}
+// the 12 coefficient fields of the line search, in the order
+// S1, S2, b (3), e (3), X1, X2, Q1, Q2 (pointwise version: one field each)
 template <typename T> struct F_ls_products {
   const T* u; const T* d; const T* G; const T* Gd; long long nr;
   T* S1; T* S2; T* b; T* e; T* X1; T* X2; T* Q1; T* Q2;   // b, e: 3 comps (stride nr)
   T gam;
   __device__ void operator()(long long i) const {
-    T uu[3], dd[3], g[9], h[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i] + gam * uu[c]; }
-#pragma unroll
-    for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i] + gam * g[a]; }
-    S1[i] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
-    S2[i] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+    T o[12];
+    ls_products_at(u, d, G, Gd, nr, gam, i, o);
+    S1[i] = o[0];
+    S2[i] = o[1];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      T bj = 0, ej = 0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        bj += uu[a] * h[a * 3 + j] + dd[a] * g[a * 3 + j];
-        ej += dd[a] * h[a * 3 + j];
-      }
-      b[j * nr + i] = bj;
-      e[j * nr + i] = ej;
+      b[j * nr + i] = o[2 + j];
+      e[j * nr + i] = o[5 + j];
     }
-    T x1 = 0, x2 = 0, q1 = 0, q2 = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        x1 += g[c * 3 + a] * h[a * 3 + c] + h[c * 3 + a] * g[a * 3 + c];
-        x2 += h[c * 3 + a] * h[a * 3 + c];
-        q1 += g[a * 3 + c] * h[a * 3 + c];
-        q2 += h[a * 3 + c] * h[a * 3 + c];
-      }
-    X1[i] = x1;
-    X2[i] = x2;
-    Q1[i] = (T)2 * q1;
-    Q2[i] = q2;
+    X1[i] = o[8];
+    X2[i] = o[9];
+    Q1[i] = o[10];
+    Q2[i] = o[11];
+  }
+};
+// the same 12 values as a producer of the fused forward x pass (k_xfwd_p)
+template <typename T> struct P_ls_products {
+  static constexpr int NB = 12;
+  const T* u; const T* d; const T* G; const T* Gd; long long nr; T gam;
+  __device__ void operator()(long long i, T (&o)[12]) const {
+    ls_products_at(u, d, G, Gd, nr, gam, i, o);
   }
 };
 

@@ -140,6 +140,33 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
 // x pass, forward: two real rows packed as one complex row (a + i b), one
 // N-point FFT, split into the two half spectra.
 // ---------------------------------------------------------------------------
+// Transform of the packed pair in v (row view L of the thread's row pair) and
+// the split into the half spectra at d (row0) and d + PX (row0 + 1); only
+// kx <= kmax is stored.
+template <typename T, int N>
+__device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t,
+                                          const double2* __restrict__ tw, const LdsRow<T>& L,
+                                          c2<T>* d, int kmax) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
+  fft_core<double, N, E, -1>(v, t, tw, L);
+#pragma unroll
+  for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E / 2; ++m) {
+    const int k = t + m * Tf;
+    if (k > kmax) continue;
+    const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
+    d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
+    d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
+  }
+  if (t == 0 && kmax == N / 2) {
+    const c2<double> z = ccvt<double>(L.ld(N / 2));
+    d[N / 2] = cmk<T>((T)z.x, (T)0);
+    d[PX + N / 2] = cmk<T>((T)z.y, (T)0);
+  }
+}
+
 template <typename T, int N, int NF, bool PR>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
@@ -156,25 +183,44 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
   LdsRow<T> L{sm + f * LP};
-  fft_core<double, N, E, -1>(v, t, a.tw, L);
-#pragma unroll
-  for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
-  __syncthreads();
-  c2<T>* d = a.spec + b * a.sstride + row0 * PX;
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
+  xfwd_pair<T, N>(v, t, a.tw, L, a.spec + b * a.sstride + row0 * PX, kmax);
+}
+
+// Forward x pass of NB fields produced in the kernel (one row pair per
+// workgroup): P computes the NB values of each of the pair's 2N points from
+// its inputs (the values of one point share their loads), stages them in the
+// LDS rows, and the NB transforms run on the staged rows.  Replaces a
+// pointwise kernel that writes NB fields plus the x pass that reads them back.
+template <class P, typename T, int N>
+constexpr bool xfwd_p_fits() {
+  return (long long)P::NB * LdsRow<T>::pitch(N) * (long long)sizeof(c2<T>) <= 160 * 1024 &&
+         P::NB * FftShape<N>::Tf <= 1024;
+}
+template <typename T, int N, bool PR, class P>
+__global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs<T> a) {
+  constexpr int NB = P::NB, E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
+  constexpr int LP = LdsRow<T>::pitch(N);
+  static_assert(2 * N <= 2 * LP, "a row pair's 2N reals fit its LDS row");
+  __shared__ c2<T> sm[NB * LP];
+  T* stg = reinterpret_cast<T*>(sm);   // field f: 2N reals at stg + 2 LP f
+  const long long row0 = 2 * (long long)blockIdx.x;
+  for (int p = threadIdx.x; p < 2 * N; p += NB * Tf) {
+    T o[NB];
+    prod(row0 * N + p, o);
 #pragma unroll
-  for (int m = 0; m < E / 2; ++m) {
-    const int k = t + m * Tf;
-    if (k > kmax) continue;
-    const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
-    d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
-    d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
+    for (int f = 0; f < NB; ++f) stg[2 * LP * f + p] = o[f];
   }
-  if (t == 0 && kmax == N / 2) {
-    const c2<double> z = ccvt<double>(L.ld(N / 2));
-    d[N / 2] = cmk<T>((T)z.x, (T)0);
-    d[PX + N / 2] = cmk<T>((T)z.y, (T)0);
-  }
+  __syncthreads();
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const T* s = stg + 2 * LP * f;
+  c2<double> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
+  __syncthreads();   // the transform rewrites the staged rows
+  LdsRow<T> L{sm + f * LP};
+  const int kmax = PR ? a.keep : N / 2;
+  xfwd_pair<T, N>(v, t, a.tw, L, a.spec + f * a.sstride + row0 * PX, kmax);
 }
 
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
