@@ -166,8 +166,22 @@ template <typename T> class FastRcg {
     }
     // aux2 = D(|u|^(q-4) D(u.(nu Lap u - grad p)))           (:1131-1138)
     RT aux2 = e_.rtmp();
-    e_.map(F_udotforce<T>{u_, lap, gp, nr_, (T)nuvc_, (T)pc_, aux2}, nr_);
-    dealias1(aux2);
+    {
+      // D(u.force): the product goes straight into its masked spectrum
+      const F_udotforce<T> uf{u_, lap, gp, nr_, (T)nuvc_, (T)pc_, aux2};
+      ST h = e_.stmp();
+      C* hd = h;
+      const SpecOp op = sp_mask(M_);
+      if (e_.fwd_prod(P_udotforce<T>{uf}, &hd, &op)) {
+        InvSrc is{{hd, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        const SpecOp none = sp_none();
+        T* o = aux2;
+        e_.inv(1, &is, &none, &o, M_);
+      } else {
+        e_.map(uf, nr_);
+        dealias1(aux2);
+      }
+    }
     if (!q4_) {
       e_.map(F_mul<T>{Bq_, aux2, aux2}, nr_);
       dealias1(aux2);
@@ -192,10 +206,17 @@ template <typename T> class FastRcg {
     // and of the normal ||u||_q^(1-q) W (:162-169)
     ST Vh = e_.stmp(3);
     {
-      RT Y = e_.rtmp(3);
-      e_.map(F_grad_terms<T>{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
-                             (T)nuvc_, Y}, nr_);
-      fwd3(Y, Vh, sp_mask(M_));
+      const F_grad_terms<T> gt{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+                               (T)nuvc_, nullptr};
+      C* vd[3] = {Vh, Vh + ns_, Vh + 2 * ns_};
+      const SpecOp ops[3] = {sp_mask(M_), sp_mask(M_), sp_mask(M_)};
+      if (!e_.fwd_prod(P_grad_terms<T>{gt}, vd, ops)) {
+        RT Y = e_.rtmp(3);
+        F_grad_terms<T> g2 = gt;
+        g2.Y = Y;
+        e_.map(g2, nr_);
+        fwd3(Y, Vh, sp_mask(M_));
+      }
     }
     ST nH = e_.stmp(3);
     e_.map(S_dir_prep_k<T>{Vh, W_, nH, ns_, n_, pp_, M_, hwt_, nuvc_, std::pow(L_, 1.0 - q_)},
@@ -441,9 +462,13 @@ template <typename T> class FastRcg {
     ST Wphi = e_.stmp(3);
     RT unew = e_.rtmp(3);
     {
-      RT Au = e_.rtmp(3);
-      e_.map(F_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, Au, unew}, nr_);
-      fwd3(Au, Wphi, sp_mask(M_, wsc));
+      C* wd[3] = {Wphi, Wphi + ns_, Wphi + 2 * ns_};
+      const SpecOp ops[3] = {sp_mask(M_, wsc), sp_mask(M_, wsc), sp_mask(M_, wsc)};
+      if (!e_.fwd_prod(P_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, unew}, wd, ops)) {
+        RT Au = e_.rtmp(3);
+        e_.map(F_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, Au, unew}, nr_);
+        fwd3(Au, Wphi, sp_mask(M_, wsc));
+      }
     }
     // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u
     double ir[3];

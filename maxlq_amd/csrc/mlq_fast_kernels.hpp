@@ -149,6 +149,20 @@ template <typename T> struct F_step {
     }
   }
 };
+// producer form: A phi_c feeds the fused forward x pass, sc phi_c is stored
+template <typename T> struct P_step {
+  static constexpr int NB = 3;
+  const T* u; const T* d; const T* A; long long nr; T a, b, sc; T* un;
+  __device__ void operator()(long long i, T (&o)[3]) const {
+    const T w = A[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T phi = a * u[c * nr + i] + b * d[c * nr + i];
+      o[c] = w * phi;
+      un[c * nr + i] = sc * phi;
+    }
+  }
+};
 
 // o = sc * (x0 + t x1 + t^2 x2)
 template <typename T> struct F_quad {
@@ -169,17 +183,29 @@ template <typename T> struct S_quad {
 template <typename T> struct F_grad_terms {
   const T* aux2; const T* u; const T* A; const T* gp; const T* G; const T* gphi; const T* lap;
   long long nr; T q2, pc, vnu; T* Y;
-  __device__ void operator()(long long i) const {
+  __device__ void at(long long i, T (&y)[3]) const {
     const T a2 = aux2[i], Ai = A ? A[i] : (T)1;
     const T f0 = gphi[i], f1 = gphi[nr + i], f2 = gphi[2 * nr + i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const T t5 = G[(0 * 3 + c) * nr + i] * f0 + G[(1 * 3 + c) * nr + i] * f1 +
                    G[(2 * 3 + c) * nr + i] * f2;
-      Y[c * nr + i] = q2 * a2 * u[c * nr + i] - pc * Ai * gp[c * nr + i] + (T)2 * pc * t5 +
-                      vnu * Ai * lap[c * nr + i];
+      y[c] = q2 * a2 * u[c * nr + i] - pc * Ai * gp[c * nr + i] + (T)2 * pc * t5 +
+             vnu * Ai * lap[c * nr + i];
     }
   }
+  __device__ void operator()(long long i) const {
+    T y[3];
+    at(i, y);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Y[c * nr + i] = y[c];
+  }
+};
+// the same three components as a producer of the fused forward x pass
+template <typename T> struct P_grad_terms {
+  static constexpr int NB = 3;
+  F_grad_terms<T> f;   // Y unused
+  __device__ void operator()(long long i, T (&o)[3]) const { f.at(i, o); }
 };
 
 // Hilbert weight then Leray projection on three half spectra, in place

@@ -109,12 +109,19 @@ template <typename T> struct F_unit {
 // aux2 = sum_i v_i (vnu * Lap u_i - pc * grad_i p)   (function_ops.f90:1119, 1132)
 template <typename T> struct F_udotforce {
   const T* v; const T* L; const T* P; long long nr; T vnu, pc; T* o;
-  __device__ void operator()(long long i) const {
+  __device__ T at(long long i) const {
     T s = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) s += v[c * nr + i] * (vnu * L[c * nr + i] - pc * P[c * nr + i]);
-    o[i] = s;
+    return s;
   }
+  __device__ void operator()(long long i) const { o[i] = at(i); }
+};
+// producer form for the fused forward x pass
+template <typename T> struct P_udotforce {
+  static constexpr int NB = 1;
+  F_udotforce<T> f;   // o unused
+  __device__ void operator()(long long i, T (&r)[1]) const { r[0] = f.at(i); }
 };
 // aux5_i = sum_j d_i u_j * d_j phi   (function_ops.f90:1165-1169)
 template <typename T> struct F_gradT_dot {
