@@ -238,6 +238,9 @@ template <typename T> template <class F> void Engine<T>::map(const F& f, long lo
 
 template <typename T> template <int K, class F>
 void Engine<T>::reduce(const F& f, long long n, double* out) {
+  constexpr int vec = RedVec<F>::v;
+  if (n % vec) throw Error(MLQ_EINVAL, "reduction length not a multiple of its vector width");
+  n /= vec;
   prof_begin(PC_REDUCE);
   hipLaunchKernelGGL((k_reduce<K, F>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, f, n, part_);
   MLQ_HIP(hipGetLastError());

@@ -79,6 +79,8 @@ template <typename T> class FastRcg {
 
   // ---- state from a physical field ----------------------------------------------
   void init_state(const T* u_in) {
+    g_pending_ = false;
+    Gd_prev_ = RT();
     u_ = e_.rtmp(3);
     e_.map(F_copy<T>{u_in, u_}, nr_ * 3);
     uh_ = e_.stmp(3);
@@ -206,17 +208,26 @@ template <typename T> class FastRcg {
     // and of the normal ||u||_q^(1-q) W (:162-169)
     ST Vh = e_.stmp(3);
     {
-      const F_grad_terms<T> gt{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
-                               (T)nuvc_, nullptr};
+      F_grad_terms<T> gt{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+                         (T)nuvc_, nullptr};
+      if (g_pending_) {   // the last step's update of G rides along
+        gt.Gd0 = Gd_prev_;
+        gt.ga = (T)gA_;
+        gt.gb = (T)gB_;
+        gt.Gout = G_;
+      }
       C* vd[3] = {Vh, Vh + ns_, Vh + 2 * ns_};
       const SpecOp ops[3] = {sp_mask(M_), sp_mask(M_), sp_mask(M_)};
       if (!e_.fwd_prod(P_grad_terms<T>{gt}, vd, ops)) {
+        flush_G();
         RT Y = e_.rtmp(3);
-        F_grad_terms<T> g2 = gt;
-        g2.Y = Y;
+        F_grad_terms<T> g2{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+                           (T)nuvc_, Y};
         e_.map(g2, nr_);
         fwd3(Y, Vh, sp_mask(M_));
       }
+      g_pending_ = false;
+      Gd_prev_ = RT();
     }
     ST nH = e_.stmp(3);
     e_.map(S_dir_prep_k<T>{Vh, W_, nH, ns_, n_, pp_, M_, hwt_, nuvc_, std::pow(L_, 1.0 - q_)},
@@ -273,6 +284,7 @@ template <typename T> class FastRcg {
 
   // ---- line-search coefficient fields for phi = u + tau d --------------------------
   void precompute_ls() {
+    flush_G();
     d_ = e_.rtmp(3);
     Gd_ = e_.rtmp(9);
     {
@@ -498,7 +510,12 @@ template <typename T> class FastRcg {
     have_cg_ = true;
     // retraction and state update by homogeneity
     u_ = std::move(unew);
-    e_.map(F_axpby<T>{G_, Gd_, G_, (T)(sc * opg), (T)(sc * tau1)}, nr_ * 9);
+    // G = grad u_new = sc opg G + sc tau1 grad dB: deferred into the next
+    // gradient-terms pass (which reads G anyway)
+    Gd_prev_ = std::move(Gd_);
+    gA_ = sc * opg;
+    gB_ = sc * tau1;
+    g_pending_ = true;
     e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
     for (int j = 0; j < 3; ++j)
@@ -529,6 +546,14 @@ template <typename T> class FastRcg {
     L_ = sc * L1;
     release_ls();
     return L1;
+  }
+
+  // apply a deferred update of G now
+  void flush_G() {
+    if (!g_pending_) return;
+    e_.map(F_axpby<T>{G_, Gd_prev_, G_, (T)gA_, (T)gB_}, nr_ * 9);
+    g_pending_ = false;
+    Gd_prev_ = RT();
   }
 
   void release_ls() {
@@ -745,6 +770,10 @@ template <typename T> class FastRcg {
   bool have_cg_ = false;
   // line-search coefficients
   RT d_, Gd_, S1_, S2_, b_, ee_, Q1_, Q2_, P_;
+  // deferred update G := gA_ G + gB_ Gd_prev_ (applied by the next direction)
+  RT Gd_prev_;
+  double gA_ = 1.0, gB_ = 0.0;
+  bool g_pending_ = false;
   ST P12h_, Q12h_;
   double cpoly_[5] = {0}, rpoly_[6] = {0};
   // line-search polynomials of ||phi||_q^q and the R1 numerator (valid where

@@ -89,32 +89,54 @@ template <typename T> struct P_ls_products {
 //   p(tau) * sum_j phi_j w_j, p = P0 + tau P1 + tau^2 P2   -> r0..r5
 template <typename T> struct R_ls_poly {
   static constexpr int unroll = 1;   // 18 input streams: enough loads in flight
+  static constexpr int vec = V16<T>::w;   // consecutive points per call (16-byte loads)
   const T* a; const T* b; const T* e; const T* u; const T* d;
   const T* P0; const T* P1; const T* P2; long long nr; double gam;   // d + gam u
-  __device__ void operator()(long long i, double (&acc)[11]) const {
-    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+  __device__ void operator()(long long iv, double (&acc)[11]) const {
+    constexpr int W = vec;
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    double v0[W], v1[W], v2[W], v3[W];
+#pragma unroll
+    for (int l = 0; l < W; ++l) v0[l] = v1[l] = v2[l] = v3[l] = 0.0;
+    const long long nv = nr / W;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double aj = a[j * nr + i], bj = b[j * nr + i], ej = e[j * nr + i];
-      const double uj = u[j * nr + i], dj = (double)d[j * nr + i] + gam * uj;
-      c0 += aj * aj;
-      c1 += 2.0 * aj * bj;
-      c2 += bj * bj + 2.0 * aj * ej;
-      c3 += 2.0 * bj * ej;
-      c4 += ej * ej;
-      v0 += uj * aj;
-      v1 += uj * bj + dj * aj;
-      v2 += uj * ej + dj * bj;
-      v3 += dj * ej;
+      T av[W], bv[W], ev[W], uv[W], dv[W];
+      ld16(a, j * nv + iv, av);
+      ld16(b, j * nv + iv, bv);
+      ld16(e, j * nv + iv, ev);
+      ld16(u, j * nv + iv, uv);
+      ld16(d, j * nv + iv, dv);
+#pragma unroll
+      for (int l = 0; l < W; ++l) {
+        const double aj = av[l], bj = bv[l], ej = ev[l];
+        const double uj = uv[l], dj = (double)dv[l] + gam * uj;
+        c0 += aj * aj;
+        c1 += 2.0 * aj * bj;
+        c2 += bj * bj + 2.0 * aj * ej;
+        c3 += 2.0 * bj * ej;
+        c4 += ej * ej;
+        v0[l] += uj * aj;
+        v1[l] += uj * bj + dj * aj;
+        v2[l] += uj * ej + dj * bj;
+        v3[l] += dj * ej;
+      }
     }
-    const double p0 = P0[i], p1 = P1[i], p2 = P2[i];
+    T p0v[W], p1v[W], p2v[W];
+    ld16(P0, iv, p0v);
+    ld16(P1, iv, p1v);
+    ld16(P2, iv, p2v);
     acc[0] += c0; acc[1] += c1; acc[2] += c2; acc[3] += c3; acc[4] += c4;
-    acc[5] += p0 * v0;
-    acc[6] += p0 * v1 + p1 * v0;
-    acc[7] += p0 * v2 + p1 * v1 + p2 * v0;
-    acc[8] += p0 * v3 + p1 * v2 + p2 * v1;
-    acc[9] += p1 * v3 + p2 * v2;
-    acc[10] += p2 * v3;
+#pragma unroll
+    for (int l = 0; l < W; ++l) {
+      const double p0 = p0v[l], p1 = p1v[l], p2 = p2v[l];
+      acc[5] += p0 * v0[l];
+      acc[6] += p0 * v1[l] + p1 * v0[l];
+      acc[7] += p0 * v2[l] + p1 * v1[l] + p2 * v0[l];
+      acc[8] += p0 * v3[l] + p1 * v2[l] + p2 * v1[l];
+      acc[9] += p1 * v3[l] + p2 * v2[l];
+      acc[10] += p2 * v3[l];
+    }
   }
 };
 
@@ -183,13 +205,25 @@ template <typename T> struct S_quad {
 template <typename T> struct F_grad_terms {
   const T* aux2; const T* u; const T* A; const T* gp; const T* G; const T* gphi; const T* lap;
   long long nr; T q2, pc, vnu; T* Y;
+  // pending state update of G (optional): G := ga G + gb Gd0, written to Gout
+  // by the thread that uses it (the same point, so in place is safe)
+  const T* Gd0 = nullptr; T ga = 1, gb = 0; T* Gout = nullptr;
   __device__ void at(long long i, T (&y)[3]) const {
+    T g[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = G[k * nr + i];
+    if (Gd0) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        g[k] = ga * g[k] + gb * Gd0[k * nr + i];
+        Gout[k * nr + i] = g[k];
+      }
+    }
     const T a2 = aux2[i], Ai = A ? A[i] : (T)1;
     const T f0 = gphi[i], f1 = gphi[nr + i], f2 = gphi[2 * nr + i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T t5 = G[(0 * 3 + c) * nr + i] * f0 + G[(1 * 3 + c) * nr + i] * f1 +
-                   G[(2 * 3 + c) * nr + i] * f2;
+      const T t5 = g[0 * 3 + c] * f0 + g[1 * 3 + c] * f1 + g[2 * 3 + c] * f2;
       y[c] = q2 * a2 * u[c * nr + i] - pc * Ai * gp[c * nr + i] + (T)2 * pc * t5 +
              vnu * Ai * lap[c * nr + i];
     }

@@ -496,6 +496,22 @@ template <int K> __device__ __forceinline__ void block_reduce_store(double (&acc
   }
 }
 
+// 16-byte vector loads: w consecutive values of one field
+template <typename T> struct V16;
+template <> struct V16<float> { using type = float4; static constexpr int w = 4; };
+template <> struct V16<double> { using type = double2; static constexpr int w = 2; };
+template <typename T>
+__device__ __forceinline__ void ld16(const T* p, long long iv, T (&o)[V16<T>::w]) {
+  const typename V16<T>::type v = reinterpret_cast<const typename V16<T>::type*>(p)[iv];
+  __builtin_memcpy(o, &v, sizeof(v));
+}
+// points per functor call (`static constexpr int vec`, default 1): the
+// functor's index then counts vectors of that many consecutive points
+template <class F, class = void> struct RedVec { static constexpr int v = 1; };
+template <class F> struct RedVec<F, std::void_t<decltype(F::vec)>> {
+  static constexpr int v = F::vec;
+};
+
 // elements per grid-stride trip: 2 unless the functor says otherwise
 // (`static constexpr int unroll`, for functors heavy in registers)
 template <class F, class = void> struct RedUnroll { static constexpr int v = 2; };
