@@ -227,6 +227,9 @@ template <typename T> void Engine<T>::prof_get(int cls, long long* count, double
 // generic launches
 // ---------------------------------------------------------------------------
 template <typename T> template <class F> void Engine<T>::map(const F& f, long long n, int cls) {
+  constexpr int vec = VecW<F>::v;
+  if (n % vec) throw Error(MLQ_EINVAL, "map length not a multiple of its vector width");
+  n /= vec;
   if (n <= 0) return;
   long long blocks = (n + MAP_THREADS - 1) / MAP_THREADS;
   blocks = std::min<long long>(blocks, 256LL * 64);
@@ -238,7 +241,7 @@ template <typename T> template <class F> void Engine<T>::map(const F& f, long lo
 
 template <typename T> template <int K, class F>
 void Engine<T>::reduce(const F& f, long long n, double* out) {
-  constexpr int vec = RedVec<F>::v;
+  constexpr int vec = VecW<F>::v;
   if (n % vec) throw Error(MLQ_EINVAL, "reduction length not a multiple of its vector width");
   n /= vec;
   prof_begin(PC_REDUCE);

@@ -518,9 +518,7 @@ template <typename T> class FastRcg {
     g_pending_ = true;
     e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
-    for (int j = 0; j < 3; ++j)
-      e_.map(F_quad<T>{a_ + j * nr_, b_ + j * nr_, ee_ + j * nr_, a_ + j * nr_, (T)tau1,
-                       (T)(sc * sc)}, nr_);
+    e_.map(F_quad<T>{a_, b_, ee_, a_, (T)tau1, (T)(sc * sc)}, 3 * nr_);   // 3 components
     // masked spectra: only the kept cube changes
     e_.map(S_quad_k<T>{ph_, P12h_, P12h_ + ns_, ph_, n_, pp_, M_, tau1, sc * sc}, nk_, PC_SPECTRAL);
     e_.map(S_quad_k<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, n_, pp_, M_, tau1, sc * sc}, nk_,

@@ -188,8 +188,17 @@ template <typename T> struct P_step {
 
 // o = sc * (x0 + t x1 + t^2 x2)
 template <typename T> struct F_quad {
+  static constexpr int vec = V16<T>::w;   // 16-byte accesses; o may alias x0
   const T* x0; const T* x1; const T* x2; T* o; T t, sc;
-  __device__ void operator()(long long i) const { o[i] = sc * (x0[i] + t * (x1[i] + t * x2[i])); }
+  __device__ void operator()(long long iv) const {
+    T a[vec], b[vec], c[vec], r[vec];
+    ld16(x0, iv, a);
+    ld16(x1, iv, b);
+    ld16(x2, iv, c);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) r[l] = sc * (a[l] + t * (b[l] + t * c[l]));
+    st16(o, iv, r);
+  }
 };
 template <typename T> struct S_quad {
   const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; double t, sc;

@@ -255,8 +255,12 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
       const int k = t + m * Tf;
       const bool in = k <= kmax;
       const int ks = in ? k : 0;
-      c2<T> As = sp[ks], Bs = sp[PX + ks];
-      if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
+      c2<T> As = cmk<T>((T)0, (T)0), Bs = As;
+      if (!PR || m * Tf <= kmax) {   // uniform: a slot run wholly past kmax loads nothing
+        As = sp[ks];
+        Bs = sp[PX + ks];
+        if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
+      }
       if (xik) {   // i K_x, K_x = 2 pi k (k < N/2 here)
         const double K = TWO_PI * k;
         As = cmk<T>((T)(-K * (double)As.y), (T)(K * (double)As.x));
@@ -505,10 +509,17 @@ __device__ __forceinline__ void ld16(const T* p, long long iv, T (&o)[V16<T>::w]
   const typename V16<T>::type v = reinterpret_cast<const typename V16<T>::type*>(p)[iv];
   __builtin_memcpy(o, &v, sizeof(v));
 }
-// points per functor call (`static constexpr int vec`, default 1): the
-// functor's index then counts vectors of that many consecutive points
-template <class F, class = void> struct RedVec { static constexpr int v = 1; };
-template <class F> struct RedVec<F, std::void_t<decltype(F::vec)>> {
+template <typename T>
+__device__ __forceinline__ void st16(T* p, long long iv, const T (&o)[V16<T>::w]) {
+  typename V16<T>::type v;
+  __builtin_memcpy(&v, o, sizeof(v));
+  reinterpret_cast<typename V16<T>::type*>(p)[iv] = v;
+}
+// points per functor call of a map or reduction (`static constexpr int vec`,
+// default 1): the functor's index then counts vectors of that many
+// consecutive points
+template <class F, class = void> struct VecW { static constexpr int v = 1; };
+template <class F> struct VecW<F, std::void_t<decltype(F::vec)>> {
   static constexpr int v = F::vec;
 };
 
