@@ -305,22 +305,20 @@ template <typename T, int N, int V> struct SVar {
 };
 
 template <typename T, int N, int V, int DIR, int MODE>
-void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, long long es, long long ps,
-                      int planes) {
+void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, int planes) {
   using S = SVar<T, N, V>;
   hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), dim3((ncols + S::C - 1) / S::C, planes),
-                     dim3(S::C * (N / S::E)), 0, st, sa, ncols, es, ps);
+                     dim3(S::C * (N / S::E)), 0, st, sa, ncols);
 }
 // var: bits 0-1 the tile variant, bit 2 the XCD-contiguous tile order
 template <typename T, int N, int DIR, int MODE>
-void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, long long es,
-                    long long ps, int planes) {
+void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, int planes) {
   sa.remap = (var >> 2) & 1;
   switch (var & 3) {
-    case 1: launch_strided_v<T, N, 1, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
-    case 2: launch_strided_v<T, N, 2, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
-    case 3: launch_strided_v<T, N, 3, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
-    default: launch_strided_v<T, N, 0, DIR, MODE>(st, sa, ncols, es, ps, planes); break;
+    case 1: launch_strided_v<T, N, 1, DIR, MODE>(st, sa, ncols, planes); break;
+    case 2: launch_strided_v<T, N, 2, DIR, MODE>(st, sa, ncols, planes); break;
+    case 3: launch_strided_v<T, N, 3, DIR, MODE>(st, sa, ncols, planes); break;
+    default: launch_strided_v<T, N, 0, DIR, MODE>(st, sa, ncols, planes); break;
   }
 }
 
@@ -376,13 +374,12 @@ void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep) {
   sa.sstride = ns_;
   sa.keep = keep;
   prof_begin(PC_YFWD);
-  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep >= 0 ? keep + 1 : PX, (long long)PP,
-                              (long long)N * PP, N * nb);
+  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep >= 0 ? keep + 1 : PX, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2 * sb);
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
   prof_begin(PC_ZFWD);
-  launch_strided<T, N, -1, 2>(svar_[1], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nb);
+  launch_strided<T, N, -1, 2>(svar_[1], st_, sa, N * PP, nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZFWD, 2 * sb);
 }
@@ -491,7 +488,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   }
   const double rb = (double)nb * real_bytes(), sb = (double)spec_bytes();
   prof_begin(PC_ZINV);
-  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, (long long)N * PP, (long long)ns_, nz);
+  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, nz);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZINV, nz * sb + zin * spec_bytes());
   prof_begin(PC_YINV);
@@ -507,8 +504,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
       ++nf;
     }
     if (nf == 0) continue;
-    launch_strided<T, N, 1, 0>(svar_[3], st_, ya, keep >= 0 ? keep + 1 : PX, (long long)PP,
-                               (long long)N * PP, N * nf);
+    launch_strided<T, N, 1, 0>(svar_[3], st_, ya, keep >= 0 ? keep + 1 : PX, N * nf);
     MLQ_HIP(hipGetLastError());
   }
   prof_end(PC_YINV, 2 * ny * sb);
@@ -642,10 +638,10 @@ double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
   MLQ_HIP(hipEventCreate(&z));
   auto run = [&]() {
     switch (pass) {
-      case 0: launch_strided<T, N, -1, 0>(var, st_, sa, PX, PP, (long long)N * PP, N * batch); break;
-      case 1: launch_strided<T, N, -1, 2>(var, st_, sa, N * PP, (long long)N * PP, ns_, batch); break;
-      case 2: launch_strided<T, N, 1, 1>(var, st_, sa, N * PP, (long long)N * PP, ns_, batch); break;
-      default: launch_strided<T, N, 1, 0>(var, st_, sa, PX, PP, (long long)N * PP, N * batch); break;
+      case 0: launch_strided<T, N, -1, 0>(var, st_, sa, PX, N * batch); break;
+      case 1: launch_strided<T, N, -1, 2>(var, st_, sa, N * PP, batch); break;
+      case 2: launch_strided<T, N, 1, 1>(var, st_, sa, N * PP, batch); break;
+      default: launch_strided<T, N, 1, 0>(var, st_, sa, PX, N * batch); break;
     }
   };
   run();  // warm-up

@@ -1022,7 +1022,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   sa.sstride = ns_;
   sa.keep = keep;
   prof_begin(PC_YFWD);
-  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep + 1, (long long)PP, (long long)N * PP, N * nb);
+  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep + 1, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2.0 * nb * spec_bytes());
   prof_begin(PC_ZFWD);

@@ -308,9 +308,12 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 // reading scratch, writing factor * result to dst (epilogue).
 // ---------------------------------------------------------------------------
 template <typename T, int N, int E, int C, int DIR, int MODE>
-__global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, long long estride,
-                                                       long long pstride) {
+__global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
+  // element and plane strides are fixed by the pass (y: rows of one z plane;
+  // z: planes of one spectrum); element offsets within a column fit 32 bits
+  constexpr int estride = MODE == 0 ? PP : N * PP;
+  constexpr long long pstride = MODE == 0 ? (long long)N * PP : (long long)N * N * PP;
   __shared__ c2<T> sm[N * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   unsigned bx, by;
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     if (MODE == 2 && valid) {
       c2<T>* d = a.dst[plane] + col;
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+      for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
     }
     return;
   }
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
   // loads stay unconditional (no per-element branch)
   auto ldrow = [&](const c2<T>* base, int n) {
     const bool r = rowin(n);
-    const c2<T> x = base[(long long)(r ? n : 0) * estride];
+    const c2<T> x = base[(r ? n : 0) * estride];
     return r ? x : cmk<T>((T)0, (T)0);
   };
   if (MODE == 1) {
@@ -437,7 +440,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     c2<T>* d = a.dst[bb] + col;
     if (!kept) {   // outside the kept cube: the masked spectrum is zero
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+      for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
     } else if (op.flags != 0 || op.scale != 1.0) {
       // rolled epilogue through the thread's own LDS slots (every read of the
       // last FFT pass precedes its barrier, so the slots are free)
@@ -447,11 +450,11 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
       for (int m = 0; m < E; ++m) {
         const int kz = t + m * Tf;
         const c2<double> p = cmul(ccvt<double>(sm[kz * C + c]), spec_factor(op, kx, ky, kz, N));
-        d[(long long)kz * estride] = cmk<T>((T)p.x, (T)p.y);
+        d[kz * estride] = cmk<T>((T)p.x, (T)p.y);
       }
     } else {
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[(long long)(t + m * Tf) * estride] = v[m];
+      for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = v[m];
     }
   } else {
     // y passes and the inverse z pass: the forward y pass stores only the
@@ -465,7 +468,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols, lo
     for (int m = 0; m < E; ++m) {
       const int n = t + m * Tf;
       if (MODE == 0 && DIR < 0 && keep >= 0 && mabs(n, N) > keep) continue;
-      d[(long long)n * estride] = v[m];
+      d[n * estride] = v[m];
     }
   }
 }
