@@ -85,9 +85,9 @@ def fft_roofline(g, batch=12, reps=5):
 
 def pmc_traffic(kernel, n, precision):
     """HBM bytes per launch of one pass kernel (batch 12), from the committed
-    summary of the FETCH_SIZE / WRITE_SIZE counter runs of `bench.py
-    --fft-only` (counters scaled by the copy-kernel calibration).  None when
-    no summary for this size and precision is present."""
+    summary of the FETCH_SIZE / WRITE_SIZE counter runs of this benchmark's
+    transform sweep (counters scaled by the copy-kernel calibration of the
+    same run).  None when no summary for this size and precision is present."""
     tag = "fp32" if precision == "mixed" else "fp64"
     path = os.path.join(ROOT, "profiles", f"pmc_fft_passes_{n}_{tag}.md")
     if not os.path.exists(path):
@@ -272,6 +272,10 @@ def main():
     mem_used, mem_peak, _, _ = gm.mem_info()
     fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
+    # the practical ceiling of the same sweep: a plain copy of 12 fields
+    rbm, _ = gm.sizes()
+    tcopy = gm.time_copy(12, 5)
+    copy_gbs = 2 * 12 * rbm / (tcopy * 1e-3) / 1e9
 
     t_fp64 = None
     t_ref = None
@@ -331,7 +335,8 @@ def main():
                    "dR_over_R": dR, "n_J_per_iter_mixed": nJm, "mem_peak_GB_mixed": mem_peak / 1e9,
                    "n_J_poly_fraction_mixed": resm.n_J_poly / max(resm.n_J_computed, 1),
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
-                   "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()}},
+                   "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()},
+                   "copy_12_fields_GBs": round(copy_gbs, 1)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

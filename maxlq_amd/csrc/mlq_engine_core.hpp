@@ -1,5 +1,7 @@
 // Engine<T> definitions: memory, transforms, primitives.
 #pragma once
+#include <cstdio>
+#include <cstdlib>
 #include "mlq_engine_impl.hpp"
 
 namespace mlq {
@@ -73,6 +75,8 @@ template <typename T> void Engine<T>::resolve_params() {
 
 template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
   void* p = nullptr;
+  static const bool log = std::getenv("MLQ_ALLOC_LOG") != nullptr;
+  if (log) std::fprintf(stderr, "[mlq] device allocation of %zu bytes (in use %zu)\n", bytes, used_);
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     // the pool caches freed temporaries by size; give them back and retry

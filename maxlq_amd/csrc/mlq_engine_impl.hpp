@@ -225,7 +225,8 @@ template <typename T> class Engine final : public IEngine {
   // a few more real fields for the |phi|^k chain at other q)
   double fast_path_bytes() const {
     const bool q4 = std::fabs(p_.q - 4.0) < 1e-14;
-    return (q4 ? 48.0 : 56.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+    // + 9 fields: the velocity-gradient operand of the deferred state update
+    return (q4 ? 57.0 : 65.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
   }
  public:
   bool uses_fast_path() const override { return fast_path_ok(); }

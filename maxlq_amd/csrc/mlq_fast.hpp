@@ -80,7 +80,15 @@ template <typename T> class FastRcg {
   // ---- state from a physical field ----------------------------------------------
   void init_state(const T* u_in) {
     g_pending_ = false;
-    Gd_prev_ = RT();
+    // Held like the buffers that later iterations keep alive across the
+    // direction update (the deferred update's operand; the previous
+    // gradient and direction spectra of the CG recursion) and released by
+    // the first direction: the first iteration then reaches the steady-state
+    // footprint and later ones allocate nothing (a multi-GB hipMalloc
+    // inside an iteration costs ~0.1 s).
+    Gd_prev_ = e_.rtmp(9);
+    warm_[0] = e_.stmp(3);
+    warm_[1] = e_.stmp(3);
     u_ = e_.rtmp(3);
     e_.map(F_copy<T>{u_in, u_}, nr_ * 3);
     uh_ = e_.stmp(3);
@@ -269,6 +277,8 @@ template <typename T> class FastRcg {
                              c, alpha, beta * vd_B_}, nk_, PC_SPECTRAL);
     gam_ = have_cg_ ? beta * vd_u_ : 0.0;
     dB_ = std::move(dBn);
+    warm_[0] = ST();
+    warm_[1] = ST();
     dB_limited_ = true;
     gu1n_ = gu1n;
     // zero-mode checks (:251-261): the k = 0 coefficient is the unnormalised mean
@@ -770,6 +780,7 @@ template <typename T> class FastRcg {
   RT d_, Gd_, S1_, S2_, b_, ee_, Q1_, Q2_, P_;
   // deferred update G := gA_ G + gB_ Gd_prev_ (applied by the next direction)
   RT Gd_prev_;
+  ST warm_[2];   // see init_state
   double gA_ = 1.0, gB_ = 0.0;
   bool g_pending_ = false;
   ST P12h_, Q12h_;
