@@ -44,12 +44,14 @@ def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0, flags=0):
     g.seeded_field(h, seed, "gauss", a)
     g.lib.mlq_sync(g.ctx)
     stamps = {}
+    counts = {}
     nJ = {}
     cur = {"it": 0, "nJ": 0}
 
     def cb(user, kind, it, i0, i1, d):
         if kind == capi.MLQ_LOG_ITER:
             stamps[i0] = time.perf_counter()
+            counts[i0] = g.transform_counts()
             log(f"{precision} iter {i0} J={d[2]:.10e} tau={d[0]:.4e} beta={d[1]:.4e}")
         elif kind == capi.MLQ_LOG_BRENT and i0 == 999:
             pass
@@ -64,6 +66,7 @@ def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0, flags=0):
         raise RuntimeError(f"{precision}: solve stopped after {res.iters} iterations "
                            f"(reason {res.stop_reason})")
     t_it = (stamps[warmup + steps] - stamps[warmup]) / steps
+    g.tr_per_iter = tuple((counts[warmup + steps][k] - counts[warmup][k]) / steps for k in (0, 1))
     return g, h, res, t_it, t1 - t0
 
 
@@ -336,7 +339,10 @@ def main():
                    "n_J_poly_fraction_mixed": resm.n_J_poly / max(resm.n_J_computed, 1),
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()},
-                   "copy_12_fields_GBs": round(copy_gbs, 1)},
+                   "copy_12_fields_GBs": round(copy_gbs, 1),
+                   "transforms_per_iter_mixed": {"forward": gm.tr_per_iter[0],
+                                                 "inverse": gm.tr_per_iter[1],
+                                                 "reference_c2c": 548 + 94 * nJm}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -407,4 +407,10 @@ int mlq_mem_info(mlq_ctx* ctx, double* used, double* peak, double* dfree, double
   });
 }
 
+int mlq_transform_counts(const mlq_ctx* ctx, long long* fwd_fields, long long* inv_fields) {
+  if (!ctx || !fwd_fields || !inv_fields) return MLQ_EINVAL;
+  ctx->eng->transform_counts(fwd_fields, inv_fields);
+  return MLQ_OK;
+}
+
 }  // extern "C"

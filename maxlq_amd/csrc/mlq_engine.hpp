@@ -98,6 +98,7 @@ class IEngine {
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;
   virtual void mem_info(double* used, double* peak) = 0;
+  virtual void transform_counts(long long* fwd, long long* inv) const = 0;
 };
 
 std::unique_ptr<IEngine> make_engine_fp64(int n, const mlq_params& p);

@@ -355,6 +355,7 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   xa.scale = (T)1;
   const int keep = fwd_keep(nb, ops);
   xa.keep = keep;
+  nfwd_ += nb;
   const long long pairs = (long long)nb * N * N / 2;
   prof_begin(PC_XFWD);
   if (keep >= 0)
@@ -409,6 +410,7 @@ void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool
     MLQ_HIP(hipGetLastError());
     prof_end(PC_XFWD, P::NB * spec_bytes());
     fwd_yz_n<N>(P::NB, dst, ops, keep);
+    nfwd_ += P::NB;
     done = true;
   }
 }
@@ -427,6 +429,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   }
   if (km >= 0 && (keep < 0 || km < keep)) keep = km;
   if (keep >= N / 2) keep = -1;
+  ninv_ += nb;
   // Factored plan.  The i K_x and i K_y parts of a spectral factor commute
   // with the z pass (i K_x also with the y pass), so they are applied in the
   // x and y passes instead, and outputs that differ only in them share their

@@ -100,6 +100,11 @@ template <typename T> class Engine final : public IEngine {
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
   void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }
+  void transform_counts(long long* fwd, long long* inv) const override {
+    *fwd = nfwd_;
+    *inv = ninv_;
+  }
+  long long nfwd_ = 0, ninv_ = 0;   // fields transformed since creation
 
   // pool of device temporaries, keyed by size
   RT rtmp(int k = 1);

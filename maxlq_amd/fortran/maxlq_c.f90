@@ -366,6 +366,12 @@ module maxlq_c
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: used_bytes, peak_bytes, device_free, device_total
       end function
+      integer(c_int) function mlq_transform_counts(ctx, fwd_fields, inv_fields) &
+            bind(C, name="mlq_transform_counts")
+         import :: c_ptr, c_int, c_long_long
+         type(c_ptr), value :: ctx
+         integer(c_long_long), intent(out) :: fwd_fields, inv_fields
+      end function
    end interface
 
 end module maxlq_c

@@ -351,3 +351,9 @@ class MaxLq:
         v = [C.c_double() for _ in range(4)]
         self._chk(self.lib.mlq_mem_info(self.ctx, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def transform_counts(self):
+        """(forward, inverse) fields transformed since the context was created."""
+        f, i = C.c_longlong(), C.c_longlong()
+        self._chk(self.lib.mlq_transform_counts(self.ctx, C.byref(f), C.byref(i)))
+        return f.value, i.value

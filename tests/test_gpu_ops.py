@@ -212,3 +212,16 @@ def test_fortran_caller_smoke():
     assert os.path.exists(exe), "run __graft_entry__.build() first"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+def test_transform_counts():
+    """mlq_transform_counts: one scalar R2C/C2R pair counts one field each
+    way; a 3-component gradient inverse counts 3 inverse fields."""
+    n = 16
+    g = _ctx(n)
+    f0, i0 = g.transform_counts()
+    u = np.random.default_rng(7).standard_normal((n, n, n))
+    F = g.fftfwdr(u)
+    g.fftbwdr(F)
+    f1, i1 = g.transform_counts()
+    assert (f1 - f0, i1 - i0) == (1, 1)
