@@ -216,7 +216,7 @@ def test_fortran_caller_smoke():
 
 def test_transform_counts():
     """mlq_transform_counts: one scalar R2C/C2R pair counts one field each
-    way; a 3-component gradient inverse counts 3 inverse fields."""
+    way."""
     n = 16
     g = _ctx(n)
     f0, i0 = g.transform_counts()
