@@ -256,10 +256,10 @@ template <typename T> struct P_grad_terms {
 template <typename T> struct S_hilbert_leray {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px; double lam, e2s, scale;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
@@ -289,10 +289,10 @@ template <typename T> struct S_hilbert_leray {
 template <typename T> struct S_dir_prep {
   c2<T>* V; const c2<T>* W; c2<T>* nH; long long ns; int n, px; double lam, e2s, nu, sW;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
@@ -336,10 +336,10 @@ template <typename T> struct R_dir {
   const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px;
   double lam, s;
   __device__ void operator()(long long i, double (&acc)[11]) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double K = TWO_PI * sqrt((double)mx * mx + (double)my * my + (double)mz * mz);
@@ -389,10 +389,11 @@ struct KeptIdx { int kx, my, mz, m2; long long off; };
 __device__ __forceinline__ KeptIdx kept_decode(long long i, int n, int px, int M) {
   const int W = 2 * M + 1;
   KeptIdx k;
-  k.kx = (int)(i % (M + 1));
-  const long long r = i / (M + 1);
-  k.my = (int)(r % W) - M;
-  k.mz = (int)(r / W) - M;
+  const unsigned ii = (unsigned)i, r = ii / (unsigned)(M + 1);
+  k.kx = (int)(ii - r * (unsigned)(M + 1));
+  const unsigned rz = r / (unsigned)W;
+  k.my = (int)(r - rz * (unsigned)W) - M;
+  k.mz = (int)rz - M;
   const int ky = k.my < 0 ? k.my + n : k.my, kz = k.mz < 0 ? k.mz + n : k.mz;
   k.off = ((long long)kz * n + ky) * px + k.kx;
   k.m2 = k.kx * k.kx + k.my * k.my + k.mz * k.mz;
@@ -527,10 +528,10 @@ template <typename T> struct R_uh_update {
   c2<T>* U; const c2<T>* D; long long ns; int n, px, M; bool dlim; double a, b;
   const double* hl;
   __device__ void operator()(long long i, double (&acc)[1]) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const bool in = !dlim || (kx <= M && my <= M && my >= -M && mz <= M && mz >= -M);
@@ -590,10 +591,10 @@ template <typename T> struct S_axpby3 {
 template <typename T> struct S_add_lap {
   c2<T>* V; const c2<T>* W; long long ns; int n, px; double c;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double ksq = TWO_PI * TWO_PI * ((double)kx * kx + (double)my * my + (double)mz * mz);
@@ -612,10 +613,10 @@ template <typename T> struct R_specdot3 {
   static constexpr int unroll = 1;
   const c2<T>* a; const c2<T>* b; long long ns; int n, px, kind; double lam, s;
   __device__ void operator()(long long i, double (&acc)[1]) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     double w = 1.0;
@@ -637,10 +638,10 @@ template <typename T> struct R_specdot3 {
 template <typename T> struct R_divnorm {
   const c2<T>* g; long long ns; int n, px;
   __device__ void operator()(long long i, double (&acc)[1]) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double e1 = mx == h ? 0.0 : TWO_PI * mx, e2 = my == h ? 0.0 : TWO_PI * my,
@@ -662,9 +663,10 @@ template <typename T> struct R_ls_spec {
   double tau[MAXB]; int nb, n, px, M;
   __device__ void operator()(long long i, double (&acc)[2 * MAXB]) const {
     const int W = 2 * M + 1;
-    const int kx = (int)(i % (M + 1));
-    const long long r = i / (M + 1);
-    const int jy = (int)(r % W), jz = (int)(r / W);
+    const unsigned ii = (unsigned)i, r = ii / (unsigned)(M + 1);
+    const int kx = (int)(ii - r * (unsigned)(M + 1));
+    const unsigned rz = r / (unsigned)W;
+    const int jy = (int)(r - rz * (unsigned)W), jz = (int)rz;
     const int my = jy - M, mz = jz - M;
     const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
     const long long off = ((long long)kz * n + ky) * px + kx;
@@ -719,13 +721,9 @@ template <typename T> struct F_pB {
 template <typename T> struct R_kept_dot3 {
   const c2<T>* W; const c2<T>* x; const c2<T>* y; const c2<T>* z; long long ns; int n, px, M;
   __device__ void operator()(long long i, double (&acc)[3]) const {
-    const int Wd = 2 * M + 1;
-    const int kx = (int)(i % (M + 1));
-    const long long r = i / (M + 1);
-    const int jy = (int)(r % Wd), jz = (int)(r / Wd);
-    const int my = jy - M, mz = jz - M;
-    const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
-    const long long off = ((long long)kz * n + ky) * px + kx;
+    const KeptIdx k = kept_decode(i, n, px, M);
+    const int kx = k.kx;
+    const long long off = k.off;
     const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -751,9 +749,10 @@ template <typename T> struct R_ls_poly4 {
   const c2<T>* Q0; const c2<T>* Q1; const c2<T>* Q2; int n, px, M;
   __device__ void operator()(long long i, double (&acc)[10]) const {
     const int W = 2 * M + 1;
-    const int kx = (int)(i % (M + 1));
-    const long long r = i / (M + 1);
-    const int jy = (int)(r % W), jz = (int)(r / W);
+    const unsigned ii = (unsigned)i, r = ii / (unsigned)(M + 1);
+    const int kx = (int)(ii - r * (unsigned)(M + 1));
+    const unsigned rz = r / (unsigned)W;
+    const int jy = (int)(r - rz * (unsigned)W), jz = (int)rz;
     const int my = jy - M, mz = jz - M;
     const int ky = my < 0 ? my + n : my, kz = mz < 0 ? mz + n : mz;
     const long long off = ((long long)kz * n + ky) * px + kx;

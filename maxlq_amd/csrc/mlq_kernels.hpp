@@ -26,15 +26,26 @@ template <typename T> constexpr int spec_pitch(int n) {
   return ((n / 2 + 1) + a - 1) / a * a;
 }
 
+// Flat spectral indices are below 2^31 (one half spectrum, padding
+// included, up to 1024^3), so they decode in 32-bit arithmetic; n is a
+// power of two.
+struct RowIdx { unsigned row; int kx; };
+__device__ __forceinline__ RowIdx row_split(long long i, int pp) {
+  const unsigned ii = (unsigned)i, row = ii / (unsigned)pp;
+  return RowIdx{row, (int)(ii - row * (unsigned)pp)};
+}
+__device__ __forceinline__ int row_ky(unsigned row, int n) { return (int)(row & (unsigned)(n - 1)); }
+__device__ __forceinline__ int row_kz(unsigned row, int n) { return (int)(row >> (__ffs(n) - 1)); }
+
 // Decode a flat half-spectrum index (pitch pp) into mode indices; false on
 // padding.  m_y, m_z in (-n/2, n/2] (Nyquist counted +n/2).
 struct KIdx { int kx, ky, kz, mx, my, mz; };
 __device__ __forceinline__ bool kdecode(long long i, int n, int pp, KIdx& k) {
-  const long long row = i / pp;
-  k.kx = (int)(i - row * pp);
+  const RowIdx r = row_split(i, pp);
+  k.kx = r.kx;
   if (k.kx > (n >> 1)) return false;
-  k.ky = (int)(row % n);
-  k.kz = (int)(row / n);
+  k.ky = row_ky(r.row, n);
+  k.kz = row_kz(r.row, n);
   const int h = n >> 1;
   k.mx = k.kx;
   k.my = k.ky <= h ? k.ky : k.ky - n;

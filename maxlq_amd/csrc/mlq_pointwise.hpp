@@ -195,10 +195,10 @@ template <typename T> struct R_specdot {
   static constexpr int unroll = 1;
   const c2<T>* a; const c2<T>* b; int n, px, kind; double lam, s;
   __device__ void operator()(long long i, double (&acc)[1]) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     double w = 1.0;
@@ -219,10 +219,10 @@ template <typename T> struct R_specdot {
 template <typename T> struct S_leray {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
@@ -246,10 +246,10 @@ template <typename T> struct S_leray {
 template <typename T> struct S_hilbert3 {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px; double lam, e;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
@@ -264,10 +264,10 @@ template <typename T> struct S_hilbert3 {
 template <typename T> struct S_div {
   const c2<T>* w0; const c2<T>* w1; const c2<T>* w2; c2<T>* o; int n, px;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double e1 = mx == h ? 0.0 : TWO_PI * mx, e2 = my == h ? 0.0 : TWO_PI * my,
@@ -290,10 +290,10 @@ template <typename T> struct S_axpby {
 template <typename T> struct S_envelope_leray {
   c2<T>* u0; c2<T>* u1; c2<T>* u2; int n, px, kind; double a;
   __device__ void operator()(long long i) const {
-    const long long row = i / px;
-    const int kx = (int)(i - row * px);   // px: row pitch
+    const RowIdx ri = row_split(i, px);   // px: row pitch
+    const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
-    const int ky = (int)(row % n), kz = (int)(row / n);
+    const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int mx = kx, my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
     const double m2 = (double)mx * mx + (double)my * my + (double)mz * mz;
