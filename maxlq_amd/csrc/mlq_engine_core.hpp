@@ -17,6 +17,10 @@ template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
   ns_ = (long long)n * n * pp_;
   resolve_params();
   MLQ_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  {
+    const char* f = std::getenv("MLQ_FUSED");
+    fused_ = !(f && f[0] == '0');
+  }
   // twiddles exp(-2 pi i k/N), computed in FP64 on the host
   std::vector<double2> tw(n);
   for (int k = 0; k < n; ++k) {
@@ -556,7 +560,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
 // hold the batch, the caller then materialises the fields
 template <typename T> template <class P>
 bool Engine<T>::fwd_prod(const P& prod, C* const* dst, const SpecOp* ops) {
-  if (P::NB > sb_) return false;
+  if (!fused_ || P::NB > sb_) return false;
   bool done = false;
   MLQ_DISPATCH_N(fwd_prod_n, prod, dst, ops, done);
   return done;

@@ -105,6 +105,10 @@ template <typename T> class Engine final : public IEngine {
     *inv = ninv_;
   }
   long long nfwd_ = 0, ninv_ = 0;   // fields transformed since creation
+  // producer x passes (k_xfwd_p) enabled; MLQ_FUSED=0 at context creation
+  // selects the pointwise-kernel + transform form instead (same results up to
+  // rounding; kept for tests of that form)
+  bool fused_ = true;
 
   // pool of device temporaries, keyed by size
   RT rtmp(int k = 1);

@@ -76,6 +76,23 @@ def test_rcg_fast_matches_reference_path_32():
     assert r2.n_J_computed <= r2.n_J + 2 * r2.iters
 
 
+def test_rcg_fast_unfused_matches_fused_32(monkeypatch):
+    """The producer x passes (pointwise work inside the forward transform)
+    against the pointwise-kernel + transform form (MLQ_FUSED=0): same solve,
+    FP64, to rounding."""
+    n, q, B, iters = 32, 4.0, 1.0, 3
+    u0 = seeded_field(n, 2, "gauss", 16.0, q=q, B=B)
+    r1, t1, u1 = _solve_gpu(u0, n, q, B, iters, flags=0, batch=3)
+    monkeypatch.setenv("MLQ_FUSED", "0")
+    r2, t2, u2 = _solve_gpu(u0, n, q, B, iters, flags=0, batch=3)
+    assert r1.iters == r2.iters and r1.n_J == r2.n_J
+    for a, b in zip(t1["br"], t2["br"]):
+        assert a[:2] == b[:2]
+        assert abs(a[3] - b[3]) <= 1e-12 * abs(a[3])
+    assert rel(u2, u1) < 1e-12
+    assert abs(r2.J1 - r1.J1) <= 1e-12 * abs(r1.J1)
+
+
 def test_rcg_fast_with_clipping_matches_reference_path():
     """A rough field (flat spectrum up to |m| = n/2): the 2/3 mask truncates
     u.u and D(phi.phi) goes negative at some points, so the line search mixes
