@@ -398,12 +398,42 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
                       (T)(c0 * (double)v[m].y + ci * (double)ld[m].y));
       c0 = 1.0;
     }
-    // Spectral factor (uniform branch: op and c0 are per plane).  The tile
-    // goes through the thread's own LDS slots so that the factor code runs
-    // in a rolled loop, one element at a time, and does not multiply the
-    // register footprint by E; the barrier orders these slot accesses before
-    // the first FFT pass writes other threads' slots.
-    if (op.flags != 0 || op.scale != 1.0 || c0 != 1.0) {
+    // Spectral factor (uniform branch: op and c0 are per plane).  A plain
+    // scale, i K_z (the z part of a factored derivative) and -|K|^2 have a
+    // closed form along the column and are applied in registers.  Other
+    // factors: the tile goes through the thread's own LDS slots so that the
+    // factor code runs in a rolled loop, one element at a time, and does not
+    // multiply the register footprint by E; the barrier orders these slot
+    // accesses before the first FFT pass writes other threads' slots.
+    const double fs = c0 * op.scale;
+    const int fl = op.flags;
+    if (fl == 0 || (fl == SP_IK && op.dim == 2) || fl == SP_NEGKSQ) {
+      if (fl != 0 || fs != 1.0) {
+        constexpr int h = N / 2;
+        double kxy = 0.0;
+        if (fl == SP_NEGKSQ) {
+          const double K1 = TWO_PI * kx, K2 = TWO_PI * (ky <= h ? ky : ky - N);
+          kxy = K1 * K1 + K2 * K2;
+        }
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int n = t + m * Tf;
+          const double Kz = TWO_PI * (n <= h ? n : n - N);
+          const c2<double> x = ccvt<double>(v[m]);
+          c2<double> r;
+          if (fl == 0) {
+            r = cmk<double>(fs * x.x, fs * x.y);
+          } else if (fl == SP_NEGKSQ) {
+            const double f = -fs * (kxy + Kz * Kz);
+            r = cmk<double>(f * x.x, f * x.y);
+          } else {   // i K_z, zero at the own-dimension Nyquist
+            const double f = n == h ? 0.0 : fs * Kz;
+            r = cmk<double>(-f * x.y, f * x.x);
+          }
+          v[m] = cmk<T>((T)r.x, (T)r.y);
+        }
+      }
+    } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) sm[(t + m * Tf) * C + c] = v[m];
 #pragma unroll 1
@@ -452,7 +482,22 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
     if (!kept) {   // outside the kept cube: the masked spectrum is zero
 #pragma unroll
       for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
-    } else if (op.flags != 0 || op.scale != 1.0) {
+    } else if ((op.flags & ~SP_MASK) == 0) {
+      // a scale and the cube mask: closed form along the column, in registers
+      // (the column's kx, ky are inside the cube when keep >= 0 made it kept)
+      constexpr int h = N / 2;
+      const double sc = op.scale;
+      const int mxy = max(kx, abs(ky <= h ? ky : ky - N));
+      const bool masked = (op.flags & SP_MASK) != 0;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = t + m * Tf;
+        const int mz = abs(n <= h ? n : n - N);
+        const bool in = !masked || max(mxy, mz) <= op.mcut;
+        const c2<double> x = ccvt<double>(v[m]);
+        d[n * estride] = in ? cmk<T>((T)(sc * x.x), (T)(sc * x.y)) : cmk<T>((T)0, (T)0);
+      }
+    } else {
       // rolled epilogue through the thread's own LDS slots (every read of the
       // last FFT pass precedes its barrier, so the slots are free)
 #pragma unroll
@@ -463,9 +508,6 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
         const c2<double> p = cmul(ccvt<double>(sm[kz * C + c]), spec_factor(op, kx, ky, kz, N));
         d[kz * estride] = cmk<T>((T)p.x, (T)p.y);
       }
-    } else {
-#pragma unroll
-      for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = v[m];
     }
   } else {
     // y passes and the inverse z pass: the forward y pass stores only the
