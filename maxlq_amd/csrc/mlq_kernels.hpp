@@ -142,6 +142,38 @@ __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, 
   return f;
 }
 
+// The spectral factor along one z column (fixed kx, ky): the same values as
+// spec_factor for every op whose i K part (if any) is along z, with the
+// column's terms precomputed (cz: the column's part of the mask and |K|^2).
+struct ColFactor {
+  int fl, mcut, mxy;   // mxy = max(|m_x|, |m_y|)
+  double s, kxy;       // scale, K_x^2 + K_y^2
+};
+__device__ __forceinline__ bool col_factor_ok(const SpecOp& o) {
+  return !(o.flags & SP_IK) || o.dim == 2;
+}
+__device__ __forceinline__ ColFactor col_factor_init(const SpecOp& o, double scale, int kx, int ky,
+                                                     int n) {
+  const int h = n >> 1, my = ky <= h ? ky : ky - n;
+  const double K1 = TWO_PI * kx, K2 = TWO_PI * my;
+  return ColFactor{o.flags, o.mcut, max(kx, abs(my)), scale, K1 * K1 + K2 * K2};
+}
+__device__ __forceinline__ double2 col_factor(const ColFactor& c, int kz, int n) {
+  const int h = n >> 1, mz = kz <= h ? kz : kz - n;
+  const int fl = c.fl;
+  double f = c.s;
+  if ((fl & SP_MASK) && max(c.mxy, abs(mz)) > c.mcut) return make_double2(0.0, 0.0);
+  const double Kz = TWO_PI * mz, ksq = c.kxy + Kz * Kz;
+  if ((fl & SP_ZEROK0) && ksq == 0.0) return make_double2(0.0, 0.0);
+  if (fl & SP_NEGKSQ) f *= -ksq;
+  if (fl & SP_INVKSQ) {
+    if (ksq == 0.0) return make_double2(0.0, 0.0);
+    f *= -1.0 / ksq;
+  }
+  if (fl & SP_IK) return make_double2(0.0, mz == h ? 0.0 : f * Kz);   // i K_z
+  return make_double2(f, 0.0);
+}
+
 template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, double2 f) {
   const T fx = (T)f.x, fy = (T)f.y;
   return cmk<T>(v.x * fx - v.y * fy, v.x * fy + v.y * fx);
@@ -398,38 +430,21 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
                       (T)(c0 * (double)v[m].y + ci * (double)ld[m].y));
       c0 = 1.0;
     }
-    // Spectral factor (uniform branch: op and c0 are per plane).  A plain
-    // scale, i K_z (the z part of a factored derivative) and -|K|^2 have a
-    // closed form along the column and are applied in registers.  Other
-    // factors: the tile goes through the thread's own LDS slots so that the
-    // factor code runs in a rolled loop, one element at a time, and does not
-    // multiply the register footprint by E; the barrier orders these slot
-    // accesses before the first FFT pass writes other threads' slots.
+    // Spectral factor (uniform branch: op and c0 are per plane).  Factors
+    // whose i K part (if any) is along z are applied in registers from the
+    // column's precomputed terms.  Others (i K_x or i K_y, unfactored): the
+    // tile goes through the thread's own LDS slots so that the factor code
+    // runs in a rolled loop, one element at a time, and does not multiply the
+    // register footprint by E; the barrier orders these slot accesses before
+    // the first FFT pass writes other threads' slots.
     const double fs = c0 * op.scale;
-    const int fl = op.flags;
-    if (fl == 0 || (fl == SP_IK && op.dim == 2) || fl == SP_NEGKSQ) {
-      if (fl != 0 || fs != 1.0) {
-        constexpr int h = N / 2;
-        double kxy = 0.0;
-        if (fl == SP_NEGKSQ) {
-          const double K1 = TWO_PI * kx, K2 = TWO_PI * (ky <= h ? ky : ky - N);
-          kxy = K1 * K1 + K2 * K2;
-        }
+    if (col_factor_ok(op)) {
+      if (op.flags != 0 || fs != 1.0) {
+        const ColFactor cf = col_factor_init(op, fs, kx, ky, N);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-          const int n = t + m * Tf;
-          const double Kz = TWO_PI * (n <= h ? n : n - N);
-          const c2<double> x = ccvt<double>(v[m]);
-          c2<double> r;
-          if (fl == 0) {
-            r = cmk<double>(fs * x.x, fs * x.y);
-          } else if (fl == SP_NEGKSQ) {
-            const double f = -fs * (kxy + Kz * Kz);
-            r = cmk<double>(f * x.x, f * x.y);
-          } else {   // i K_z, zero at the own-dimension Nyquist
-            const double f = n == h ? 0.0 : fs * Kz;
-            r = cmk<double>(-f * x.y, f * x.x);
-          }
+          const double2 f = col_factor(cf, t + m * Tf, N);
+          const c2<double> r = cmul(ccvt<double>(v[m]), f);
           v[m] = cmk<T>((T)r.x, (T)r.y);
         }
       }
@@ -482,20 +497,14 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
     if (!kept) {   // outside the kept cube: the masked spectrum is zero
 #pragma unroll
       for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
-    } else if ((op.flags & ~SP_MASK) == 0) {
-      // a scale and the cube mask: closed form along the column, in registers
-      // (the column's kx, ky are inside the cube when keep >= 0 made it kept)
-      constexpr int h = N / 2;
-      const double sc = op.scale;
-      const int mxy = max(kx, abs(ky <= h ? ky : ky - N));
-      const bool masked = (op.flags & SP_MASK) != 0;
+    } else if (col_factor_ok(op)) {
+      // closed form along the column, in registers
+      const ColFactor cf = col_factor_init(op, op.scale, kx, ky, N);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int n = t + m * Tf;
-        const int mz = abs(n <= h ? n : n - N);
-        const bool in = !masked || max(mxy, mz) <= op.mcut;
-        const c2<double> x = ccvt<double>(v[m]);
-        d[n * estride] = in ? cmk<T>((T)(sc * x.x), (T)(sc * x.y)) : cmk<T>((T)0, (T)0);
+        const c2<double> p = cmul(ccvt<double>(v[m]), col_factor(cf, n, N));
+        d[n * estride] = cmk<T>((T)p.x, (T)p.y);
       }
     } else {
       // rolled epilogue through the thread's own LDS slots (every read of the
