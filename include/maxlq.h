@@ -221,9 +221,11 @@ int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms);
 /* tile autotuner of the strided transform passes: times each of 8 tile
  * candidates (4 shapes of elements per thread x columns per workgroup, each
  * in dispatch order and in XCD-contiguous order: candidate = shape + 4*order)
- * of the {y fwd, z fwd, z inv, y inv} passes on `batch` fields and keeps the
- * fastest.  times[pass*8+cand] (ms per launch) and choice[pass] may be null.
- * mlq_init runs it for n >= 128 unless the environment sets MLQ_AUTOTUNE=0. */
+ * of the {y fwd, z fwd, z inv, y inv} passes on `batch` fields, for full
+ * transforms (passes 0-3) and for band-limited ones at the 2/3 dealias cube
+ * (passes 4-7), and keeps the fastest of each.  times[pass*8+cand] (64 values,
+ * ms per launch) and choice[pass] (8 values) may be null.  mlq_init runs it
+ * for n >= 128 unless the environment sets MLQ_AUTOTUNE=0. */
 int mlq_autotune(mlq_ctx* ctx, int batch, double* times, int* choice);
 /* bytes of one real field and of one half spectrum in the active precision */
 int mlq_sizes(const mlq_ctx* ctx, double* real_bytes, double* spec_bytes);

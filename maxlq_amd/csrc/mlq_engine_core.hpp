@@ -383,12 +383,12 @@ void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep) {
   sa.sstride = ns_;
   sa.keep = keep;
   prof_begin(PC_YFWD);
-  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep >= 0 ? keep + 1 : PX, N * nb);
+  launch_strided<T, N, -1, 0>(svar_[keep >= 0 ? 4 : 0], st_, sa, keep >= 0 ? keep + 1 : PX, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2 * sb);
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
   prof_begin(PC_ZFWD);
-  launch_strided<T, N, -1, 2>(svar_[1], st_, sa, N * PP, nb);
+  launch_strided<T, N, -1, 2>(svar_[keep >= 0 ? 5 : 1], st_, sa, N * PP, nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZFWD, 2 * sb);
 }
@@ -499,7 +499,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   }
   const double rb = (double)nb * real_bytes(), sb = (double)spec_bytes();
   prof_begin(PC_ZINV);
-  launch_strided<T, N, 1, 1>(svar_[2], st_, sa, N * PP, nz);
+  launch_strided<T, N, 1, 1>(svar_[keep >= 0 ? 6 : 2], st_, sa, N * PP, nz);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_ZINV, nz * sb + zin * spec_bytes());
   prof_begin(PC_YINV);
@@ -515,7 +515,8 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
       ++nf;
     }
     if (nf == 0) continue;
-    launch_strided<T, N, 1, 0>(svar_[3], st_, ya, keep >= 0 ? keep + 1 : PX, N * nf);
+    launch_strided<T, N, 1, 0>(svar_[keep >= 0 ? 7 : 3], st_, ya, keep >= 0 ? keep + 1 : PX,
+                               N * nf);
     MLQ_HIP(hipGetLastError());
   }
   prof_end(PC_YINV, 2 * ny * sb);
@@ -631,12 +632,14 @@ template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms
 // One strided pass alone, in place on the scratch spectra, timed with HIP
 // events on the engine stream (ms per launch).
 template <typename T> template <int N>
-double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
+double Engine<T>::time_pass_n(int pass, int var, int batch, int reps, int keep) {
   constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
   sa.sstride = ns_;
+  sa.keep = keep;
+  const int ycols = keep >= 0 ? keep + 1 : PX;
   for (int b = 0; b < batch; ++b) {
     sa.src[b][0] = scratch_ + b * ns_;
     sa.coef[b][0] = 1.0;
@@ -649,10 +652,10 @@ double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
   MLQ_HIP(hipEventCreate(&z));
   auto run = [&]() {
     switch (pass) {
-      case 0: launch_strided<T, N, -1, 0>(var, st_, sa, PX, N * batch); break;
+      case 0: launch_strided<T, N, -1, 0>(var, st_, sa, ycols, N * batch); break;
       case 1: launch_strided<T, N, -1, 2>(var, st_, sa, N * PP, batch); break;
       case 2: launch_strided<T, N, 1, 1>(var, st_, sa, N * PP, batch); break;
-      default: launch_strided<T, N, 1, 0>(var, st_, sa, PX, N * batch); break;
+      default: launch_strided<T, N, 1, 0>(var, st_, sa, ycols, N * batch); break;
     }
   };
   run();  // warm-up
@@ -667,32 +670,36 @@ double Engine<T>::time_pass_n(int pass, int var, int batch, int reps) {
   return (double)ms / reps;
 }
 
-// candidates per pass: 4 tile shapes x {dispatch order, XCD-contiguous order}
+// candidates per pass: 4 tile shapes x {dispatch order, XCD-contiguous order};
+// passes 0-3 full transforms, 4-7 band-limited ones (the 2/3 mask's cube)
 constexpr int NTILE = 8;
 template <typename T> void Engine<T>::autotune(int batch, double* times, int* choice) {
   batch = std::max(1, std::min(batch, sb_));
   MLQ_HIP(hipMemsetAsync(scratch_, 0, sizeof(C) * ns_ * batch, st_));
-  for (int pass = 0; pass < 4; ++pass) {
+  const int mc = mcut_of(2.0);
+  for (int slot = 0; slot < 8; ++slot) {
+    const int pass = slot & 3;
+    const int keep = slot >= 4 && !mask_is_identity(mc) ? mc : -1;
     double best = 1e300;
     int bv = 0;
     for (int v = 0; v < NTILE; ++v) {
       double t = 0.0;
       switch (n_) {
-        case 8: t = time_pass_n<8>(pass, v, batch, 3); break;
-        case 16: t = time_pass_n<16>(pass, v, batch, 3); break;
-        case 32: t = time_pass_n<32>(pass, v, batch, 3); break;
-        case 64: t = time_pass_n<64>(pass, v, batch, 3); break;
-        case 128: t = time_pass_n<128>(pass, v, batch, 3); break;
-        case 256: t = time_pass_n<256>(pass, v, batch, 3); break;
-        case 512: t = time_pass_n<512>(pass, v, batch, 3); break;
-        case 1024: t = time_pass_n<1024>(pass, v, batch, 3); break;
+        case 8: t = time_pass_n<8>(pass, v, batch, 3, keep); break;
+        case 16: t = time_pass_n<16>(pass, v, batch, 3, keep); break;
+        case 32: t = time_pass_n<32>(pass, v, batch, 3, keep); break;
+        case 64: t = time_pass_n<64>(pass, v, batch, 3, keep); break;
+        case 128: t = time_pass_n<128>(pass, v, batch, 3, keep); break;
+        case 256: t = time_pass_n<256>(pass, v, batch, 3, keep); break;
+        case 512: t = time_pass_n<512>(pass, v, batch, 3, keep); break;
+        case 1024: t = time_pass_n<1024>(pass, v, batch, 3, keep); break;
         default: throw Error(MLQ_EINVAL, "unsupported n");
       }
-      if (times) times[pass * NTILE + v] = t;
+      if (times) times[slot * NTILE + v] = t;
       if (t < best) { best = t; bv = v; }
     }
-    svar_[pass] = bv;
-    if (choice) choice[pass] = bv;
+    svar_[slot] = bv;
+    if (choice) choice[slot] = bv;
   }
 }
 

@@ -193,8 +193,8 @@ template <typename T> class Engine final : public IEngine {
   int n_, px_, pp_;   // grid size, n/2+1 modes along x, spectral row pitch
   int sb_ = MAXB;     // spectra in the transform scratch (largest transform batch)
   // tile variant of the strided passes {y fwd, z fwd, z inv, y inv} (autotuned)
-  int svar_[4] = {0, 0, 0, 0};
-  template <int N> double time_pass_n(int pass, int var, int batch, int reps);
+  int svar_[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // {y fwd, z fwd, z inv, y inv} x {full, band-limited}
+  template <int N> double time_pass_n(int pass, int var, int batch, int reps, int keep = -1);
   long long nr_, ns_;
   mlq_params p_;
   bool use_e_u_ = false;

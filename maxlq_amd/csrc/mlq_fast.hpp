@@ -995,7 +995,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   constexpr int NF = xpass_NF<N>();
   constexpr int Tf = FftShape<N>::Tf;
   constexpr int PP = spec_pitch<T>(N);
-  const int var = svar_[1];   // tile of the forward z pass
+  const int var = svar_[5];   // tile of the band-limited forward z pass
   int C0 = 0;
   switch (var & 3) {
     case 1: C0 = SVar<T, N, 1>::C; break;
@@ -1033,7 +1033,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   sa.sstride = ns_;
   sa.keep = keep;
   prof_begin(PC_YFWD);
-  launch_strided<T, N, -1, 0>(svar_[0], st_, sa, keep + 1, N * nb);
+  launch_strided<T, N, -1, 0>(svar_[4], st_, sa, keep + 1, N * nb);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2.0 * nb * spec_bytes());
   prof_begin(PC_ZFWD);

@@ -336,8 +336,8 @@ module maxlq_c
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx
          integer(c_int), value :: batch
-         real(c_double), intent(out) :: times(32)
-         integer(c_int), intent(out) :: choice(4)
+         real(c_double), intent(out) :: times(64)
+         integer(c_int), intent(out) :: choice(8)
       end function
       integer(c_int) function mlq_sizes(ctx, real_bytes, spec_bytes) bind(C, name="mlq_sizes")
          import :: c_ptr, c_int, c_double

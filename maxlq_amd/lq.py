@@ -319,12 +319,14 @@ class MaxLq:
         return out
 
     def autotune(self, batch=3):
-        """Time the strided-pass tile candidates; returns (times[4][8] ms,
-        choice[4]); candidate = shape (0-3) + 4 * (XCD-contiguous order)."""
-        t = np.zeros(32)
-        ch = (C.c_int * 4)()
+        """Time the strided-pass tile candidates; returns (times[8][8] ms,
+        choice[8]): passes {y fwd, z fwd, z inv, y inv} for full transforms
+        (0-3) and band-limited ones (4-7); candidate = shape (0-3) + 4 *
+        (XCD-contiguous order)."""
+        t = np.zeros(64)
+        ch = (C.c_int * 8)()
         self._chk(self.lib.mlq_autotune(self.ctx, batch, _p(t), ch))
-        return t.reshape(4, 8), list(ch)
+        return t.reshape(8, 8), list(ch)
 
     def time_copy(self, nfields=3, reps=5):
         r = C.c_double()
