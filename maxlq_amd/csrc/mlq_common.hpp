@@ -37,6 +37,8 @@ struct Error : std::runtime_error {
 //   NEGKSQ  -|K|^2                                   (laplacian, :3272)
 //   INVKSQ  -1/|K|^2, 0 at k = 0                     (solve_poisson, :1977-1981)
 //   ZEROK0  zero the k = 0 mode                      (divAvg_free, :2891-2893)
+//   HSCALE  times ``hs`` outside the cube max_j |m_j| <= mcut (a spectrum
+//           whose high modes are stored up to a pending scalar factor)
 // times the real ``scale``.  (The Sobolev weight of HilbertGradient needs a
 // pow per mode and runs as its own spectral map, so that the pass kernels
 // carrying these factors stay within 64 VGPRs.)
@@ -46,6 +48,7 @@ enum SpecFlags : int {
   SP_NEGKSQ = 4,
   SP_INVKSQ = 8,
   SP_ZEROK0 = 32,
+  SP_HSCALE = 64,
 };
 
 struct SpecOp {
@@ -54,6 +57,7 @@ struct SpecOp {
   int mcut = 1 << 30;
   int pad = 0;
   double scale = 1.0;
+  double hs = 1.0;   // SP_HSCALE factor
 };
 
 inline SpecOp sp_none(double s = 1.0) { SpecOp o; o.scale = s; return o; }

@@ -94,10 +94,14 @@ template <typename T> class FastRcg {
     uh_ = e_.stmp(3);
     spec3(u_, uh_);
     {
-      double r;   // <u_hat, u_hat>_H_l (the update with a = 1, b = 0 rewrites it unchanged)
-      e_.template reduce<1>(R_uh_update<T>{uh_, nullptr, ns_, n_, pp_, -1, true, 1.0, 0.0, hlt_},
-                            ns_, &r);
-      uu_ = r / N6_;
+      // <u_hat, u_hat>_H_l inside and outside the kept cube (the update with
+      // a = 1, b = 0 rewrites u_hat unchanged)
+      double r[2];
+      e_.template reduce<2>(R_uh_update<T>{uh_, nullptr, ns_, n_, pp_, M_, true, 1.0, 1.0, 0.0,
+                                           hlt_}, ns_, r);
+      uu_ = (r[0] + r[1]) / N6_;
+      hsum_ = r[1];
+      hk_ = 1.0;
     }
     G_ = e_.rtmp(9);
     grad9(uh_, G_);
@@ -168,7 +172,11 @@ template <typename T> class FastRcg {
         ops[j] = sp_ik(j);
         dst[j] = gp + j * nr_;
         il[j] = InvSrc{{uh_ + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
-        SpecOp l; l.flags = SP_NEGKSQ; opl[j] = l;
+        SpecOp l;   // -|K|^2, and the pending factor of u_hat outside the cube
+        l.flags = SP_NEGKSQ | SP_HSCALE;
+        l.mcut = M_;
+        l.hs = hk_;
+        opl[j] = l;
         dsl[j] = lap + j * nr_;
       }
       e_.inv(3, is, ops, dst, M_);
@@ -509,11 +517,24 @@ template <typename T> class FastRcg {
     vg_u_ = -c0 / sc;
     vd_B_ = sc / opg;
     vd_u_ = gam_ / opg - c1 / sc;
-    {
+    // u_hat_new = sc opg u_hat + sc tau1 dB.  With dB band-limited only the
+    // kept cube changes: the modes outside are only rescaled, which is kept as
+    // the pending factor hk_ (their stored values times hk_), and their
+    // H_l norm is hk_^2 hsum_.  Otherwise the whole spectrum is updated and the
+    // pending factor applied.
+    if (dB_limited_) {
       double r;
-      e_.template reduce<1>(R_uh_update<T>{uh_, dB_, ns_, n_, pp_, M_, dB_limited_, sc * opg,
-                                           sc * tau1, hlt_}, ns_, &r);
-      uu_ = r / N6_;
+      e_.template reduce<1>(R_uh_update_k<T>{uh_, dB_, ns_, n_, pp_, M_, sc * opg, sc * tau1, hlt_},
+                            nk_, &r);
+      hk_ *= sc * opg;
+      uu_ = (r + hk_ * hk_ * hsum_) / N6_;
+    } else {
+      double r[2];
+      e_.template reduce<2>(R_uh_update<T>{uh_, dB_, ns_, n_, pp_, M_, false, sc * opg,
+                                           sc * opg * hk_, sc * tau1, hlt_}, ns_, r);
+      uu_ = (r[0] + r[1]) / N6_;
+      hsum_ = r[1];
+      hk_ = 1.0;
     }
     gu0n_ = gu1n_;
     gu0_ = std::move(gu1_);
@@ -762,6 +783,9 @@ template <typename T> class FastRcg {
   double* hlt_ = nullptr;             // squared H_l weight by |m|^2
   size_t tab_bytes_ = 0;
   double uu_ = 0.0;                   // <u_hat, u_hat>_H_l of the state
+  // u_hat outside the kept cube is stored up to the factor hk_; hsum_ is the
+  // stored part's <., .>_H_l there (unnormalised)
+  double hk_ = 1.0, hsum_ = 0.0;
   // state
   RT u_, G_, s_, A_, Bq_, a_, Q0_;
   ST uh_, ph_, W_, Q0h_;

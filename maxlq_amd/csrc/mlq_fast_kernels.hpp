@@ -524,31 +524,59 @@ template <typename T> struct S_div_k {
 
 // u_hat <- a u_hat + b dB over the whole half spectrum (dB read only on the
 // kept cube when band-limited) and <u_hat, u_hat>_H_l of the result
+// u_hat := a u_hat + b D over the whole half spectrum, with the modes
+// outside the cube max|m_j| <= M multiplied by ah instead of a (their stored
+// values carry a pending factor); D is added inside the cube only when dlim
+// (a band-limited D), everywhere otherwise.  acc[0], acc[1]: <u_hat, u_hat>_H_l
+// inside and outside the cube (unnormalised).
 template <typename T> struct R_uh_update {
-  c2<T>* U; const c2<T>* D; long long ns; int n, px, M; bool dlim; double a, b;
+  c2<T>* U; const c2<T>* D; long long ns; int n, px, M; bool dlim; double a, ah, b;
   const double* hl;
-  __device__ void operator()(long long i, double (&acc)[1]) const {
+  __device__ void operator()(long long i, double (&acc)[2]) const {
     const RowIdx ri = row_split(i, px);   // px: row pitch
     const int kx = ri.kx;
     if (kx > (n >> 1)) return;             // row padding
     const int ky = row_ky(ri.row, n), kz = row_kz(ri.row, n);
     const int h = n >> 1;
     const int my = ky <= h ? ky : ky - n, mz = kz <= h ? kz : kz - n;
-    const bool in = !dlim || (kx <= M && my <= M && my >= -M && mz <= M && mz >= -M);
+    const bool cube = kx <= M && my <= M && my >= -M && mz <= M && mz >= -M;
+    const bool addD = D && (!dlim || cube);
+    const double am = cube ? a : ah;
     const double m = (kx == 0 || kx == h) ? 1.0 : 2.0;
     const double w = m * hl[kx * kx + my * my + mz * mz];
     double s = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const c2<T> u = U[c * ns + i];
-      double ur = a * (double)u.x, ui = a * (double)u.y;
-      if (in) {
+      double ur = am * (double)u.x, ui = am * (double)u.y;
+      if (addD) {
         const c2<T> d = D[c * ns + i];
         ur += b * (double)d.x;
         ui += b * (double)d.y;
       }
       const c2<T> r = cmk<T>((T)ur, (T)ui);
       U[c * ns + i] = r;
+      s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+    }
+    acc[cube ? 0 : 1] += w * s;
+  }
+};
+// the same update inside the cube only (D band-limited; the modes outside are
+// left with their pending factor): acc[0] = <u_hat, u_hat>_H_l inside
+template <typename T> struct R_uh_update_k {
+  c2<T>* U; const c2<T>* D; long long ns; int n, px, M; double a, b; const double* hl;
+  __device__ void operator()(long long i, double (&acc)[1]) const {
+    const KeptIdx k = kept_decode(i, n, px, M);
+    const long long o = k.off;
+    const double m = (k.kx == 0 || k.kx == (n >> 1)) ? 1.0 : 2.0;
+    const double w = m * hl[k.m2];
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> u = U[c * ns + o], d = D[c * ns + o];
+      const double ur = a * (double)u.x + b * (double)d.x, ui = a * (double)u.y + b * (double)d.y;
+      const c2<T> r = cmk<T>((T)ur, (T)ui);
+      U[c * ns + o] = r;
       s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
     }
     acc[0] += w * s;

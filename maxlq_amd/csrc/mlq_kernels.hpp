@@ -121,9 +121,10 @@ __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, 
   double2 f = make_double2(o.scale, 0.0);
   const int fl = o.flags;
   if (fl == 0) return f;
-  if (fl & SP_MASK) {
-    int mm = max(abs(mx), max(abs(my), abs(mz)));
-    if (mm > o.mcut) return make_double2(0.0, 0.0);
+  if (fl & (SP_MASK | SP_HSCALE)) {
+    const int mm = max(abs(mx), max(abs(my), abs(mz)));
+    if ((fl & SP_MASK) && mm > o.mcut) return make_double2(0.0, 0.0);
+    if ((fl & SP_HSCALE) && mm > o.mcut) f.x *= o.hs;
   }
   if ((fl & SP_ZEROK0) && mx == 0 && my == 0 && mz == 0) return make_double2(0.0, 0.0);
   const double K1 = TWO_PI * mx, K2 = TWO_PI * my, K3 = TWO_PI * mz;
@@ -147,7 +148,7 @@ __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, 
 // column's terms precomputed (cz: the column's part of the mask and |K|^2).
 struct ColFactor {
   int fl, mcut, mxy;   // mxy = max(|m_x|, |m_y|)
-  double s, kxy;       // scale, K_x^2 + K_y^2
+  double s, kxy, hs;   // scale, K_x^2 + K_y^2, SP_HSCALE factor
 };
 __device__ __forceinline__ bool col_factor_ok(const SpecOp& o) {
   return !(o.flags & SP_IK) || o.dim == 2;
@@ -156,13 +157,14 @@ __device__ __forceinline__ ColFactor col_factor_init(const SpecOp& o, double sca
                                                      int n) {
   const int h = n >> 1, my = ky <= h ? ky : ky - n;
   const double K1 = TWO_PI * kx, K2 = TWO_PI * my;
-  return ColFactor{o.flags, o.mcut, max(kx, abs(my)), scale, K1 * K1 + K2 * K2};
+  return ColFactor{o.flags, o.mcut, max(kx, abs(my)), scale, K1 * K1 + K2 * K2, o.hs};
 }
 __device__ __forceinline__ double2 col_factor(const ColFactor& c, int kz, int n) {
   const int h = n >> 1, mz = kz <= h ? kz : kz - n;
   const int fl = c.fl;
   double f = c.s;
   if ((fl & SP_MASK) && max(c.mxy, abs(mz)) > c.mcut) return make_double2(0.0, 0.0);
+  if ((fl & SP_HSCALE) && max(c.mxy, abs(mz)) > c.mcut) f *= c.hs;
   const double Kz = TWO_PI * mz, ksq = c.kxy + Kz * Kz;
   if ((fl & SP_ZEROK0) && ksq == 0.0) return make_double2(0.0, 0.0);
   if (fl & SP_NEGKSQ) f *= -ksq;
