@@ -374,7 +374,7 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
 // y and z passes of a forward transform whose x pass left nb spectra in the
 // scratch slots 0..nb-1
 template <typename T> template <int N>
-void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep) {
+void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep, bool kept_only) {
   constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
   const double sb = (double)nb * spec_bytes();
   SArgs<T> sa{};
@@ -387,6 +387,7 @@ void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep) {
   MLQ_HIP(hipGetLastError());
   prof_end(PC_YFWD, 2 * sb);
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
+  sa.kzero = kept_only ? 0 : 1;
   prof_begin(PC_ZFWD);
   launch_strided<T, N, -1, 2>(svar_[keep >= 0 ? 5 : 1], st_, sa, N * PP, nb);
   MLQ_HIP(hipGetLastError());
@@ -413,7 +414,7 @@ void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool
       hipLaunchKernelGGL((k_xfwd_p<T, N, false, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);
     MLQ_HIP(hipGetLastError());
     prof_end(PC_XFWD, P::NB * spec_bytes());
-    fwd_yz_n<N>(P::NB, dst, ops, keep);
+    fwd_yz_n<N>(P::NB, dst, ops, keep, true);
     nfwd_ += P::NB;
     done = true;
   }
@@ -558,7 +559,9 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
 
 // forward transforms of the P::NB fields a producer computes pointwise
 // (k_xfwd_p); false (nothing launched) when the scratch or the LDS cannot
-// hold the batch, the caller then materialises the fields
+// hold the batch, the caller then materialises the fields.  A band-limited
+// batch stores the kept cube of its spectra only: every reader of these
+// outputs (band-limited inverses, kept-cube spectral kernels) stays inside it.
 template <typename T> template <class P>
 bool Engine<T>::fwd_prod(const P& prod, C* const* dst, const SpecOp* ops) {
   if (!fused_ || P::NB > sb_) return false;

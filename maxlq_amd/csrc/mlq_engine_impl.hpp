@@ -139,7 +139,8 @@ template <typename T> class Engine final : public IEngine {
   struct InvSrc { const C* s[3]; double c[3]; int ns; };
   void inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep = -1);
   template <int N> void fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* ops);
-  template <int N> void fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep);
+  template <int N> void fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep,
+                                 bool kept_only = false);
   int fwd_keep(int nb, const SpecOp* ops) const;
   // forward transforms of the P::NB fields that the producer P computes per
   // point (no intermediate fields); false when it cannot run (the caller

@@ -95,6 +95,10 @@ template <typename T> struct SArgs {
   int yin[MAXB];
   int yout[MAXB];
   int yik[MAXB];
+  // forward z pass of a pruned launch: 1 stores the zeros of the masked
+  // spectrum outside the kept cube, 0 stores the kept cube only (for outputs
+  // whose every reader stays inside it)
+  int kzero = 1;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
   // unwritten (the pruned y pass reads kept columns only); the forward pass
   // stores the zeros of the masked spectrum.
   if (MODE != 0 && keep >= 0 && !__syncthreads_or(ld_ok)) {
-    if (MODE == 2 && valid) {
+    if (MODE == 2 && valid && a.kzero) {
       c2<T>* d = a.dst[plane] + col;
 #pragma unroll
       for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
@@ -496,15 +500,21 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
     const int bb = plane;
     const SpecOp op = a.op[bb];
     c2<T>* d = a.dst[bb] + col;
+    // kzero = 0 (a pruned launch whose readers stay inside the kept cube):
+    // nothing outside the cube is stored
+    const bool zout = a.kzero || keep < 0;
     if (!kept) {   // outside the kept cube: the masked spectrum is zero
+      if (zout) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+        for (int m = 0; m < E; ++m) d[(t + m * Tf) * estride] = cmk<T>((T)0, (T)0);
+      }
     } else if (col_factor_ok(op)) {
       // closed form along the column, in registers
       const ColFactor cf = col_factor_init(op, op.scale, kx, ky, N);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int n = t + m * Tf;
+        if (!zout && mabs(n, N) > keep) continue;
         const c2<double> p = cmul(ccvt<double>(v[m]), col_factor(cf, n, N));
         d[n * estride] = cmk<T>((T)p.x, (T)p.y);
       }
