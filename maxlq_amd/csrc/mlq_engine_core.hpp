@@ -39,7 +39,7 @@ template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
     sb_ = std::max(3, std::min(MAXB, (int)(0.06 * (double)tot / per)));
   }
   scratch_ = (C*)dmalloc(sizeof(C) * ns_ * sb_);
-  part_ = (double*)dmalloc(sizeof(double) * RED_BLOCKS * RED_MAXK);
+  part_ = (double*)dmalloc(sizeof(double) * std::max(RED_BLOCKS, RED_MAXBLK) * RED_MAXK);
   red_dev_ = (double*)dmalloc(sizeof(double) * 64);
   MLQ_HIP(hipHostMalloc((void**)&red_host_, sizeof(double) * 64, hipHostMallocDefault));
 }
@@ -239,8 +239,11 @@ template <typename T> template <class F> void Engine<T>::map(const F& f, long lo
   if (n % vec) throw Error(MLQ_EINVAL, "map length not a multiple of its vector width");
   n /= vec;
   if (n <= 0) return;
+  // one call per thread (the grid-stride loop only runs past 2^24
+  // workgroups): a capped grid that strides over the fields measured 25%
+  // below the streaming rate of a one-pass grid
   long long blocks = (n + MAP_THREADS - 1) / MAP_THREADS;
-  blocks = std::min<long long>(blocks, 256LL * 64);
+  blocks = std::min<long long>(blocks, MLQ_MAP_CAP);
   prof_begin(cls);
   hipLaunchKernelGGL((k_map<F>), dim3((unsigned)blocks), dim3(MAP_THREADS), 0, st_, f, n);
   MLQ_HIP(hipGetLastError());
@@ -253,9 +256,10 @@ void Engine<T>::reduce(const F& f, long long n, double* out) {
   if (n % vec) throw Error(MLQ_EINVAL, "reduction length not a multiple of its vector width");
   n /= vec;
   prof_begin(PC_REDUCE);
-  hipLaunchKernelGGL((k_reduce<K, F>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, f, n, part_);
+  const int nblk = (int)red_blocks(n);
+  hipLaunchKernelGGL((k_reduce<K, F>), dim3(nblk), dim3(RED_THREADS), 0, st_, f, n, part_);
   MLQ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_final, dim3(K), dim3(RED_THREADS), 0, st_, part_, RED_BLOCKS, K,
+  hipLaunchKernelGGL(k_reduce_final, dim3(K), dim3(RED_THREADS), 0, st_, part_, nblk, K,
                      red_dev_);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_REDUCE, 0);

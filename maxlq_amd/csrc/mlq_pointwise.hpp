@@ -5,6 +5,9 @@
 namespace mlq {
 
 constexpr int MAP_THREADS = 256;
+#ifndef MLQ_MAP_CAP
+#define MLQ_MAP_CAP ((1LL << 24) - 1)
+#endif
 
 template <class F> __global__ __launch_bounds__(MAP_THREADS) void k_map(F f, long long n) {
   for (long long i = (long long)blockIdx.x * MAP_THREADS + threadIdx.x; i < n;
@@ -14,7 +17,15 @@ template <class F> __global__ __launch_bounds__(MAP_THREADS) void k_map(F f, lon
 
 // ---- physical-space functors -------------------------------------------------
 template <typename T> struct F_fill { T* o; T v; __device__ void operator()(long long i) const { o[i] = v; } };
-template <typename T> struct F_copy { const T* a; T* o; __device__ void operator()(long long i) const { o[i] = a[i]; } };
+template <typename T> struct F_copy {   // 16-byte vectors
+  static constexpr int vec = V16<T>::w;
+  const T* a; T* o;
+  __device__ void operator()(long long i) const {
+    T v[vec];
+    ld16(a, i, v);
+    st16(o, i, v);
+  }
+};
 template <typename T> struct F_scale { T* o; T s; __device__ void operator()(long long i) const { o[i] *= s; } };
 // o = a*x + b*y
 template <typename T> struct F_axpby {
