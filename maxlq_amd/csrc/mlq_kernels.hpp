@@ -161,6 +161,12 @@ __device__ __forceinline__ void xcd_tile(bool on, unsigned& bx, unsigned& by) {
 
 // |m| of FFT index k on an n-point axis (Nyquist counted +n/2)
 __device__ __forceinline__ int mabs(int k, int n) { return k <= (n >> 1) ? k : n - k; }
+// FFT index n in [0, N) lies outside the band |m| <= keep, for 0 <= keep < N/2:
+// keep < n < N - keep, as one unsigned compare (a per-element mabs test costs
+// the strided passes registers)
+__device__ __forceinline__ bool out_band(int n, int keep, int N) {
+  return (unsigned)(n - keep - 1) < (unsigned)(N - 2 * keep - 1);
+}
 
 // Spectral factor of one half-spectrum mode (see SpecOp in mlq_common.hpp).
 __device__ __forceinline__ double2 spec_factor(const SpecOp& o, int kx, int ky, int kz, int n) {
@@ -442,7 +448,8 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
     }
     return;
   }
-  auto rowin = [&](int n) { return DIR < 0 || keep < 0 || mabs(n, N) <= keep; };
+  const bool band = keep >= 0 && 2 * keep < N;   // else no row is outside
+  auto rowin = [&](int n) { return DIR < 0 || !band || !out_band(n, keep, N); };
   // rows outside the kept cube: the load is redirected to row 0 of the same
   // column (always kept, in bounds) and its value replaced by zero, so the E
   // loads stay unconditional (no per-element branch)
@@ -469,18 +476,23 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
     for (int i = 1; i < ns; ++i) {
       const c2<T>* si = a.src[bb][i] + col;
       const double ci = a.coef[bb][i];
-      c2<T> ld[E];
-      if (ld_ok) {
+      // half the column at a time: loading all E elements of the source
+      // while v is live cost the kernel 16 registers (5 instead of 7 waves
+      // per SIMD at E = 8)
 #pragma unroll
-        for (int m = 0; m < E; ++m) ld[m] = ldrow(si, t + m * Tf);
-      } else {
+      for (int h = 0; h < 2; ++h) {
+        c2<T> ld[E / 2];
 #pragma unroll
-        for (int m = 0; m < E; ++m) ld[m] = cmk<T>((T)0, (T)0);
+        for (int mm = 0; mm < E / 2; ++mm)
+          ld[mm] = ld_ok ? ldrow(si, t + (h * E / 2 + mm) * Tf) : cmk<T>((T)0, (T)0);
+#pragma unroll
+        for (int mm = 0; mm < E / 2; ++mm) {
+          const int m = h * E / 2 + mm;
+          v[m] = cmk<T>((T)(c0 * (double)v[m].x + ci * (double)ld[mm].x),
+                        (T)(c0 * (double)v[m].y + ci * (double)ld[mm].y));
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-#pragma unroll
-      for (int m = 0; m < E; ++m)
-        v[m] = cmk<T>((T)(c0 * (double)v[m].x + ci * (double)ld[m].x),
-                      (T)(c0 * (double)v[m].y + ci * (double)ld[m].y));
       c0 = 1.0;
     }
     // Spectral factor (uniform branch: op and c0 are per plane).  Factors
@@ -561,7 +573,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int n = t + m * Tf;
-        if (!zout && mabs(n, N) > keep) continue;
+        if (!zout && band && out_band(n, keep, N)) continue;
         const c2<double> p = cmul(ccvt<double>(v[m]), col_factor(cf, n, N));
         gst<NTS>(d + n * estride, cmk<T>((T)p.x, (T)p.y));
       }
@@ -588,7 +600,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = t + m * Tf;
-      if (MODE == 0 && DIR < 0 && keep >= 0 && mabs(n, N) > keep) continue;
+      if (MODE == 0 && DIR < 0 && band && out_band(n, keep, N)) continue;
       gst<NTS>(d + n * estride, v[m]);
     }
   }
