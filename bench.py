@@ -102,6 +102,39 @@ def pmc_traffic(kernel, n, precision):
     return None
 
 
+def strided_tiles(n, wbytes):
+    """(elements per thread, columns per workgroup) of the strided-pass tile
+    shapes 0-3 (the library's tile table; shapes that do not fit fall back to
+    shape 0)."""
+    cmin = 16 if wbytes == 4 else 8
+    e0 = 16 if n >= 1024 else 8
+    c0 = max(cmin, 256 // (n // e0))
+    out = []
+    for v in range(4):
+        er = e0 if v == 0 else (8 if v == 3 else 16)
+        cr = c0 if v == 0 else (cmin if v == 1 else (2 * cmin if v == 2 else cmin // 2))
+        ok = v == 0 or (n >= 128 and er <= n and (n // er) * cr <= 1024 and
+                        n * cr * 2 * wbytes <= 160 * 1024)
+        out.append((er, cr) if ok else (e0, c0))
+    return out
+
+
+def column_copy_rates(g, n, wbytes, nfields=12, reps=5):
+    """GB/s of a 12-spectrum copy in the y and z access patterns of the
+    strided passes, per tile shape (the better of the two workgroup orders)."""
+    out = {}
+    for axis, name in ((1, "y"), (2, "z")):
+        rates = {}
+        for v, (e, c) in enumerate(strided_tiles(n, wbytes)):
+            best = 0.0
+            for order in (0, 4):
+                ms, by = g.time_column_copy(axis, v + order, nfields, reps)
+                best = max(best, by / (ms * 1e-3) / 1e9)
+            rates[f"E{e}xC{c}"] = round(best, 1)
+        out[name] = rates
+    return out
+
+
 def cpu_baseline(q, B, threads):
     """Bounded sample of the reference-structured FP64 CPU path (the oracle):
     one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
@@ -262,9 +295,11 @@ def main():
         passes, gbs, strict = fft_roofline(g)
         rb, _ = g.sizes()
         tc = g.time_copy(12, 5)
+        wb = 4 if args.fft_only == "mixed" else 8
         print(json.dumps({"fft_passes": passes, "fft_pass_GBs": gbs, "fft_strict_GBs": strict,
                           "copy": {"fields": 12, "ms": tc, "bytes": 2 * 12 * rb,
-                                   "GBs": 2 * 12 * rb / (tc * 1e-3) / 1e9}}), flush=True)
+                                   "GBs": 2 * 12 * rb / (tc * 1e-3) / 1e9},
+                          "column_copy_12_spectra_GBs": column_copy_rates(g, n, wb)}), flush=True)
         return
 
     log(f"mixed precision, batched line search: {n}^3 q={q} B={B}")
@@ -279,6 +314,8 @@ def main():
     rbm, _ = gm.sizes()
     tcopy = gm.time_copy(12, 5)
     copy_gbs = 2 * 12 * rbm / (tcopy * 1e-3) / 1e9
+    colcopy = column_copy_rates(gm, n, 4)
+    log("column copies (GB/s): " + json.dumps(colcopy))
 
     t_fp64 = None
     t_ref = None
@@ -304,6 +341,12 @@ def main():
             "unit": "GB/s", "frac": round(dp["GBs"] / PEAK_HBM_GBS, 4),
             "traffic": pmc_traffic(f"fft_{dom}", n, "mixed"),
             "kernel": f"fft_{dom}", "algorithmic_bytes": dp["bytes"], "launch_ms": round(dp["ms"], 4)}
+    # the ceiling the pass's memory access pattern sets: the best copy rate of
+    # that pattern with no arithmetic (any tile; x passes: the plain copy)
+    axis = {"y": "y", "z": "z"}.get(dom[0])
+    ceil = max(colcopy[axis].values()) if axis else copy_gbs
+    roof["pattern_ceiling_GBs"] = round(ceil, 1)
+    roof["frac_of_pattern_ceiling"] = round(dp["GBs"] / ceil, 4)
 
     cpu = None
     if not args.skip_cpu:
@@ -340,6 +383,7 @@ def main():
                    "fft_pass_GBs": round(fft_gbs, 1), "fft_strict_GBs": round(fft_strict, 1),
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()},
                    "copy_12_fields_GBs": round(copy_gbs, 1),
+                   "column_copy_12_spectra_GBs": colcopy,
                    "transforms_per_iter_mixed": {"forward": gm.tr_per_iter[0],
                                                  "inverse": gm.tr_per_iter[1],
                                                  "reference_c2c": 548 + 94 * nJm}},

@@ -218,6 +218,15 @@ int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms);
 /* time `reps` launches of a plain device copy of `nfields` real fields (the
  * calibration kernel for HBM byte counters); average ms per launch in *t_ms */
 int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms);
+/* time `reps` launches of a copy of `nfields` half spectra with the memory
+ * access pattern of a strided transform pass and none of its arithmetic:
+ * axis 1 the y pattern (columns inside one z plane), 2 the z pattern (columns
+ * across the planes); `tile` = tile shape + 4*order as in mlq_autotune.  The
+ * rate is the ceiling that access pattern sets for the pass.  *t_ms: ms per
+ * launch; *bytes: algorithmic bytes per launch (read + write, padding
+ * excluded).  No counterpart in the reference (measurement support). */
+int mlq_time_column_copy(mlq_ctx* ctx, int axis, int tile, int nfields, int reps, double* t_ms,
+                         double* bytes);
 /* tile autotuner of the strided transform passes: times each of 8 tile
  * candidates (4 shapes of elements per thread x columns per workgroup, each
  * in dispatch order and in XCD-contiguous order: candidate = shape + 4*order)

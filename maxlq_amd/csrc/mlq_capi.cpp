@@ -379,6 +379,10 @@ int mlq_autotune(mlq_ctx* ctx, int batch, double* times, int* choice) {
 int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms) {
   return guarded(ctx, [&] { ctx->eng->time_copy(nfields, reps, t_ms); });
 }
+int mlq_time_column_copy(mlq_ctx* ctx, int axis, int tile, int nfields, int reps, double* t_ms,
+                         double* bytes) {
+  return guarded(ctx, [&] { ctx->eng->time_column_copy(axis, tile, nfields, reps, t_ms, bytes); });
+}
 int mlq_sizes(const mlq_ctx* ctx, double* rb, double* sb) {
   if (!ctx || !rb || !sb) return MLQ_EINVAL;
   *rb = ctx->eng->real_bytes();

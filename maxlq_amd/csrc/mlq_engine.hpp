@@ -92,6 +92,8 @@ class IEngine {
   // measurement
   virtual void time_fft(int batch, int reps, double* t_ms) = 0;
   virtual void time_copy(int nfields, int reps, double* t_ms) = 0;
+  virtual void time_column_copy(int axis, int var, int nfields, int reps, double* t_ms,
+                                double* bytes) = 0;
   // time the tile candidates of the strided passes and keep the fastest;
   // times[pass*8+cand] in ms per launch, choice[pass]
   virtual void autotune(int batch, double* times, int* choice) = 0;

@@ -710,6 +710,63 @@ template <typename T> void Engine<T>::autotune(int batch, double* times, int* ch
   }
 }
 
+// column copies in the y or z pattern of tile shape `var` (bits 0-1) and
+// order (bit 2), nfields spectra: ms per launch and algorithmic bytes
+template <typename T> template <int N>
+double Engine<T>::time_colcopy_n(int axis, int var, const C* src, C* dst, int nfields, int reps) {
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  const int ncols = axis == 1 ? PX : N * PP;
+  const int planes = axis == 1 ? N * nfields : nfields;
+  const int remap = (var >> 2) & 1;
+  auto launch = [&](auto s) {
+    using S = decltype(s);
+    const dim3 grid((ncols + S::C - 1) / S::C, planes), block(S::C * (N / S::E));
+    if (axis == 1)
+      hipLaunchKernelGGL((k_colcopy<T, N, S::E, S::C, 1>), grid, block, 0, st_, src, dst, ncols, remap);
+    else
+      hipLaunchKernelGGL((k_colcopy<T, N, S::E, S::C, 2>), grid, block, 0, st_, src, dst, ncols, remap);
+  };
+  auto run = [&]() {
+    switch (var & 3) {
+      case 1: launch(SVar<T, N, 1>{}); break;
+      case 2: launch(SVar<T, N, 2>{}); break;
+      case 3: launch(SVar<T, N, 3>{}); break;
+      default: launch(SVar<T, N, 0>{}); break;
+    }
+    MLQ_HIP(hipGetLastError());
+  };
+  hipEvent_t a, z;
+  MLQ_HIP(hipEventCreate(&a));
+  MLQ_HIP(hipEventCreate(&z));
+  run();  // warm-up
+  MLQ_HIP(hipEventRecord(a, st_));
+  for (int i = 0; i < reps; ++i) run();
+  MLQ_HIP(hipEventRecord(z, st_));
+  MLQ_HIP(hipEventSynchronize(z));
+  float ms = 0.f;
+  MLQ_HIP(hipEventElapsedTime(&ms, a, z));
+  MLQ_HIP(hipEventDestroy(a));
+  MLQ_HIP(hipEventDestroy(z));
+  return (double)ms / reps;
+}
+
+template <typename T>
+void Engine<T>::time_column_copy(int axis, int var, int nfields, int reps, double* t_ms,
+                                 double* bytes) {
+  if ((axis != 1 && axis != 2) || var < 0 || var > 7 || nfields < 1 || reps < 1)
+    throw Error(MLQ_EINVAL, "time_column_copy: axis 1 or 2, var 0..7, nfields and reps >= 1");
+  const size_t sz = sizeof(C) * (size_t)ns_ * (size_t)nfields;
+  C* a = (C*)dmalloc(sz);
+  C* b = (C*)dmalloc(sz);
+  MLQ_HIP(hipMemsetAsync(a, 0, sz, st_));
+  double t = 0.0;
+  MLQ_DISPATCH_N(t = time_colcopy_n, axis, var, a, b, nfields, reps);
+  dfree(a, sz);
+  dfree(b, sz);
+  *t_ms = t;
+  *bytes = 2.0 * nfields * 2.0 * sizeof(T) * (double)(n_ / 2 + 1) * (double)n_ * (double)n_;
+}
+
 template <typename T> void Engine<T>::time_copy(int nfields, int reps, double* t_ms) {
   nfields = std::max(1, nfields);
   RT a = rtmp(nfields), b = rtmp(nfields);

@@ -96,6 +96,8 @@ template <typename T> class Engine final : public IEngine {
   void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) override;
   void time_fft(int batch, int reps, double* t_ms) override;
   void time_copy(int nfields, int reps, double* t_ms) override;
+  void time_column_copy(int axis, int var, int nfields, int reps, double* t_ms,
+                        double* bytes) override;
   void autotune(int batch, double* times, int* choice) override;
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
@@ -196,6 +198,8 @@ template <typename T> class Engine final : public IEngine {
   // tile variant of the strided passes {y fwd, z fwd, z inv, y inv} (autotuned)
   int svar_[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // {y fwd, z fwd, z inv, y inv} x {full, band-limited}
   template <int N> double time_pass_n(int pass, int var, int batch, int reps, int keep = -1);
+  template <int N>
+  double time_colcopy_n(int axis, int var, const C* src, C* dst, int nfields, int reps);
   long long nr_, ns_;
   mlq_params p_;
   bool use_e_u_ = false;

@@ -606,6 +606,32 @@ __global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
   }
 }
 
+// Column copy with the memory access pattern of a strided pass and none of
+// its arithmetic: C adjacent columns per workgroup, E elements per thread at
+// the pass's element stride (AXIS 1: along y inside one z plane; AXIS 2:
+// along z across the planes), the pass's load and store hints, padding
+// columns skipped.  Its rate is the ceiling that pattern sets for the pass.
+template <typename T, int N, int E, int C, int AXIS>
+__global__ __launch_bounds__(C*(N / E)) void k_colcopy(const c2<T>* __restrict__ src,
+                                                       c2<T>* __restrict__ dst, int ncols,
+                                                       int remap) {
+  constexpr int Tf = N / E, PP = spec_pitch<T>(N);
+  constexpr int estride = AXIS == 1 ? PP : N * PP;
+  constexpr long long pstride = AXIS == 1 ? (long long)N * PP : (long long)N * N * PP;
+  constexpr int CLS = AXIS == 1 ? PC_YF : PC_ZF;
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  unsigned bx, by;
+  xcd_tile(remap != 0, bx, by);
+  const int col = (int)bx * C + c;
+  if (col >= ncols || col % PP > N / 2) return;   // no barrier follows
+  const long long off = (long long)by * pstride + col;
+  c2<T> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = gld<nt_ld<CLS>()>(src + off + (long long)(t + m * Tf) * estride);
+#pragma unroll
+  for (int m = 0; m < E; ++m) gst<nt_st<CLS>()>(dst + off + (long long)(t + m * Tf) * estride, v[m]);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 reductions: per-block partial sums (wavefront shuffles + LDS), then a
 // single-block final sum; fixed block count, so results are deterministic.

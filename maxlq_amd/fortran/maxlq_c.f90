@@ -332,6 +332,13 @@ module maxlq_c
          integer(c_int), value :: nfields, reps
          real(c_double), intent(out) :: t_ms
       end function
+      integer(c_int) function mlq_time_column_copy(ctx, axis, tile, nfields, reps, t_ms, bytes) &
+            bind(C, name="mlq_time_column_copy")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: axis, tile, nfields, reps
+         real(c_double), intent(out) :: t_ms, bytes
+      end function
       integer(c_int) function mlq_autotune(ctx, batch, times, choice) bind(C, name="mlq_autotune")
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx

@@ -333,6 +333,14 @@ class MaxLq:
         self._chk(self.lib.mlq_time_copy(self.ctx, nfields, reps, C.byref(r)))
         return r.value
 
+    def time_column_copy(self, axis, tile=0, nfields=3, reps=5):
+        """Copy of nfields half spectra in the access pattern of the y (axis 1)
+        or z (axis 2) transform pass with tile `tile`: (ms per launch, bytes)."""
+        r, b = C.c_double(), C.c_double()
+        self._chk(self.lib.mlq_time_column_copy(self.ctx, axis, tile, nfields, reps,
+                                                C.byref(r), C.byref(b)))
+        return r.value, b.value
+
     def sizes(self):
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.mlq_sizes(self.ctx, C.byref(a), C.byref(b)))

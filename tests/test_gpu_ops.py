@@ -225,3 +225,18 @@ def test_transform_counts():
     g.fftbwdr(F)
     f1, i1 = g.transform_counts()
     assert (f1 - f0, i1 - i0) == (1, 1)
+
+
+@pytest.mark.gpu
+def test_time_column_copy():
+    """mlq_time_column_copy: both access patterns, every tile; algorithmic
+    bytes = read + write of nfields half spectra (padding excluded)."""
+    n = 128
+    g = _ctx(n)
+    for axis in (1, 2):
+        for tile in range(8):
+            ms, by = g.time_column_copy(axis, tile, nfields=2, reps=2)
+            assert ms > 0.0
+            assert by == 2 * 2 * 2 * 8 * (n // 2 + 1) * n * n
+    with pytest.raises(RuntimeError):
+        g.time_column_copy(3, 0)
