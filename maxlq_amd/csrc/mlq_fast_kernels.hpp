@@ -22,9 +22,15 @@ __device__ __forceinline__ void ls_products_at(const T* u, const T* d, const T* 
                                                long long nr, T gam, long long i, T (&o)[12]) {
   T uu[3], dd[3], g[9], h[9];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { uu[c] = u[c * nr + i]; dd[c] = d[c * nr + i] + gam * uu[c]; }
+  for (int c = 0; c < 3; ++c) {
+    uu[c] = gld<NT_PROD>(u + c * nr + i);
+    dd[c] = gld<NT_PROD>(d + c * nr + i) + gam * uu[c];
+  }
 #pragma unroll
-  for (int a = 0; a < 9; ++a) { g[a] = G[a * nr + i]; h[a] = Gd[a * nr + i] + gam * g[a]; }
+  for (int a = 0; a < 9; ++a) {
+    g[a] = gld<NT_PROD>(G + a * nr + i);
+    h[a] = gld<NT_PROD>(Gd + a * nr + i) + gam * g[a];
+  }
   o[0] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
   o[1] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
 #pragma unroll
@@ -220,21 +226,22 @@ template <typename T> struct F_grad_terms {
   __device__ void at(long long i, T (&y)[3]) const {
     T g[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) g[k] = G[k * nr + i];
+    for (int k = 0; k < 9; ++k) g[k] = gld<NT_PROD>(G + k * nr + i);
     if (Gd0) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        g[k] = ga * g[k] + gb * Gd0[k * nr + i];
-        Gout[k * nr + i] = g[k];
+        g[k] = ga * g[k] + gb * gld<NT_PROD>(Gd0 + k * nr + i);
+        gst<NT_PROD>(Gout + k * nr + i, g[k]);
       }
     }
-    const T a2 = aux2[i], Ai = A ? A[i] : (T)1;
-    const T f0 = gphi[i], f1 = gphi[nr + i], f2 = gphi[2 * nr + i];
+    const T a2 = gld<NT_PROD>(aux2 + i), Ai = A ? gld<NT_PROD>(A + i) : (T)1;
+    const T f0 = gld<NT_PROD>(gphi + i), f1 = gld<NT_PROD>(gphi + nr + i),
+            f2 = gld<NT_PROD>(gphi + 2 * nr + i);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const T t5 = g[0 * 3 + c] * f0 + g[1 * 3 + c] * f1 + g[2 * 3 + c] * f2;
-      y[c] = q2 * a2 * u[c * nr + i] - pc * Ai * gp[c * nr + i] + (T)2 * pc * t5 +
-             vnu * Ai * lap[c * nr + i];
+      y[c] = q2 * a2 * gld<NT_PROD>(u + c * nr + i) - pc * Ai * gld<NT_PROD>(gp + c * nr + i) +
+             (T)2 * pc * t5 + vnu * Ai * gld<NT_PROD>(lap + c * nr + i);
     }
   }
   __device__ void operator()(long long i) const {

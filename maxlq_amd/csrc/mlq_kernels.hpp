@@ -37,6 +37,11 @@ template <int CLS> constexpr bool nt_st() { return (MLQ_NT_ST >> CLS) & 1; }
 template <int MODE, int DIR> constexpr int strided_class() {
   return MODE == 1 ? PC_ZI : MODE == 2 ? PC_ZF : DIR > 0 ? PC_YI : PC_YF;
 }
+// the pointwise producers of the fused forward x passes read each input once
+#ifndef MLQ_NT_PROD
+#define MLQ_NT_PROD 1
+#endif
+constexpr bool NT_PROD = MLQ_NT_PROD != 0;
 template <typename V> struct NatV { using type = V; };
 template <> struct NatV<float2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct NatV<double2> { typedef double type __attribute__((ext_vector_type(2))); };
@@ -408,8 +413,16 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 // spectral factor (prologue), writing scratch.  MODE 2: forward z pass
 // reading scratch, writing factor * result to dst (epilogue).
 // ---------------------------------------------------------------------------
+// FP32 inverse z pass at E = 8: compiled for 8 waves per SIMD (64 VGPRs, a
+// 12-byte spill) instead of 72 VGPRs, which at 16 columns (1024 threads) let
+// only one workgroup per CU run; measured 0.4 ms faster per mixed iteration
+#ifndef MLQ_ZI_WAVES
+#define MLQ_ZI_WAVES 8
+#endif
 template <typename T, int N, int E, int C, int DIR, int MODE>
-__global__ __launch_bounds__(C*(N / E)) void k_strided(SArgs<T> a, int ncols) {
+__global__ __launch_bounds__(C*(N / E))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == 1 && E == 8 ? MLQ_ZI_WAVES : 1)))
+void k_strided(SArgs<T> a, int ncols) {
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   constexpr bool NTL = nt_ld<strided_class<MODE, DIR>()>(), NTS = nt_st<strided_class<MODE, DIR>()>();
   // element and plane strides are fixed by the pass (y: rows of one z plane;
@@ -680,16 +693,28 @@ template <int K> __device__ __forceinline__ void block_reduce_store(double (&acc
 template <typename T> struct V16;
 template <> struct V16<float> { using type = float4; static constexpr int w = 4; };
 template <> struct V16<double> { using type = double2; static constexpr int w = 2; };
+// (streaming: the nontemporal hint unless MLQ_NT_VEC=0)
+#ifndef MLQ_NT_VEC
+#define MLQ_NT_VEC 1
+#endif
+template <typename T> struct V16N;
+template <> struct V16N<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct V16N<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <typename T>
 __device__ __forceinline__ void ld16(const T* p, long long iv, T (&o)[V16<T>::w]) {
-  const typename V16<T>::type v = reinterpret_cast<const typename V16<T>::type*>(p)[iv];
+  using NV = typename V16N<T>::type;
+  const NV* q = reinterpret_cast<const NV*>(p) + iv;
+  const NV v = MLQ_NT_VEC ? __builtin_nontemporal_load(q) : *q;
   __builtin_memcpy(o, &v, sizeof(v));
 }
 template <typename T>
 __device__ __forceinline__ void st16(T* p, long long iv, const T (&o)[V16<T>::w]) {
-  typename V16<T>::type v;
+  using NV = typename V16N<T>::type;
+  NV v;
   __builtin_memcpy(&v, o, sizeof(v));
-  reinterpret_cast<typename V16<T>::type*>(p)[iv] = v;
+  NV* q = reinterpret_cast<NV*>(p) + iv;
+  if (MLQ_NT_VEC) __builtin_nontemporal_store(v, q);
+  else *q = v;
 }
 // points per functor call of a map or reduction (`static constexpr int vec`,
 // default 1): the functor's index then counts vectors of that many
