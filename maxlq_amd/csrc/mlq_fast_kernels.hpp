@@ -182,12 +182,12 @@ template <typename T> struct P_step {
   static constexpr int NB = 3;
   const T* u; const T* d; const T* A; long long nr; T a, b, sc; T* un;
   __device__ void operator()(long long i, T (&o)[3]) const {
-    const T w = A[i];
+    const T w = gld<NT_PROD>(A + i);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T phi = a * u[c * nr + i] + b * d[c * nr + i];
+      const T phi = a * gld<NT_PROD>(u + c * nr + i) + b * gld<NT_PROD>(d + c * nr + i);
       o[c] = w * phi;
-      un[c * nr + i] = sc * phi;
+      gst<NT_PROD>(un + c * nr + i, (T)(sc * phi));
     }
   }
 };
@@ -416,7 +416,10 @@ template <typename T> struct S_dir_prep_k {
     const long long o = k.off;
     if (k.m2 == 0) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { V[c * ns + o] = cmk<T>(0, 0); nH[c * ns + o] = cmk<T>(0, 0); }
+      for (int c = 0; c < 3; ++c) {
+        gst<NT_PROD>(V + c * ns + o, cmk<T>(0, 0));
+        gst<NT_PROD>(nH + c * ns + o, cmk<T>(0, 0));
+      }
       return;
     }
     const double e[3] = {TWO_PI * k.kx, TWO_PI * k.my, TWO_PI * k.mz};
@@ -425,7 +428,7 @@ template <typename T> struct S_dir_prep_k {
     double vr[3], vi[3], wr[3], wi[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const c2<T> v = V[c * ns + o], w = W[c * ns + o];
+      const c2<T> v = gld<NT_PROD>(V + c * ns + o), w = gld<NT_PROD>(W + c * ns + o);
       vr[c] = h * ((double)v.x + f * (double)w.x);
       vi[c] = h * ((double)v.y + f * (double)w.y);
       wr[c] = h * (sW * (double)w.x);
@@ -437,8 +440,8 @@ template <typename T> struct S_dir_prep_k {
     const double dwi = (e[0] * wi[0] + e[1] * wi[1] + e[2] * wi[2]) / ksq;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      V[c * ns + o] = cmk<T>((T)(vr[c] - e[c] * dvr), (T)(vi[c] - e[c] * dvi));
-      nH[c * ns + o] = cmk<T>((T)(wr[c] - e[c] * dwr), (T)(wi[c] - e[c] * dwi));
+      gst<NT_PROD>(V + c * ns + o, cmk<T>((T)(vr[c] - e[c] * dvr), (T)(vi[c] - e[c] * dvi)));
+      gst<NT_PROD>(nH + c * ns + o, cmk<T>((T)(wr[c] - e[c] * dwr), (T)(wi[c] - e[c] * dwi)));
     }
   }
 };
@@ -459,7 +462,8 @@ template <typename T> struct R_dir_k {
     double dvr = 0, dvi = 0, dnr = 0, dni = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const c2<T> v = V[c * ns + o], a = N[c * ns + o], u = U[c * ns + o];
+      const c2<T> v = gld<NT_PROD>(V + c * ns + o), a = gld<NT_PROD>(N + c * ns + o),
+                  u = gld<NT_PROD>(U + c * ns + o);
       const double vx = v.x, vy = v.y, ax = a.x, ay = a.y, ux = u.x, uy = u.y;
       vv += vx * vx + vy * vy;
       vn += vx * ax + vy * ay;
@@ -467,7 +471,7 @@ template <typename T> struct R_dir_k {
       vu += vx * ux + vy * uy;
       nu += ax * ux + ay * uy;
       if (G0) {
-        const c2<T> g = G0[c * ns + o];
+        const c2<T> g = gld<NT_PROD>(G0 + c * ns + o);
         vg += vx * (double)g.x + vy * (double)g.y;
         ng += ax * (double)g.x + ay * (double)g.y;
       }
@@ -491,15 +495,15 @@ template <typename T> struct S_dir_update_k {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long k = c * ns + o;
-      const c2<T> v = V[k], a = N[k];
+      const c2<T> v = gld<NT_PROD>(V + k), a = gld<NT_PROD>(N + k);
       const double gr = cg * ((double)v.x - alpha * (double)a.x);
       const double gi = cg * ((double)v.y - alpha * (double)a.y);
-      gu1[k] = cmk<T>((T)gr, (T)gi);
+      gst<NT_PROD>(gu1 + k, cmk<T>((T)gr, (T)gi));
       if (D0) {
-        const c2<T> q = D0[k];
-        d[k] = cmk<T>((T)(gr + beta * (double)q.x), (T)(gi + beta * (double)q.y));
+        const c2<T> q = gld<NT_PROD>(D0 + k);
+        gst<NT_PROD>(d + k, cmk<T>((T)(gr + beta * (double)q.x), (T)(gi + beta * (double)q.y)));
       } else {
-        d[k] = cmk<T>((T)gr, (T)gi);
+        gst<NT_PROD>(d + k, cmk<T>((T)gr, (T)gi));
       }
     }
   }
@@ -510,8 +514,9 @@ template <typename T> struct S_quad_k {
   const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; int n, px, M; double t, sc;
   __device__ void operator()(long long i) const {
     const long long k = kept_decode(i, n, px, M).off;
-    const c2<T> a = x0[k], b = x1[k], c = x2[k];
-    o[k] = cmk<T>((T)(sc * (a.x + t * (b.x + t * c.x))), (T)(sc * (a.y + t * (b.y + t * c.y))));
+    const c2<T> a = gld<NT_PROD>(x0 + k), b = gld<NT_PROD>(x1 + k), c = gld<NT_PROD>(x2 + k);
+    gst<NT_PROD>(o + k, cmk<T>((T)(sc * (a.x + t * (b.x + t * c.x))),
+                               (T)(sc * (a.y + t * (b.y + t * c.y)))));
   }
 };
 
@@ -580,10 +585,10 @@ template <typename T> struct R_uh_update_k {
     double s = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const c2<T> u = U[c * ns + o], d = D[c * ns + o];
+      const c2<T> u = gld<NT_PROD>(U + c * ns + o), d = gld<NT_PROD>(D + c * ns + o);
       const double ur = a * (double)u.x + b * (double)d.x, ui = a * (double)u.y + b * (double)d.y;
       const c2<T> r = cmk<T>((T)ur, (T)ui);
-      U[c * ns + o] = r;
+      gst<NT_PROD>(U + c * ns + o, r);
       s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
     }
     acc[0] += w * s;
@@ -762,8 +767,8 @@ template <typename T> struct R_kept_dot3 {
     const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const c2<T> w = W[c * ns + off], a = x[c * ns + off], b = y[c * ns + off],
-                  zz = z[c * ns + off];
+      const c2<T> w = gld<NT_PROD>(W + c * ns + off), a = gld<NT_PROD>(x + c * ns + off),
+                  b = gld<NT_PROD>(y + c * ns + off), zz = gld<NT_PROD>(z + c * ns + off);
       acc[0] += m * ((double)w.x * a.x + (double)w.y * a.y);
       acc[1] += m * ((double)w.x * b.x + (double)w.y * b.y);
       acc[2] += m * ((double)w.x * zz.x + (double)w.y * zz.y);

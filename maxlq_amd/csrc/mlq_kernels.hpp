@@ -37,7 +37,8 @@ template <int CLS> constexpr bool nt_st() { return (MLQ_NT_ST >> CLS) & 1; }
 template <int MODE, int DIR> constexpr int strided_class() {
   return MODE == 1 ? PC_ZI : MODE == 2 ? PC_ZF : DIR > 0 ? PC_YI : PC_YF;
 }
-// the pointwise producers of the fused forward x passes read each input once
+// the pointwise producers of the fused forward x passes and the kept-cube
+// spectral kernels read (and write) each value once per launch
 #ifndef MLQ_NT_PROD
 #define MLQ_NT_PROD 1
 #endif

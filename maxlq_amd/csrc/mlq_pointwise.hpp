@@ -123,7 +123,9 @@ template <typename T> struct F_udotforce {
   __device__ T at(long long i) const {
     T s = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s += v[c * nr + i] * (vnu * L[c * nr + i] - pc * P[c * nr + i]);
+    for (int c = 0; c < 3; ++c)
+      s += gld<NT_PROD>(v + c * nr + i) *
+           (vnu * gld<NT_PROD>(L + c * nr + i) - pc * gld<NT_PROD>(P + c * nr + i));
     return s;
   }
   __device__ void operator()(long long i) const { o[i] = at(i); }
