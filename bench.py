@@ -322,7 +322,8 @@ def main():
     dR = None
     if not args.skip_fp64:
         log("FP64, reference-structured iteration (every operator recomputed from the field)")
-        gr, hr, _, t_ref, _ = run_rcg(n, q, B, "fp64", 0, 0, 1, flags=capi_flag_reference())
+        # one warm-up iteration: the first one allocates its temporaries
+        gr, hr, _, t_ref, _ = run_rcg(n, q, B, "fp64", 0, 1, 1, flags=capi_flag_reference())
         gr.close()
         log("plain FP64 path")
         g64, h64, res64, t_fp64, _ = run_rcg(n, q, B, "fp64", 0, args.warmup, args.steps)
