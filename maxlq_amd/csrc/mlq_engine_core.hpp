@@ -334,9 +334,14 @@ void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, int planes)
   }
 }
 
+// threads per workgroup of the x passes (row pairs of Tf threads each):
+// 256 measured faster than 128 and 512 at 512^3
+#ifndef MLQ_XPASS_THREADS
+#define MLQ_XPASS_THREADS 256
+#endif
 template <int N> constexpr int xpass_NF() {
   constexpr int Tf = FftShape<N>::Tf;
-  constexpr int nf = 256 / Tf > 0 ? 256 / Tf : 1;
+  constexpr int nf = MLQ_XPASS_THREADS / Tf > 0 ? MLQ_XPASS_THREADS / Tf : 1;
   return nf < N * N / 2 ? nf : N * N / 2;
 }
 
