@@ -138,7 +138,7 @@ def column_copy_rates(g, n, wbytes, nfields=12, reps=5):
 def cpu_baseline(q, B, threads):
     """Bounded sample of the reference-structured FP64 CPU path (the oracle):
     one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
-    iteration with the reference's transform counts (SURVEY: 548 C2C
+    iteration with the reference's transform counts at q = 4 (548 C2C
     transforms per iteration outside the line search, 94 per J evaluation)."""
     from oracle import lq_oracle as L
     from oracle.seeded import seeded_field
@@ -282,7 +282,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(run it with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     n, q, B = args.n, args.q, args.B
     if args.case != "iteration":

@@ -18,7 +18,7 @@ SOURCES = ["mlq_engine_fp64.hip", "mlq_engine_fp32.hip", "mlq_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MLQ_OFFLOAD_ARCH", "gfx950")
-# packed FP32 instructions are disabled for every device pass (see DESIGN.md)
+# packed FP32 instructions are disabled for every device pass
 DEVFLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wno-unused-result", "-Wno-unused-value",
           "-I", os.path.join(HERE, "..", "include")]

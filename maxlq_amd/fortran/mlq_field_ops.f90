@@ -1,6 +1,6 @@
 ! The function_ops routines of the RCG hot path on the GPU, with the
 ! reference's dummies (function_ops.f90); function_ops.f90 forwards to these
-! (INTEGRATION.md, step 3).  Every call mirrors the current module globals
+! from its own routines of the same names.  Every call mirrors the current module globals
 ! into the device context first, so q- and B-continuation keep working.
 module mlq_field_ops
    use, intrinsic :: iso_c_binding
