@@ -105,6 +105,18 @@ int mlq_calc0thFourierModes(mlq_ctx* ctx, const double* vec, double out[3]); /* 
 int mlq_calc_uk(mlq_ctx* ctx, const double* u, double k, double* uk);     /* :2324-2343 */
 int mlq_calc_gk_order2(mlq_ctx* ctx, const double* g, double k, double* gk); /* :2281-2319 */
 int mlq_calc_global_Lq_norm(mlq_ctx* ctx, const double* u, double* norm); /* :1675-1692 */
+/* calculate_spectral_data(u, spectral_data), function_ops.f90:2614-2677:
+ * the energy spectrum in spherical shells.  spectral_data is the reference's
+ * (1:n1, 1:2) array in Fortran order: spectral_data[i] = 2 pi (i+1) (the
+ * shell's upper bound), spectral_data[n1+i] = sum of |fftfwd(u_c)|^2 over the
+ * three components and the modes with 2 pi i < |K| <= 2 pi (i+1); with
+ * normalize (normalizeSpectrumByL2Norm, global_variables.f90:76, default
+ * true) column 2 is scaled by ||u||_2^2 / (2 pi sum(column 2)) (:2667-2671). */
+int mlq_calculate_spectral_data(mlq_ctx* ctx, const double* u, int normalize,
+                                double* spectral_data);
+/* the same on a device-resident vector field (no counterpart in the
+ * reference: its fields are host arrays) */
+int mlq_spectral_data_field(mlq_ctx* ctx, mlq_field u, int normalize, double* spectral_data);
 int mlq_rescaleLqNorm(mlq_ctx* ctx, double* u, double B);                 /* :1699-1709 */
 int mlq_calcConstraintDerivativeL2(mlq_ctx* ctx, const double* u, double* out); /* :984-1006 */
 int mlq_vectorTransport(mlq_ctx* ctx, const double* u, double q, double B,

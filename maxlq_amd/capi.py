@@ -27,6 +27,7 @@ EXPORTS = [
     "mlq_fftfwdV", "mlq_fftbwdV", "mlq_gradient", "mlq_divergence", "mlq_laplacian",
     "mlq_solve_poisson", "mlq_dealias_scalar", "mlq_divAvg_free", "mlq_HilbertGradient",
     "mlq_calc0thFourierModes", "mlq_calc_uk", "mlq_calc_gk_order2", "mlq_calc_global_Lq_norm",
+    "mlq_calculate_spectral_data", "mlq_spectral_data_field",
     "mlq_rescaleLqNorm", "mlq_calcConstraintDerivativeL2", "mlq_vectorTransport",
     "mlq_inner_product", "mlq_global_inner_product", "mlq_global_summed_field_inner_product",
     "mlq_calc_local_dLqdt_inclParts", "mlq_calc_global_dLqdt_inclParts", "mlq_GradL2ForLq",
@@ -98,7 +99,9 @@ def load(path: str = LIB_PATH):
         "mlq_dealias_scalar": (ci, [vp, dp, cd]), "mlq_divAvg_free": (ci, [vp, dp]),
         "mlq_HilbertGradient": (ci, [vp, dp, cd]), "mlq_calc0thFourierModes": (ci, [vp, dp, dp]),
         "mlq_calc_uk": (ci, [vp, dp, cd, dp]), "mlq_calc_gk_order2": (ci, [vp, dp, cd, dp]),
-        "mlq_calc_global_Lq_norm": (ci, [vp, dp, dp]), "mlq_rescaleLqNorm": (ci, [vp, dp, cd]),
+        "mlq_calc_global_Lq_norm": (ci, [vp, dp, dp]),
+        "mlq_calculate_spectral_data": (ci, [vp, dp, ci, dp]),
+        "mlq_spectral_data_field": (ci, [vp, ci, ci, dp]), "mlq_rescaleLqNorm": (ci, [vp, dp, cd]),
         "mlq_calcConstraintDerivativeL2": (ci, [vp, dp, dp]),
         "mlq_vectorTransport": (ci, [vp, dp, cd, cd, dp, dp, dp]),
         "mlq_inner_product": (ci, [vp, dp, dp, ci, dp]),

@@ -223,6 +223,17 @@ int mlq_calc_gk_order2(mlq_ctx* ctx, const double* g, double k, double* gk) {
     e.download(sca(ctx, 1), gk);
   });
 }
+int mlq_calculate_spectral_data(mlq_ctx* ctx, const double* u, int normalize,
+                                double* spectral_data) {
+  return guarded(ctx, [&] {
+    IEngine& e = *ctx->eng;
+    e.upload(vec(ctx, 0), u);
+    e.spectral_data(vec(ctx, 0), normalize, spectral_data);
+  });
+}
+int mlq_spectral_data_field(mlq_ctx* ctx, mlq_field u, int normalize, double* spectral_data) {
+  return guarded(ctx, [&] { ctx->eng->spectral_data(u, normalize, spectral_data); });
+}
 int mlq_calc_global_Lq_norm(mlq_ctx* ctx, const double* u, double* norm) {
   return guarded(ctx, [&] {
     IEngine& e = *ctx->eng;

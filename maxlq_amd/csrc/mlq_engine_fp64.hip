@@ -20,6 +20,16 @@ __global__ void k_reduce_final(const double* part, int nblk, int K, double* out)
   if (threadIdx.x == 0) out[k] = sh[0];
 }
 
+// out[s] = sum of the workgroup partials part[b * n + s] in workgroup order
+__global__ void k_shell_final(const double* part, int nblk, int n, double* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int b = 0; b < nblk; ++b) acc += part[(long long)b * n + s];
+  out[s] = acc;
+}
+
 template class Engine<double>;
 
 std::unique_ptr<IEngine> make_engine_fp64(int n, const mlq_params& p) {

@@ -85,6 +85,8 @@ template <typename T> class Engine final : public IEngine {
   void vector_transport(int u, double q, double B, int eta, int xi, int out) override;
   double inner(int f, int g, int kind) override;
   void dLqdt_parts(int u_vec, double q, double out[3]) override;
+  void spectral_data(int u_vec, int normalize, double* out) override;
+  void spectral_data_dev(const T* u, bool normalize, double* out);
   void grad_L2(int u_vec, int out_vec) override;
   double eval_J(int f_vec, int system) override;
   void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) override;

@@ -381,6 +381,58 @@ template <typename T> void Engine<T>::calc0thFourierModes(int v, double out[3]) 
   for (int c = 0; c < 3; ++c) out[c] = sum(fphys(v, c));   // unnormalised k=0 coefficient
 }
 
+// calculate_spectral_data (function_ops.f90:2614-2677): shell sums of
+// |fftfwd(u_c)|^2 over the three components; out is the reference's
+// spectral_data(1:n, 1:2) in Fortran order (out[i] = 2 pi (i+1), out[n+i] =
+// shell i), scaled by ||u||_2^2 / (2 pi sum) when normalize
+// (normalizeSpectrumByL2Norm, :2667-2671).
+template <typename T> void Engine<T>::spectral_data(int u_vec, int normalize, double* out) {
+  if (fld(u_vec).ncomp != 3) throw Error(MLQ_EINVAL, "spectral data of a vector field");
+  spectral_data_dev(fphys(u_vec), normalize != 0, out);
+}
+template <typename T>
+void Engine<T>::spectral_data_dev(const T* u, bool normalize, double* out) {
+  const int n = n_;
+  ST s = stmp(3);
+  C* d[3] = {s.get(), s.get() + ns_, s.get() + 2 * ns_};
+  const T* src[3] = {u, u + nr_, u + 2 * nr_};
+  const SpecOp ops[3] = {sp_none(), sp_none(), sp_none()};
+  fwd(3, src, d, ops);
+  constexpr long long MAXBLK = 2048;
+  const long long chunk0 = 256LL * 32;
+  long long nblk = std::min(MAXBLK, (ns_ + chunk0 - 1) / chunk0);
+  long long chunk = (ns_ + nblk - 1) / nblk;
+  chunk = (chunk + 255) / 256 * 256;
+  nblk = (ns_ + chunk - 1) / chunk;
+  const size_t pbytes = sizeof(double) * (size_t)nblk * (size_t)n + sizeof(double) * n;
+  double* part = (double*)dmalloc(pbytes);
+  double* shells = part + nblk * n;
+  prof_begin(PC_REDUCE);
+  hipLaunchKernelGGL((k_shell_sum<T>), dim3((unsigned)nblk), dim3(256), sizeof(double) * n, st_,
+                     d[0], d[1], d[2], n, pp_, ns_, chunk, part);
+  MLQ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_shell_final, dim3((n + 255) / 256), dim3(256), 0, st_, part, (int)nblk, n,
+                     shells);
+  MLQ_HIP(hipGetLastError());
+  prof_end(PC_REDUCE, 0);
+  std::vector<double> h(n);
+  MLQ_HIP(hipMemcpyAsync(h.data(), shells, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
+  MLQ_HIP(hipStreamSynchronize(st_));
+  dfree(part, pbytes);
+  double tot = 0.0;
+  for (int i = 0; i < n; ++i) {
+    out[i] = 2.0 * (4.0 * std::atan2(1.0, 1.0)) * (double)(i + 1);   // kk_max, :2643
+    out[n + i] = h[i];
+    tot += h[i];
+  }
+  if (normalize) {   // :2667-2671
+    double l2 = 0.0;
+    for (int c = 0; c < 3; ++c) l2 += dot(src[c], src[c]) / (double)nr_;
+    const double spectral_ener = 2.0 * (4.0 * std::atan2(1.0, 1.0)) * tot;
+    for (int i = 0; i < n; ++i) out[n + i] = l2 / spectral_ener * out[n + i];
+  }
+}
+
 template <typename T> void Engine<T>::calc_uk(int u_vec, double k, int out_sc) {
   Chain ch;
   chain_init(ch, fphys(u_vec));

@@ -327,4 +327,64 @@ template <typename T> struct S_envelope_leray {
   }
 };
 
+// ---- energy spectrum in spherical shells (calculate_spectral_data,
+// function_ops.f90:2614-2677) ----------------------------------------------
+// Mode m of the full complex spectrum lies in shell i when
+//   2 pi i < |K| <= 2 pi (i+1),  |K| = SQRT(K1**2 + K2**2 + K3**2),
+// K_j = (2 pi) m_j (initialize.f90:170-192), i = 0 .. n-1 (:2640-2659).  |K|
+// is evaluated in the reference's FP64 operation order with no contraction
+// and a correctly rounded square root, and the shell bounds as the reference
+// forms them, so a mode on a shell boundary (|m| an integer) lands in the
+// same shell.  The half spectrum holds the modes with 0 < kx < n/2 for
+// themselves and their mirror -m (same |K|, same |u_hat|^2): weight 2.
+__device__ __forceinline__ double shell_bound(int i) {   // 2.0_pr*PI*REAL(i,pr)
+  return TWO_PI * (double)i;
+}
+// K1**2 + K2**2 + K3**2, K_j = 2 pi m_j, as the reference evaluates it: each
+// square rounded, summed left to right (a fused multiply-add would move modes
+// with integer |m| across a shell edge)
+__device__ __forceinline__ double kabs2_ref(int m1, int m2, int m3) {
+#pragma clang fp contract(off)
+  const double K1 = TWO_PI * (double)m1, K2 = TWO_PI * (double)m2, K3 = TWO_PI * (double)m3;
+  return (K1 * K1 + K2 * K2) + K3 * K3;
+}
+// the shell of |K| = nk > 0 (n when beyond the last shell); both loops are
+// bounded whatever nk holds
+__device__ __forceinline__ int shell_of(double nk, int n) {
+  int i = (int)fmin(fmax(nk / TWO_PI, 0.0), (double)n);
+  while (i > 0 && !(shell_bound(i) < nk)) --i;
+  while (i < n && !(nk <= shell_bound(i + 1))) ++i;
+  return i;
+}
+// Each workgroup sums its contiguous chunk of modes into LDS bins and writes
+// them to part[blockIdx.x * n + shell]; k_shell_final adds the workgroups'
+// partials in workgroup order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_shell_sum(const c2<T>* s0, const c2<T>* s1,
+                                                   const c2<T>* s2, int n, int px,
+                                                   long long nmodes, long long chunk,
+                                                   double* part) {
+  extern __shared__ double bins[];
+  for (int b = threadIdx.x; b < n; b += blockDim.x) bins[b] = 0.0;
+  __syncthreads();
+  const long long i0 = (long long)blockIdx.x * chunk;
+  const long long i1 = i0 + chunk < nmodes ? i0 + chunk : nmodes;
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    KIdx k;
+    if (!kdecode(i, n, px, k)) continue;
+    const double ksq = kabs2_ref(k.mx, k.my, k.mz);
+    if (ksq == 0.0) continue;   // |K| = 0 is in no shell
+    const int sh = shell_of(__dsqrt_rn(ksq), n);
+    if (sh >= n) continue;
+    const c2<T> a = gld<NT_PROD>(s0 + i), b = gld<NT_PROD>(s1 + i), c = gld<NT_PROD>(s2 + i);
+    const double e = ((double)a.x * a.x + (double)a.y * a.y) + ((double)b.x * b.x + (double)b.y * b.y) +
+                     ((double)c.x * c.x + (double)c.y * c.y);
+    const double w = (k.kx == 0 || k.kx == (n >> 1)) ? 1.0 : 2.0;
+    atomicAdd(&bins[sh], w * e);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < n; b += blockDim.x) part[(long long)blockIdx.x * n + b] = bins[b];
+}
+__global__ void k_shell_final(const double* part, int nblk, int n, double* out);
+
 }  // namespace mlq

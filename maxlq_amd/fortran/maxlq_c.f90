@@ -326,6 +326,21 @@ module maxlq_c
          integer(c_int), value :: batch, reps
          real(c_double), intent(out) :: t_ms(6)
       end function
+      integer(c_int) function mlq_calculate_spectral_data(ctx, u, normalize, spectral_data) &
+            bind(C, name="mlq_calculate_spectral_data")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         integer(c_int), value :: normalize
+         real(c_double), intent(out) :: spectral_data(*)
+      end function
+      integer(c_int) function mlq_spectral_data_field(ctx, u, normalize, spectral_data) &
+            bind(C, name="mlq_spectral_data_field")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: u, normalize
+         real(c_double), intent(out) :: spectral_data(*)
+      end function
       integer(c_int) function mlq_time_copy(ctx, nfields, reps, t_ms) bind(C, name="mlq_time_copy")
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx

@@ -171,6 +171,21 @@ class MaxLq:
         self._chk(self.lib.mlq_calc_global_Lq_norm(self.ctx, _p(u), C.byref(r)))
         return r.value
 
+    def calculate_spectral_data(self, u, normalize=True):
+        """calculate_spectral_data (function_ops.f90:2614-2677): (n, 2) array,
+        column 0 the shell bounds 2 pi (i+1), column 1 the shell energies."""
+        u = _f64(u, self.vshape)
+        out = np.zeros(2 * self.n)
+        self._chk(self.lib.mlq_calculate_spectral_data(self.ctx, _p(u), int(bool(normalize)),
+                                                       _p(out)))
+        return out.reshape(2, self.n).T.copy()
+
+    def spectral_data_field(self, h, normalize=True):
+        """The same on a device-resident vector field handle."""
+        out = np.zeros(2 * self.n)
+        self._chk(self.lib.mlq_spectral_data_field(self.ctx, h, int(bool(normalize)), _p(out)))
+        return out.reshape(2, self.n).T.copy()
+
     def rescaleLqNorm(self, u, B):
         a = _f64(u, self.vshape).copy()
         self._chk(self.lib.mlq_rescaleLqNorm(self.ctx, _p(a), float(B)))

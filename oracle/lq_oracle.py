@@ -260,6 +260,29 @@ class Oracle:
             s = s + self.global_inner_product(f[i], g_[i], mytype)
         return s
 
+    # -- diagnostics ------------------------------------------------------------
+    def calculate_spectral_data(self, u, normalize=True):
+        """calculate_spectral_data, function_ops.f90:2614-2677: the (n, 2)
+        array of shell upper bounds 2 pi (i+1) and shell sums of
+        |fftfwd(u_c)|^2 over the three components and the modes with
+        kk_min < |K| <= kk_max (:2640-2659); with normalize
+        (normalizeSpectrumByL2Norm, global_variables.f90:76) the sums are
+        scaled by ||u||_2^2 / (2 pi sum) (:2667-2671)."""
+        g = self.g
+        e = sum(np.abs(fftfwd(u[c])) ** 2 for c in range(3))      # :2634-2638, :2653
+        out = np.zeros((g.n, 2))
+        for i in range(g.n):
+            kk_min = 2.0 * g.pi * float(i)                          # :2641
+            kk_max = 2.0 * g.pi * float(i + 1)                      # :2642
+            out[i, 0] = kk_max                                      # :2643
+            sel = (kk_min < g.kabs) & (g.kabs <= kk_max)            # :2651-2652
+            out[i, 1] = float(e[sel].sum())
+        if normalize:                                               # :2667-2671
+            l2 = self.global_summed_field_inner_product(u, u, INNER_L2)
+            spectral_ener = 2.0 * g.pi * float(out[:, 1].sum())
+            out[:, 1] = l2 / spectral_ener * out[:, 1]
+        return out
+
     # -- pointwise nonlinear helpers -----------------------------------------
     def calc_pointwiseVectorNorm(self, u):
         """function_ops.f90:1775-1791."""
@@ -909,4 +932,5 @@ PINS = {
     "mnbrak/brent": "tests/test_oracle_linesearch.py",
     "kappa_test/kappa_best/kappa_adjust": "tests/test_oracle_pins.py::test_kappa_*",
     "maxdLqdt": "tests/test_oracle_linesearch.py::test_rcg_monotone_and_sphere",
+    "calculate_spectral_data": "tests/test_oracle_pins.py::test_spectrum_*",
 }

@@ -357,3 +357,45 @@ def test_kappa_tends_to_one_for_exact_gradient():
     assert tab[13, 3] < 0.0          # eps = 0.1: far outside the linear range
     assert abs(best - 1.0) < 1e-4 and 1 <= row <= 16
     assert abs(frac - 1.0) < 1e-3
+
+
+# ---- calculate_spectral_data (function_ops.f90:2614-2677) -------------------
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_spectrum_parseval_every_mode_once(n):
+    """The shells 2 pi i < |K| <= 2 pi (i+1), i < n, hold every mode but k = 0
+    exactly once: shell sums + |u_hat(0)|^2 = n^3 sum |u|^2 (Parseval of the
+    unnormalised DFT).  A missing or doubled shell edge fails this."""
+    u = np.random.default_rng(n).standard_normal((3, n, n, n))
+    orc = Oracle(Params(n=n, q=4.0))
+    sp = orc.calculate_spectral_data(u, normalize=False)
+    zero = sum(abs(u[c].sum()) ** 2 for c in range(3))
+    assert rel(sp[:, 1].sum() + zero, n ** 3 * float((u * u).sum())) < 1e-12
+    assert np.allclose(sp[:, 0], TP * np.arange(1, n + 1), rtol=1e-15, atol=0)
+
+
+def test_spectrum_single_mode_closed_form():
+    """cos(2 pi (x + y)): two modes +-(1, 1, 0) of |u_hat|^2 = (n^3/2)^2 at
+    |m| = sqrt 2, i.e. shell 1; normalised, 2 pi sum = ||u||_2^2 = 1/2."""
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    u = np.stack([np.cos(TP * (X + Y)), 0 * X, 0 * X])
+    orc = Oracle(Params(n=n, q=4.0))
+    sp = orc.calculate_spectral_data(u, normalize=False)
+    want = np.zeros(n)
+    want[1] = 2 * (n ** 3 / 2.0) ** 2
+    assert np.allclose(sp[:, 1], want, rtol=0, atol=1e-12 * want[1])
+    spn = orc.calculate_spectral_data(u, normalize=True)
+    assert abs(spn[1, 1] - 1.0 / (4.0 * math.pi)) < 1e-14
+    assert abs(TP * spn[:, 1].sum() - 0.5) < 1e-14
+
+
+def test_spectrum_shell_edges():
+    """|m| = 3 exactly (m = (3, 0, 0): |K| = SQRT(K1**2) = K1 = 2 pi 3, the
+    upper bound of shell 2) lies in shell 2, not 3: the upper edge belongs to
+    the shell, the lower edge does not."""
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    u = np.stack([0 * X, np.cos(TP * 3 * X), 0 * X])
+    sp = Oracle(Params(n=n, q=4.0)).calculate_spectral_data(u, normalize=False)
+    assert sp[2, 1] > 0.49 * n ** 6 and sp[3, 1] < 1e-12 * n ** 6
