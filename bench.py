@@ -184,6 +184,51 @@ def bench_linesearch8(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_spectrum(args):
+    """calculate_spectral_data (function_ops.f90:2614-2677) of a 512^3 field
+    (seed 4, exp(-|m|^2/256)) on the GPU, FP64 and mixed, against the oracle
+    (the reference's shell loop: n passes over the n^3 modes) on a 128^3 field,
+    scaled by the loop's n^4 cost."""
+    from maxlq_amd.lq import MaxLq
+    n, q, B = args.n, args.q, args.B
+    res = {}
+    for prec in ("fp64", "mixed"):
+        g = MaxLq(n, q=q, B=B, precision=prec)
+        h = g.field(3)
+        g.seeded_field(h, 4, "gauss", 256.0)
+        g.spectral_data_field(h, True)                       # warm-up (allocations)
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            sp = g.spectral_data_field(h, True)
+            ts.append(time.perf_counter() - t0)
+        w = 8 if prec == "fp64" else 4
+        rb, sb = g.sizes()
+        # 3 R2C transforms (field + half spectrum each), the shell pass over
+        # 3 half spectra and the L2 norm's read of the field
+        byts = 3 * (rb + sb) + 3 * sb + 3 * rb
+        t = min(ts)
+        res[prec] = {"ms": t * 1e3, "GBs": byts / t / 1e9, "bytes": byts,
+                     "energy_check": float(2.0 * np.pi * sp[:, 1].sum())}
+        g.close()
+    from oracle.lq_oracle import Oracle, Params
+    from oracle.seeded import seeded_field
+    nc = 128
+    u = seeded_field(nc, 4, "gauss", 256.0 * (nc / n) ** 2, q, B)
+    orc = Oracle(Params(n=nc, q=q))
+    t0 = time.perf_counter()
+    orc.calculate_spectral_data(u, True)
+    tc = time.perf_counter() - t0
+    scale = (n / nc) ** 4
+    out = {"metric": "energy spectrum in shells (calculate_spectral_data) at 512^3: GPU ms",
+           "value": res["mixed"]["ms"], "unit": "ms", "higher_is_better": False,
+           "config": {"workload": f"calculate_spectral_data of a {n}^3 field, seed 4, exp(-|m|^2/256), "
+                                  "normalised by the L2 norm", "gpu": res,
+                      "cpu_oracle": {"n": nc, "s": tc, "scaled_to_n_s": tc * scale,
+                                     "scale": f"(n/{nc})^4, the reference's n shell passes over n^3 modes"}}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_continuation(args):
     """256^3, q = 6, seed 3, |m|^2 exp(-|m|^2/64): B_j = 4^(j/9), j = 0..9,
     30 RCG iterations per B (OPTIM_TOL 0), mixed and FP64 each continuing
@@ -274,11 +319,13 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--fft-only", choices=["mixed", "fp64"], default=None,
                     help="only the batched transform sweep and a copy kernel (for profilers)")
-    ap.add_argument("--case", choices=["iteration", "linesearch8", "continuation", "n1024"],
+    ap.add_argument("--case", choices=["iteration", "linesearch8", "continuation", "n1024",
+                                       "spectrum"],
                     default="iteration",
                     help="iteration: the headline line (default); linesearch8: 8 step sizes "
                          "batched vs sequential at 512^3; continuation: 256^3 q=6 B-continuation "
-                         "mixed vs FP64; n1024: one 1024^3 q=9 iteration and its memory peak")
+                         "mixed vs FP64; n1024: one 1024^3 q=9 iteration and its memory peak; "
+                         "spectrum: the shell energy spectrum at 512^3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -287,6 +334,7 @@ def main():
     n, q, B = args.n, args.q, args.B
     if args.case != "iteration":
         {"linesearch8": bench_linesearch8, "continuation": bench_continuation,
+         "spectrum": bench_spectrum,
          "n1024": bench_n1024}[args.case](args)
         return
     if args.fft_only:

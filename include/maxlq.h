@@ -190,10 +190,17 @@ typedef struct mlq_rcg_cfg {
   int reset_every;       /* resetMomentumTermEveryXiterations (25) */
   int rolling;           /* toleranceRollingAverageSize (10) */
   int flags;             /* MLQ_RCG_* bits */
+  int spectra_every;     /* save_spectraEveryXiteration (global_variables.f90:74): every
+                            this many iterations the solve delivers MLQ_LOG_SPECTRUM
+                            records of uvec and d (optimization.f90:119-121, 377-381);
+                            < 1 never */
 } mlq_rcg_cfg;
 /* cfg.flags: run every operator from the field as the reference orders it
  * (no resident state, no tau-polynomial line search); validation path */
 #define MLQ_RCG_REFERENCE 1
+/* cfg.flags: spectra without the L2 normalisation (normalizeSpectrumByL2Norm
+ * = .false., global_variables.f90:76) */
+#define MLQ_RCG_SPECTRA_RAW 2
 
 typedef struct mlq_rcg_result {
   double J1;             /* final R(u) */
@@ -217,6 +224,10 @@ typedef struct mlq_rcg_result {
 #define MLQ_LOG_BRENT 3    /* brent-info: i0=j, d[0..1]=X,FX (optimization.f90:836-946) */
 #define MLQ_LOG_MSG 4      /* optim_msg_handle: i0=message code (data_ops.f90:1571-1632) */
 #define MLQ_LOG_ERR 5      /* optim_error_handle: i0=error code (:1527-1566) */
+#define MLQ_LOG_SPECTRUM 6 /* calculateSaveSpectrum (function_ops.f90:1011-1023): i0=0 uvec,
+                              1 d; d[0..2n1-1] = spectral_data(1:n1,1:2) as
+                              mlq_calculate_spectral_data returns it (save_spectral_data,
+                              data_ops.f90:227-261, writes it) */
 typedef void (*mlq_log_cb)(void* user, int kind, int iter, int i0, int i1, const double* d);
 
 int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_result* res,

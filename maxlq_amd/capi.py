@@ -18,6 +18,7 @@ MLQ_FP64, MLQ_MIXED = 0, 1
 MLQ_IP_L2, MLQ_IP_HL, MLQ_IP_H, MLQ_IP_OLD_H, MLQ_IP_OLD_HL = range(5)
 MLQ_J_MAXDLQDT, MLQ_J_LINEMIN, MLQ_J_LQNORM = range(3)
 MLQ_LOG_ITER, MLQ_LOG_LINEMIN, MLQ_LOG_BRENT, MLQ_LOG_MSG, MLQ_LOG_ERR = 1, 2, 3, 4, 5
+MLQ_LOG_SPECTRUM = 6
 MLQ_RCG_REFERENCE = 1   # mlq_rcg_cfg.flags: reference-structured iteration
 
 # symbols the header declares (checked by the CPU test suite)
@@ -55,7 +56,7 @@ class Params(C.Structure):
 class RcgCfg(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("optim_tol", C.c_double), ("start_iter", C.c_int),
                 ("batch_linesearch", C.c_int), ("reset_every", C.c_int), ("rolling", C.c_int),
-                ("flags", C.c_int)]
+                ("flags", C.c_int), ("spectra_every", C.c_int)]
 
 
 class RcgResult(C.Structure):

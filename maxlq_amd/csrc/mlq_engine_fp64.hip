@@ -20,14 +20,20 @@ __global__ void k_reduce_final(const double* part, int nblk, int K, double* out)
   if (threadIdx.x == 0) out[k] = sh[0];
 }
 
-// out[s] = sum of the workgroup partials part[b * n + s] in workgroup order
+// out[s] = sum of the workgroup partials part[b * n + s]: one workgroup per
+// shell (grid n), a fixed summation order
 __global__ void k_shell_final(const double* part, int nblk, int n, double* out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  __shared__ double sh[256];
+  const int s = blockIdx.x;
   double acc = 0.0;
-#pragma unroll 8
-  for (int b = 0; b < nblk; ++b) acc += part[(long long)b * n + s];
-  out[s] = acc;
+  for (int b = threadIdx.x; b < nblk; b += 256) acc += part[(long long)b * n + s];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = sh[0];
 }
 
 template class Engine<double>;

@@ -87,6 +87,19 @@ template <typename T> class Engine final : public IEngine {
   void dLqdt_parts(int u_vec, double q, double out[3]) override;
   void spectral_data(int u_vec, int normalize, double* out) override;
   void spectral_data_dev(const T* u, bool normalize, double* out);
+  // spectra records of the solve (save_spectraEveryXiteration)
+  static bool spectra_due(const mlq_rcg_cfg& cfg, int iter) {
+    return cfg.spectra_every > 0 && iter % cfg.spectra_every == 0;
+  }
+  void spectrum_of(const mlq_rcg_cfg& cfg, const T* u, std::vector<double>& out) {
+    out.assign(2 * (size_t)n_, 0.0);
+    spectral_data_dev(u, !(cfg.flags & MLQ_RCG_SPECTRA_RAW), out.data());
+  }
+  void emit_spectrum(const LogSink& log, const mlq_rcg_cfg& cfg, int iter, int which, const T* u) {
+    std::vector<double> s;
+    spectrum_of(cfg, u, s);
+    log.emit(MLQ_LOG_SPECTRUM, iter, which, 0, s.data());
+  }
   void grad_L2(int u_vec, int out_vec) override;
   double eval_J(int f_vec, int system) override;
   void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) override;

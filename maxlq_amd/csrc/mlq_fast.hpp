@@ -141,6 +141,14 @@ template <typename T> class FastRcg {
   }
 
   void download_u(T* dst) { e_.map(F_copy<T>{u_, dst}, nr_ * 3); }
+  const T* u_field() const { return u_; }
+  // the direction d1 = d_ + gam_ u_ (the state keeps its u component as the
+  // scalar gam_), between precompute_ls and the step
+  RT direction_field() {
+    RT d1 = e_.rtmp(3);
+    e_.map(F_axpby<T>{d_, u_, d1, (T)1, (T)gam_}, 3 * nr_);
+    return d1;
+  }
   // |u|^(q-2) of the state (at q = 4 without clipping it is D(u.u) = s_)
   const T* Acur() const { return A_.get() ? A_.get() : s_.get(); }
 
@@ -861,12 +869,20 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
     double d[3] = {0.0, 0.0, J0};
     log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
   }
+  if (e.spectra_due(cfg, iter)) e.emit_spectrum(log, cfg, iter, 0, e.fphys(u));   // :119-121
+  std::vector<double> dspec;
   while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
     iter += 1;
     auto dir = F.direction(iter, cfg.reset_every);
     for (int m : dir.msgs) msg(iter, m);
     const double beta = dir.beta;
     F.precompute_ls();
+    // the spectrum of d1 (:381) is taken while its physical field exists and
+    // delivered after the iteration's record
+    if (e.spectra_due(cfg, iter)) {
+      auto d1 = F.direction_field();
+      e.spectrum_of(cfg, d1, dspec);
+    }
     // speculative step sizes: the first mnbrak requests (0, tau0, GOLD*tau0)
     const double tB0 = std::max(tau0, MACH_EPSILON);
     std::vector<double> spec = {0.0, tB0, MNBRAK_GOLD * tB0, MNBRAK_CGOLD * tB0,
@@ -925,6 +941,10 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
     {
       double d[3] = {tau1, beta, J1};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
+    }
+    if (e.spectra_due(cfg, iter)) {                                // :377-381
+      e.emit_spectrum(log, cfg, iter, 0, F.u_field());
+      log.emit(MLQ_LOG_SPECTRUM, iter, 1, 0, dspec.data());
     }
     J0 = J1;
     tau0 = tau1;

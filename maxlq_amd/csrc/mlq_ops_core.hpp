@@ -411,8 +411,7 @@ void Engine<T>::spectral_data_dev(const T* u, bool normalize, double* out) {
   hipLaunchKernelGGL((k_shell_sum<T>), dim3((unsigned)nblk), dim3(256), sizeof(double) * n, st_,
                      d[0], d[1], d[2], n, pp_, ns_, chunk, part);
   MLQ_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_shell_final, dim3((n + 255) / 256), dim3(256), 0, st_, part, (int)nblk, n,
-                     shells);
+  hipLaunchKernelGGL(k_shell_final, dim3(n), dim3(256), 0, st_, part, (int)nblk, n, shells);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_REDUCE, 0);
   std::vector<double> h(n);

@@ -54,6 +54,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     double d[3] = {0.0, 0.0, J0};
     log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                       // :117
   }
+  if (spectra_due(cfg, iter)) emit_spectrum(log, cfg, iter, 0, fphys(u));   // :119-121
   const double s = 1.5 - 1.0 / q;
   while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
     iter += 1;
@@ -152,6 +153,10 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     {
       double d[3] = {tau1, beta, J1};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :375
+    }
+    if (spectra_due(cfg, iter)) {                                   // :377-381
+      emit_spectrum(log, cfg, iter, 0, fphys(u));
+      emit_spectrum(log, cfg, iter, 1, fphys(d1));
     }
     J0 = J1;                                                        // :432-436
     gu0_norm = inner(gu1, gu1, HL);

@@ -26,7 +26,7 @@ module maxlq_c
    type, bind(C) :: mlq_rcg_cfg
       integer(c_int) :: max_iter
       real(c_double) :: optim_tol
-      integer(c_int) :: start_iter, batch_linesearch, reset_every, rolling, flags
+      integer(c_int) :: start_iter, batch_linesearch, reset_every, rolling, flags, spectra_every
    end type mlq_rcg_cfg
 
    type, bind(C) :: mlq_rcg_result

@@ -63,6 +63,17 @@ contains
       call mlq_check(mlq_calc_global_Lq_norm(mlq_ctx_ptr, u, norm), "calc_global_Lq_norm")
    end function gpu_calc_global_Lq_norm
 
+   subroutine gpu_calculate_spectral_data(u, spectral_data)    ! :2614-2677
+      real(pr), dimension(1:n(1),1:n(2),1:local_N,1:3), intent(in) :: u
+      real(pr), dimension(1:n(1),1:2), intent(out) :: spectral_data
+      integer(c_int) :: norm_flag
+      call mlq_sync_params()
+      norm_flag = 0
+      if (normalizeSpectrumByL2Norm) norm_flag = 1
+      call mlq_check(mlq_calculate_spectral_data(mlq_ctx_ptr, u, norm_flag, spectral_data), &
+                     "calculate_spectral_data")
+   end subroutine gpu_calculate_spectral_data
+
    subroutine gpu_rescaleLqNorm(u, B)                          ! :1699-1709
       real(pr), dimension(1:n(1),1:n(2),1:local_N,1:3), intent(inout) :: u
       real(pr), intent(in) :: B

@@ -40,6 +40,7 @@ class RcgTrace:
     brent: list = field(default_factory=list)            # (iter, j, X, FX)
     msgs: list = field(default_factory=list)             # (iter, code)
     errors: list = field(default_factory=list)           # (iter, code)
+    spectra: list = field(default_factory=list)          # (iter, "uvec" | "d", (n, 2) array)
 
 
 class MaxLq:
@@ -301,13 +302,13 @@ class MaxLq:
         return tab.reshape(16, 5), frac.value, best.value, row.value
 
     def maxdLqdt(self, hu, max_iter=99999, optim_tol=None, start_iter=0, batch_linesearch=0,
-                 reset_every=25, rolling=10, trace=True):
+                 reset_every=25, rolling=10, trace=True, spectra_every=0, flags=0):
         """One RCG solve for the current B on the device-resident field ``hu``
         (optimization.f90:7-514).  Returns (RcgResult, RcgTrace)."""
         if optim_tol is None:
             optim_tol = 1.0e-4 if self.n >= 1024 else 1.0e-5       # initialize.f90:95-100
         cfg = capi.RcgCfg(int(max_iter), float(optim_tol), int(start_iter), int(batch_linesearch),
-                          int(reset_every), int(rolling), 0)
+                          int(reset_every), int(rolling), int(flags), int(spectra_every))
         res = capi.RcgResult()
         tr = RcgTrace()
 
@@ -322,6 +323,9 @@ class MaxLq:
                 tr.msgs.append((it, i0))
             elif kind == capi.MLQ_LOG_ERR:
                 tr.errors.append((it, i0))
+            elif kind == capi.MLQ_LOG_SPECTRUM:
+                sp = np.ctypeslib.as_array(d, shape=(2 * self.n,)).copy()
+                tr.spectra.append((it, "uvec" if i0 == 0 else "d", sp.reshape(2, self.n).T.copy()))
 
         cbf = capi.LOG_CB(cb) if trace else C.cast(None, capi.LOG_CB)
         self._chk(self.lib.mlq_rcg_solve(self.ctx, hu, C.byref(cfg), C.byref(res), cbf, None))

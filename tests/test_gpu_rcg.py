@@ -141,3 +141,38 @@ def test_mixed_vs_fp64_rcg_32():
     for a, b in zip(t32["iter"], t64["iter"]):
         print("mixed vs fp64 J per iteration:", a[0], a[3], b[3], abs(a[3] - b[3]) / abs(b[3]))
         assert abs(a[3] - b[3]) < 1e-4 * abs(b[3])
+
+
+def test_rcg_spectra_records():
+    """save_spectraEveryXiteration inside the solve (optimization.f90:119-121,
+    377-381): uvec at the start and uvec and d every 2nd iteration; the last
+    uvec record is the spectrum of the solve's result, and the resident-state
+    and reference-structured solves deliver the same spectra."""
+    from maxlq_amd.lq import MaxLq
+    n, q, B = 32, 4.0, 1.0
+    u0 = seeded_field(n, 2, "gauss", 16.0, q=q, B=B)
+    orc = L.Oracle(L.Params(n=n, q=q, B=B))
+    ustart = orc.divAvg_free(orc.rescaleLqNorm(u0, B))      # optimization.f90:61-62
+    runs = {}
+    for fl in (0, REF):
+        g = MaxLq(n, q=q, B=B, precision="fp64")
+        h = g.field(3)
+        g.upload(h, u0)
+        res, tr = g.maxdLqdt(h, max_iter=4, optim_tol=0.0, spectra_every=2, flags=fl,
+                             batch_linesearch=3 if fl == 0 else 0)
+        assert res.iters == 4
+        assert [(it, w) for it, w, _ in tr.spectra] == [(0, "uvec"), (2, "uvec"), (2, "d"),
+                                                         (4, "uvec"), (4, "d")]
+        u = g.download(h)
+        last = tr.spectra[3][2]
+        assert np.allclose(last, g.calculate_spectral_data(u, True), rtol=1e-12, atol=0)
+        assert abs(2 * np.pi * last[:, 1].sum() - float((u * u).sum()) / n ** 3) < 1e-12
+        runs[fl] = tr.spectra
+        # raw spectra (normalizeSpectrumByL2Norm = .false.) of the start field
+        g.upload(h, u0)
+        res, tr = g.maxdLqdt(h, max_iter=1, optim_tol=0.0, spectra_every=1, flags=fl | 2)
+        assert np.allclose(tr.spectra[0][2], orc.calculate_spectral_data(ustart, False),
+                           rtol=1e-10, atol=1e-10 * n ** 6)
+    for (a_it, a_w, a), (b_it, b_w, b) in zip(runs[0], runs[REF]):
+        assert (a_it, a_w) == (b_it, b_w)
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
