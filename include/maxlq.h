@@ -117,6 +117,12 @@ int mlq_calculate_spectral_data(mlq_ctx* ctx, const double* u, int normalize,
 /* the same on a device-resident vector field (no counterpart in the
  * reference: its fields are host arrays) */
 int mlq_spectral_data_field(mlq_ctx* ctx, mlq_field u, int normalize, double* spectral_data);
+/* calc_global_summed_enstrophy(u), function_ops.f90:2455-2467 (Enstrophy
+ * :2422-2442, vel2vort :2982-3042): 1/2 ||curl u||_2^2 */
+int mlq_calc_global_summed_enstrophy(mlq_ctx* ctx, const double* u, double* enstrophy);
+/* calc_global_dEnstrophydt(u), function_ops.f90:1417-1481:
+ * -viscCoefficient visc ||Lap u||_2^2 + <D((u.grad) u), Lap u>_L2 */
+int mlq_calc_global_dEnstrophydt(mlq_ctx* ctx, const double* u, double* dEdt);
 int mlq_rescaleLqNorm(mlq_ctx* ctx, double* u, double B);                 /* :1699-1709 */
 int mlq_calcConstraintDerivativeL2(mlq_ctx* ctx, const double* u, double* out); /* :984-1006 */
 int mlq_vectorTransport(mlq_ctx* ctx, const double* u, double q, double B,
@@ -215,6 +221,9 @@ typedef struct mlq_rcg_result {
   int n_J_poly;          /* of n_J_computed, those given by the exact line-search
                             polynomials (q = 4, no clipping of D(phi.phi)) */
   int pad_;
+  double enstrophy;      /* calc_global_summed_enstrophy(Uvec) of the result and */
+  double dEnstrophydt;   /* calc_global_dEnstrophydt(Uvec), for
+                            save_to_optimizationResultList (optimization.f90:491-493) */
 } mlq_rcg_result;
 
 /* log record kinds delivered to the callback (data_ops.f90 writers) */

@@ -28,7 +28,8 @@ EXPORTS = [
     "mlq_fftfwdV", "mlq_fftbwdV", "mlq_gradient", "mlq_divergence", "mlq_laplacian",
     "mlq_solve_poisson", "mlq_dealias_scalar", "mlq_divAvg_free", "mlq_HilbertGradient",
     "mlq_calc0thFourierModes", "mlq_calc_uk", "mlq_calc_gk_order2", "mlq_calc_global_Lq_norm",
-    "mlq_calculate_spectral_data", "mlq_spectral_data_field",
+    "mlq_calculate_spectral_data", "mlq_spectral_data_field", "mlq_calc_global_summed_enstrophy",
+    "mlq_calc_global_dEnstrophydt",
     "mlq_rescaleLqNorm", "mlq_calcConstraintDerivativeL2", "mlq_vectorTransport",
     "mlq_inner_product", "mlq_global_inner_product", "mlq_global_summed_field_inner_product",
     "mlq_calc_local_dLqdt_inclParts", "mlq_calc_global_dLqdt_inclParts", "mlq_GradL2ForLq",
@@ -63,7 +64,7 @@ class RcgResult(C.Structure):
     _fields_ = [("J1", C.c_double), ("iters", C.c_int), ("successful", C.c_int),
                 ("while_flag", C.c_int), ("stop_reason", C.c_int), ("n_J", C.c_int),
                 ("n_J_computed", C.c_int), ("seconds", C.c_double), ("n_J_poly", C.c_int),
-                ("pad_", C.c_int)]
+                ("pad_", C.c_int), ("enstrophy", C.c_double), ("dEnstrophydt", C.c_double)]
 
 
 LOG_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -102,7 +103,9 @@ def load(path: str = LIB_PATH):
         "mlq_calc_uk": (ci, [vp, dp, cd, dp]), "mlq_calc_gk_order2": (ci, [vp, dp, cd, dp]),
         "mlq_calc_global_Lq_norm": (ci, [vp, dp, dp]),
         "mlq_calculate_spectral_data": (ci, [vp, dp, ci, dp]),
-        "mlq_spectral_data_field": (ci, [vp, ci, ci, dp]), "mlq_rescaleLqNorm": (ci, [vp, dp, cd]),
+        "mlq_spectral_data_field": (ci, [vp, ci, ci, dp]),
+        "mlq_calc_global_summed_enstrophy": (ci, [vp, dp, dp]),
+        "mlq_calc_global_dEnstrophydt": (ci, [vp, dp, dp]), "mlq_rescaleLqNorm": (ci, [vp, dp, cd]),
         "mlq_calcConstraintDerivativeL2": (ci, [vp, dp, dp]),
         "mlq_vectorTransport": (ci, [vp, dp, cd, cd, dp, dp, dp]),
         "mlq_inner_product": (ci, [vp, dp, dp, ci, dp]),

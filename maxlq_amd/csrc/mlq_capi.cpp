@@ -231,6 +231,20 @@ int mlq_calculate_spectral_data(mlq_ctx* ctx, const double* u, int normalize,
     e.spectral_data(vec(ctx, 0), normalize, spectral_data);
   });
 }
+int mlq_calc_global_summed_enstrophy(mlq_ctx* ctx, const double* u, double* enstrophy) {
+  return guarded(ctx, [&] {
+    IEngine& e = *ctx->eng;
+    e.upload(vec(ctx, 0), u);
+    *enstrophy = e.enstrophy(vec(ctx, 0));
+  });
+}
+int mlq_calc_global_dEnstrophydt(mlq_ctx* ctx, const double* u, double* dEdt) {
+  return guarded(ctx, [&] {
+    IEngine& e = *ctx->eng;
+    e.upload(vec(ctx, 0), u);
+    *dEdt = e.dEnstrophydt(vec(ctx, 0));
+  });
+}
 int mlq_spectral_data_field(mlq_ctx* ctx, mlq_field u, int normalize, double* spectral_data) {
   return guarded(ctx, [&] { ctx->eng->spectral_data(u, normalize, spectral_data); });
 }

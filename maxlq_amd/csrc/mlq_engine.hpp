@@ -71,6 +71,8 @@ class IEngine {
   virtual double inner(int f, int g, int kind) = 0;
   virtual void dLqdt_parts(int u_vec, double q, double out[3]) = 0;
   virtual void spectral_data(int u_vec, int normalize, double* out) = 0;
+  virtual double enstrophy(int u_vec) = 0;
+  virtual double dEnstrophydt(int u_vec) = 0;
   virtual void grad_L2(int u_vec, int out_vec) = 0;
   virtual double eval_J(int f_vec, int system) = 0;
   virtual void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) = 0;

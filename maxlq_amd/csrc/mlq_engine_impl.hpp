@@ -86,6 +86,10 @@ template <typename T> class Engine final : public IEngine {
   double inner(int f, int g, int kind) override;
   void dLqdt_parts(int u_vec, double q, double out[3]) override;
   void spectral_data(int u_vec, int normalize, double* out) override;
+  double enstrophy(int u_vec) override;
+  double dEnstrophydt(int u_vec) override;
+  double enstrophy_dev(const T* u);
+  double dEnstrophydt_dev(const T* u);
   void spectral_data_dev(const T* u, bool normalize, double* out);
   // spectra records of the solve (save_spectraEveryXiteration)
   static bool spectra_due(const mlq_rcg_cfg& cfg, int iter) {

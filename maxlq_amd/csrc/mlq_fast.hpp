@@ -966,6 +966,8 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   res.n_J = (int)F.nJ_;
   res.n_J_computed = (int)F.nJc_;
   res.n_J_poly = (int)F.nJpoly_;
+  res.enstrophy = e.enstrophy_dev(e.fphys(u));                     // :491-492
+  res.dEnstrophydt = e.dEnstrophydt_dev(e.fphys(u));
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

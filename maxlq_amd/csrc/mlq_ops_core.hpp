@@ -432,6 +432,95 @@ void Engine<T>::spectral_data_dev(const T* u, bool normalize, double* out) {
   }
 }
 
+// calc_global_summed_enstrophy (function_ops.f90:2455-2467): Enstrophy
+// (:2422-2442) of the vorticity from vel2vort (:2982-3042), 1/2 sum_c
+// ||w_c||^2 dV with w = REAL(ifft(i K x u_hat)) (i K_j zero at the own-dimension
+// Nyquist, which REAL() discards).
+template <typename T> double Engine<T>::enstrophy_dev(const T* u) {
+  ST uh[3] = {stmp(), stmp(), stmp()};
+  C* uhp[3] = {uh[0], uh[1], uh[2]};
+  spec3(u, uhp);
+  // w_x = d_y u_z - d_z u_y, w_y = d_z u_x - d_x u_z, w_z = d_x u_y - d_y u_x
+  static const int comp[6] = {2, 1, 0, 2, 1, 0}, dim[6] = {1, 2, 2, 0, 0, 1};
+  RT dw = rtmp(6);
+  InvSrc is[6];
+  SpecOp ops[6];
+  T* dst[6];
+  for (int k = 0; k < 6; ++k) {
+    is[k] = InvSrc{{uhp[comp[k]], nullptr, nullptr}, {1.0, 0, 0}, 1};
+    ops[k] = sp_ik(dim[k]);
+    dst[k] = dw + (long long)k * nr_;
+  }
+  inv(6, is, ops, dst);
+  double e = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    T* w = dw + (long long)(2 * c) * nr_;
+    map(F_axpby<T>{w, w + nr_, w, (T)1, (T)-1}, nr_);
+    e += 0.5 * dot(w, w) / (double)nr_;
+  }
+  return e;
+}
+// calc_global_dEnstrophydt (function_ops.f90:1417-1481):
+//   R_1 = -viscCoefficient visc ||Lap u||_2^2,  R_2 = <D((u.grad) u), Lap u>_L2
+// with gradU(i, j) = d_j u_i from gradient() and D the order-2 dealias
+// (toDealias); one component of grad u at a time.
+template <typename T> double Engine<T>::dEnstrophydt_dev(const T* u) {
+  ST uh[3] = {stmp(), stmp(), stmp()};
+  C* uhp[3] = {uh[0], uh[1], uh[2]};
+  spec3(u, uhp);
+  RT lap = rtmp(3), adv = rtmp(3);
+  {
+    InvSrc is[3];
+    SpecOp ops[3];
+    T* dst[3];
+    for (int j = 0; j < 3; ++j) {
+      is[j] = InvSrc{{uhp[j], nullptr, nullptr}, {1.0, 0, 0}, 1};
+      SpecOp o;
+      o.flags = SP_NEGKSQ;
+      ops[j] = o;
+      dst[j] = lap + (long long)j * nr_;
+    }
+    inv(3, is, ops, dst);
+  }
+  double r1 = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const T* l = lap + (long long)c * nr_;
+    r1 += dot(l, l) / (double)nr_;
+  }
+  r1 *= -p_.viscCoefficient * p_.visc;
+  {
+    RT g = rtmp(3);
+    for (int i = 0; i < 3; ++i) {
+      InvSrc is[3];
+      SpecOp ops[3];
+      T* dst[3];
+      for (int j = 0; j < 3; ++j) {
+        is[j] = InvSrc{{uhp[i], nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[j] = sp_ik(j);
+        dst[j] = g + (long long)j * nr_;
+      }
+      inv(3, is, ops, dst);
+      map(F_udotgrad<T>{u, g, nr_, adv + (long long)i * nr_}, nr_);
+    }
+  }
+  if (p_.toDealias) {
+    T* a[3] = {adv, adv + nr_, adv + 2 * nr_};
+    dealias_dev(a, 3, 2.0);
+  }
+  double r2 = 0.0;
+  for (int c = 0; c < 3; ++c)
+    r2 += dot(adv + (long long)c * nr_, lap + (long long)c * nr_) / (double)nr_;
+  return r1 + r2;
+}
+template <typename T> double Engine<T>::enstrophy(int u_vec) {
+  if (fld(u_vec).ncomp != 3) throw Error(MLQ_EINVAL, "enstrophy of a vector field");
+  return enstrophy_dev(fphys(u_vec));
+}
+template <typename T> double Engine<T>::dEnstrophydt(int u_vec) {
+  if (fld(u_vec).ncomp != 3) throw Error(MLQ_EINVAL, "dEnstrophydt of a vector field");
+  return dEnstrophydt_dev(fphys(u_vec));
+}
+
 template <typename T> void Engine<T>::calc_uk(int u_vec, double k, int out_sc) {
   Chain ch;
   chain_init(ch, fphys(u_vec));

@@ -136,6 +136,14 @@ template <typename T> struct P_udotforce {
   F_udotforce<T> f;   // o unused
   __device__ void operator()(long long i, T (&r)[1]) const { r[0] = f.at(i); }
 };
+// o = sum_j u_j g_j (one component of (u . grad) u from g_j = d_j u_i,
+// function_ops.f90:1456-1461)
+template <typename T> struct F_udotgrad {
+  const T* u; const T* g; long long nr; T* o;
+  __device__ void operator()(long long i) const {
+    o[i] = u[i] * g[i] + u[nr + i] * g[nr + i] + u[2 * nr + i] * g[2 * nr + i];
+  }
+};
 // aux5_i = sum_j d_i u_j * d_j phi   (function_ops.f90:1165-1169)
 template <typename T> struct F_gradT_dot {
   const T* G; const T* gphi; long long nr; T* o;  // o: 3 components

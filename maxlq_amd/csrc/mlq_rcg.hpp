@@ -178,6 +178,8 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   res.n_J = (int)nJ;
   res.n_J_computed = (int)nJc;
   res.n_J_poly = 0;
+  res.enstrophy = enstrophy_dev(fphys(u));                          // :491-492
+  res.dEnstrophydt = dEnstrophydt_dev(fphys(u));
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

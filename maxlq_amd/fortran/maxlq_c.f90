@@ -34,6 +34,7 @@ module maxlq_c
       integer(c_int) :: iters, successful, while_flag, stop_reason, n_J, n_J_computed
       real(c_double) :: seconds
       integer(c_int) :: n_J_poly, pad_
+      real(c_double) :: enstrophy, dEnstrophydt
    end type mlq_rcg_result
 
    interface
@@ -340,6 +341,20 @@ module maxlq_c
          type(c_ptr), value :: ctx
          integer(c_int), value :: u, normalize
          real(c_double), intent(out) :: spectral_data(*)
+      end function
+      integer(c_int) function mlq_calc_global_summed_enstrophy(ctx, u, enstrophy) &
+            bind(C, name="mlq_calc_global_summed_enstrophy")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: enstrophy
+      end function
+      integer(c_int) function mlq_calc_global_dEnstrophydt(ctx, u, dEdt) &
+            bind(C, name="mlq_calc_global_dEnstrophydt")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: dEdt
       end function
       integer(c_int) function mlq_time_copy(ctx, nfields, reps, t_ms) bind(C, name="mlq_time_copy")
          import :: c_ptr, c_int, c_double

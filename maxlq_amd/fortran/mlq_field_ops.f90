@@ -74,6 +74,21 @@ contains
                      "calculate_spectral_data")
    end subroutine gpu_calculate_spectral_data
 
+   function gpu_calc_global_summed_enstrophy(u) result(res)    ! :2455-2467
+      real(pr), dimension(1:n(1),1:n(2),1:local_N,1:3), intent(in) :: u
+      real(pr) :: res
+      call mlq_sync_params()
+      call mlq_check(mlq_calc_global_summed_enstrophy(mlq_ctx_ptr, u, res), &
+                     "calc_global_summed_enstrophy")
+   end function gpu_calc_global_summed_enstrophy
+
+   function gpu_calc_global_dEnstrophydt(u) result(res)        ! :1417-1481
+      real(pr), dimension(1:n(1),1:n(2),1:local_N,1:3), intent(in) :: u
+      real(pr) :: res
+      call mlq_sync_params()
+      call mlq_check(mlq_calc_global_dEnstrophydt(mlq_ctx_ptr, u, res), "calc_global_dEnstrophydt")
+   end function gpu_calc_global_dEnstrophydt
+
    subroutine gpu_rescaleLqNorm(u, B)                          ! :1699-1709
       real(pr), dimension(1:n(1),1:n(2),1:local_N,1:3), intent(inout) :: u
       real(pr), intent(in) :: B

@@ -181,6 +181,20 @@ class MaxLq:
                                                        _p(out)))
         return out.reshape(2, self.n).T.copy()
 
+    def calc_global_summed_enstrophy(self, u):
+        """function_ops.f90:2455-2467: 1/2 ||curl u||_2^2."""
+        u = _f64(u, self.vshape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_calc_global_summed_enstrophy(self.ctx, _p(u), C.byref(r)))
+        return r.value
+
+    def calc_global_dEnstrophydt(self, u):
+        """function_ops.f90:1417-1481: -nu ||Lap u||^2 + <D((u.grad)u), Lap u>."""
+        u = _f64(u, self.vshape)
+        r = C.c_double()
+        self._chk(self.lib.mlq_calc_global_dEnstrophydt(self.ctx, _p(u), C.byref(r)))
+        return r.value
+
     def spectral_data_field(self, h, normalize=True):
         """The same on a device-resident vector field handle."""
         out = np.zeros(2 * self.n)

@@ -261,6 +261,36 @@ class Oracle:
         return s
 
     # -- diagnostics ------------------------------------------------------------
+    def vel2vort(self, u):
+        """vel2vort, function_ops.f90:2982-3042: REAL(ifft(i K x u_hat))."""
+        g = self.g
+        ux, uy, uz = (fftfwd(u[c]) for c in range(3))                  # :3004-3012
+        wx = 1j * g.K2 * uz - 1j * g.K3 * uy                             # :3017
+        wy = 1j * g.K3 * ux - 1j * g.K1 * uz                             # :3018
+        wz = 1j * g.K1 * uy - 1j * g.K2 * ux                             # :3019
+        return np.stack([fftbwd(wx).real, fftbwd(wy).real, fftbwd(wz).real])   # :3026-3031
+
+    def calc_global_summed_enstrophy(self, u):
+        """calc_global_summed_enstrophy, function_ops.f90:2455-2467, over
+        Enstrophy (:2422-2442): sum_c 1/2 sum W_c^2 dV."""
+        w = self.vel2vort(u)
+        return float(sum(0.5 * float(np.sum(w[c] ** 2)) * self.g.dV for c in range(3)))
+
+    def calc_global_dEnstrophydt(self, u):
+        """calc_global_dEnstrophydt, function_ops.f90:1417-1481."""
+        p = self.p
+        lap = self.laplacian(u)                                           # :1429-1430
+        r1 = -p.viscCoefficient * p.visc * self.global_summed_field_inner_product(lap, lap)  # :1433
+        gradU = [self.gradient(u[i]) for i in range(3)]                  # :1440-1446, [i][j] = d_j u_i
+        adv = np.zeros_like(u)
+        for i in range(3):                                                # :1456-1461
+            for j in range(3):
+                adv[i] = adv[i] + u[j] * gradU[i][j]
+        if p.toDealias:                                                   # :1465-1469
+            adv = np.stack([self.dealias_scalar(adv[i], 2.0) for i in range(3)])
+        r2 = self.global_summed_field_inner_product(adv, lap)             # :1472
+        return r1 + r2
+
     def calculate_spectral_data(self, u, normalize=True):
         """calculate_spectral_data, function_ops.f90:2614-2677: the (n, 2)
         array of shell upper bounds 2 pi (i+1) and shell sums of
@@ -933,4 +963,6 @@ PINS = {
     "kappa_test/kappa_best/kappa_adjust": "tests/test_oracle_pins.py::test_kappa_*",
     "maxdLqdt": "tests/test_oracle_linesearch.py::test_rcg_monotone_and_sphere",
     "calculate_spectral_data": "tests/test_oracle_pins.py::test_spectrum_*",
+    "vel2vort/calc_global_summed_enstrophy": "tests/test_oracle_pins.py::test_enstrophy_*",
+    "calc_global_dEnstrophydt": "tests/test_oracle_pins.py::test_denstrophy_*",
 }

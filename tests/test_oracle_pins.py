@@ -399,3 +399,53 @@ def test_spectrum_shell_edges():
     u = np.stack([0 * X, np.cos(TP * 3 * X), 0 * X])
     sp = Oracle(Params(n=n, q=4.0)).calculate_spectral_data(u, normalize=False)
     assert sp[2, 1] > 0.49 * n ** 6 and sp[3, 1] < 1e-12 * n ** 6
+
+
+# ---- enstrophy and its rate (function_ops.f90:1417-1481, 2422-2467) ---------
+
+def test_enstrophy_closed_forms():
+    """u = (0, sin(2 pi k x), 0): w = (0, 0, 2 pi k cos(2 pi k x)), enstrophy
+    pi^2 k^2; the ABC flow: 2 pi^2 (A^2 + B^2 + C^2)."""
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    orc = Oracle(Params(n=n, q=4.0))
+    for k in (1, 3):
+        u = np.stack([0 * X, np.sin(TP * k * X), 0 * X])
+        w = orc.vel2vort(u)
+        assert np.abs(w[2] - TP * k * np.cos(TP * k * X)).max() < 1e-11 * k
+        assert abs(orc.calc_global_summed_enstrophy(u) - math.pi ** 2 * k ** 2) < 1e-11
+    A, B, C = 1.0, 0.7, 0.4
+    u = abc_field(n, A, B, C)
+    assert abs(orc.calc_global_summed_enstrophy(u) - 2 * math.pi ** 2 * (A * A + B * B + C * C)) < 1e-11
+
+
+def test_denstrophy_single_mode():
+    """u = (0, sin(2 pi k x), 0): (u.grad) u = 0, Lap u = -(2 pi k)^2 u, so
+    dE/dt = -nu (2 pi k)^4 / 2."""
+    n = 16
+    X, Y, Z = grid_xyz(n)
+    nu = 0.7
+    orc = Oracle(Params(n=n, q=4.0, visc=nu))
+    u = np.stack([0 * X, np.sin(TP * 2 * X), 0 * X])
+    assert abs(orc.calc_global_dEnstrophydt(u) + nu * (TP * 2) ** 4 / 2) < 1e-9
+
+
+def test_denstrophy_vortex_stretching_identity():
+    """For divergence-free band-limited u (|m| <= n/3, where the dealias mask
+    does not act on the product's overlap with Lap u):
+    int (u.grad)u . Lap u = int w_i w_j d_j u_i (vortex stretching), computed
+    here from numpy spectral derivatives, and R_1 = -nu ||Lap u||^2 from
+    Parseval."""
+    n = 24
+    orc = Oracle(Params(n=n, q=4.0, visc=1.0))
+    u = bandlimit(orc.divAvg_free(np.random.default_rng(5).standard_normal((3, n, n, n))), n // 3 - 1)
+    K = TP * mode_index(n).astype(float)
+    ks = [K.reshape(1, 1, -1), K.reshape(1, -1, 1), K.reshape(-1, 1, 1)]
+    uh = [np.fft.fftn(u[c]) for c in range(3)]
+    d = [[np.fft.ifftn(1j * ks[j] * uh[i]).real for j in range(3)] for i in range(3)]  # d[i][j] = d_j u_i
+    w = np.stack([d[2][1] - d[1][2], d[0][2] - d[2][0], d[1][0] - d[0][1]])
+    stretch = sum(float(np.sum(w[i] * w[j] * d[i][j])) for i in range(3) for j in range(3)) / n ** 3
+    ksq = ks[0] ** 2 + ks[1] ** 2 + ks[2] ** 2
+    lap2 = sum(float(np.sum(np.abs(ksq * uh[c]) ** 2)) for c in range(3)) / n ** 6
+    got = orc.calc_global_dEnstrophydt(u)
+    assert rel(got, -lap2 + stretch) < 1e-10
