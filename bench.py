@@ -135,6 +135,20 @@ def column_copy_rates(g, n, wbytes, nfields=12, reps=5):
     return out
 
 
+def iteration_traffic(n):
+    """HBM bytes of one mixed RCG iteration (all kernels, FETCH_SIZE +
+    WRITE_SIZE scaled by the copy calibration) from the committed summary of
+    the counter runs of this benchmark; None when absent."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_rcg_iteration_{n}_q4_mixed.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if parts[:3] == ["per", "iteration:", "read"] and "total" in parts:
+            return float(parts[parts.index("total") + 1]) * 1e9
+    return None
+
+
 def cpu_baseline(q, B, threads):
     """Bounded sample of the reference-structured FP64 CPU path (the oracle):
     one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
@@ -433,6 +447,9 @@ def main():
                    "fft_passes": {k: round(v["GBs"], 1) for k, v in fft_passes.items()},
                    "copy_12_fields_GBs": round(copy_gbs, 1),
                    "column_copy_12_spectra_GBs": colcopy,
+                   "iteration_hbm_bytes": iteration_traffic(n),
+                   "iteration_hbm_GBs": (round(iteration_traffic(n) / t_mixed / 1e9, 1)
+                                         if iteration_traffic(n) else None),
                    "transforms_per_iter_mixed": {"forward": gm.tr_per_iter[0],
                                                  "inverse": gm.tr_per_iter[1],
                                                  "reference_c2c": 548 + 94 * nJm}},
