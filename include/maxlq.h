@@ -71,6 +71,8 @@ void mlq_default_params(mlq_params* p, double q);
 
 /* ---- context (replaces init_fft / fft_deallocate, fftwfunction.f90:169-212,
  *      and the grid set-up of initialize, initialize.f90:131-203) ---- */
+/* n1 = n2 = n3 (the reference's RESOL, initialize.f90:141-146), a power of
+ * two from 8 to 1024; other grids return MLQ_EUNSUP before any device call. */
 int mlq_init(mlq_ctx** ctx, int n1, int n2, int n3, const mlq_params* p);
 int mlq_finalize(mlq_ctx* ctx);
 int mlq_set_params(mlq_ctx* ctx, const mlq_params* p);

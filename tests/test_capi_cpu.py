@@ -77,3 +77,19 @@ def test_fortran_dropin_compiles_against_reference_globals(tmp_path):
                             str(tmp_path / (os.path.basename(src) + ".o"))],
                            capture_output=True, text=True, cwd=str(tmp_path))
         assert r.returncode == 0, (src, r.stderr)
+
+
+def test_init_rejects_unsupported_grids():
+    """Non-cubic grids and sizes that are not a power of two in [8, 1024]
+    return MLQ_EUNSUP before any device call."""
+    import ctypes as C
+    from maxlq_amd import capi
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("libmaxlq.so not built (run __graft_entry__.build())")
+    lib = capi.load()
+    p = capi.Params()
+    lib.mlq_default_params(C.byref(p), 4.0)
+    for dims in ((48, 48, 48), (64, 64, 32), (4, 4, 4), (2048, 2048, 2048), (96, 96, 96)):
+        ctx = C.c_void_p()
+        assert lib.mlq_init(C.byref(ctx), *dims, C.byref(p)) == capi.MLQ_EUNSUP, dims
+        assert not ctx.value
