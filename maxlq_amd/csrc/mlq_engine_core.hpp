@@ -334,6 +334,9 @@ void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, int planes)
   }
 }
 
+#ifndef MLQ_XINV_ZB
+#define MLQ_XINV_ZB 1   // band-limited inverse x passes: zero-band first radix pass
+#endif
 // threads per workgroup of the x passes (row pairs of Tf threads each):
 // 256 measured faster than 128 and 512 at 512^3, and equal to 64
 #ifndef MLQ_XPASS_THREADS
@@ -545,7 +548,10 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   xa.scale = (T)(1.0 / ((double)N * N * N));
   const long long pairs = (long long)nb * N * N / 2;
   prof_begin(PC_XINV);
-  if (keep >= 0)
+  if (MLQ_XINV_ZB && keep >= 0 && 8 * keep < 3 * N)   // the first radix pass skips the zero band
+    hipLaunchKernelGGL((k_xinv<T, N, NF, true, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                       st_, xa);
+  else if (keep >= 0)
     hipLaunchKernelGGL((k_xinv<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   else
     hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);

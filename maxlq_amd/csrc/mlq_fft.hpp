@@ -88,6 +88,30 @@ template <typename T, int DIR> __device__ __forceinline__ void dft8(c2<T>* x) {
   x[3] = cadd(e[3], o3);
   x[7] = csub(e[3], o3);
 }
+// dft8 of an input whose x[3] and x[4] are zero (the first pass of an
+// inverse transform of a band-limited input: elements 3N/8 .. 5N/8 - 1 lie
+// outside |m| < 3N/8); the same results as dft8 without the terms in zero
+template <typename T, int DIR> __device__ __forceinline__ void dft8_z34(c2<T>* x) {
+  // even half {x0, x2, 0, x6}
+  const c2<T> es13 = cadd(x[2], x[6]), ed13 = cmul_i<T, DIR>(csub(x[2], x[6]));
+  c2<T> e[4] = {cadd(x[0], es13), cadd(x[0], ed13), csub(x[0], es13), csub(x[0], ed13)};
+  // odd half {x1, 0, x5, x7}
+  const c2<T> os02 = cadd(x[1], x[5]), od02 = csub(x[1], x[5]);
+  const c2<T> od13 = cmul_i<T, DIR>(cmk<T>(-x[7].x, -x[7].y));
+  c2<T> o[4] = {cadd(os02, x[7]), cadd(od02, od13), csub(os02, x[7]), csub(od02, od13)};
+  const T h = (T)0.70710678118654752440084436210485;
+  c2<T> o1 = cmk<T>(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+  c2<T> o2 = cmul_i<T, DIR>(o[2]);
+  c2<T> o3 = cmk<T>(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+  x[0] = cadd(e[0], o[0]);
+  x[4] = csub(e[0], o[0]);
+  x[1] = cadd(e[1], o1);
+  x[5] = csub(e[1], o1);
+  x[2] = cadd(e[2], o2);
+  x[6] = csub(e[2], o2);
+  x[3] = cadd(e[3], o3);
+  x[7] = csub(e[3], o3);
+}
 template <typename T, int R, int DIR> __device__ __forceinline__ void dftR(c2<T>* x) {
   if constexpr (R == 8) dft8<T, DIR>(x);
   else if constexpr (R == 4) dft4<T, DIR>(x);
@@ -135,7 +159,7 @@ template <typename D> __device__ __forceinline__ c2<D> ccvt(double2 a) {
 // strided pass kernels (an FP32 field then holds E complex floats between
 // passes, the same rounding as the LDS exchange, and the kernel stays within
 // 64 VGPRs), FP64 where the caller keeps more registers to spare.
-template <typename RT, int N, int E, int DIR, int P, class L>
+template <typename RT, int N, int E, int DIR, int P, class L, bool ZB = false>
 __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
                                          const L& lds) {
   constexpr int Tf = N / E;
@@ -167,7 +191,8 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
         x[r] = cmul(x[r], w);
       }
     }
-    dftR<double, R, DIR>(x);
+    if constexpr (ZB && P == 0 && R == 8) dft8_z34<double, DIR>(x);   // inputs 3N/8..5N/8-1 zero
+    else dftR<double, R, DIR>(x);
     if constexpr (!last) {
       const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
@@ -185,10 +210,13 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
 
 // Full length-N transform; DIR = -1 forward (exp(-i...)), +1 inverse
 // (unnormalised).  tw[k] = exp(-2 pi i k / N) in FP64.
-template <typename RT, int N, int E, int DIR, class L>
+// ZB: the caller guarantees that elements 3N/8 .. 5N/8 - 1 are zero (an
+// inverse transform of a band-limited input); the first radix-8 pass skips
+// them.
+template <typename RT, int N, int E, int DIR, class L, bool ZB = false>
 __device__ __forceinline__ void fft_core(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
                                          const L& lds) {
-  fft_pass<RT, N, E, DIR, 0, L>(v, t, tw, lds);
+  fft_pass<RT, N, E, DIR, 0, L, ZB>(v, t, tw, lds);
 }
 
 }  // namespace mlq

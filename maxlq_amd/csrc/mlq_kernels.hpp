@@ -332,7 +332,7 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
 // spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
 // inverse FFT, unpack, scale.
-template <typename T, int N, int NF, bool PR>
+template <typename T, int N, int NF, bool PR, bool ZB = false>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -397,7 +397,9 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
     __syncthreads();   // the first FFT pass overwrites the staged rows
   }
   LdsRow<T> L{sm + f * LS};
-  fft_core<double, N, E, +1>(v, t, a.tw, L);
+  // ZB (pruned launches with keep < 3N/8): elements 3N/8 .. 5N/8 - 1 of the
+  // packed row are exact zeros
+  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, a.tw, L);
   T* o = a.rdst[b] + row0 * N;
   const double sc = a.scale;
 #pragma unroll
