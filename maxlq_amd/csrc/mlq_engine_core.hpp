@@ -316,11 +316,22 @@ template <typename T, int N, int V> struct SVar {
   static constexpr int C = ok ? Cr : C0;
 };
 
+#ifndef MLQ_XINV_ZB
+#define MLQ_XINV_ZB 1   // band-limited inverse passes: zero-band first radix pass
+#endif
 template <typename T, int N, int V, int DIR, int MODE>
 void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, int planes) {
   using S = SVar<T, N, V>;
-  hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), dim3((ncols + S::C - 1) / S::C, planes),
-                     dim3(S::C * (N / S::E)), 0, st, sa, ncols);
+  const dim3 grid((ncols + S::C - 1) / S::C, planes), block(S::C * (N / S::E));
+  // inverse passes of a band-limited launch: zero-band first radix pass
+  if constexpr (DIR > 0) {
+    if (MLQ_XINV_ZB && sa.keep >= 0 && 8 * sa.keep < 3 * N) {
+      hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE, true>), grid, block, 0, st, sa,
+                         ncols);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), grid, block, 0, st, sa, ncols);
 }
 // var: bits 0-1 the tile variant, bit 2 the XCD-contiguous tile order
 template <typename T, int N, int DIR, int MODE>
@@ -334,9 +345,6 @@ void launch_strided(int var, hipStream_t st, SArgs<T> sa, int ncols, int planes)
   }
 }
 
-#ifndef MLQ_XINV_ZB
-#define MLQ_XINV_ZB 1   // band-limited inverse x passes: zero-band first radix pass
-#endif
 // threads per workgroup of the x passes (row pairs of Tf threads each):
 // 256 measured faster than 128 and 512 at 512^3, and equal to 64
 #ifndef MLQ_XPASS_THREADS

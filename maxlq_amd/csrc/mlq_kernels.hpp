@@ -422,7 +422,10 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 #ifndef MLQ_ZI_WAVES
 #define MLQ_ZI_WAVES 8
 #endif
-template <typename T, int N, int E, int C, int DIR, int MODE>
+// ZB (inverse passes of a band-limited launch with keep < 3N/8): rows
+// 3N/8 .. 5N/8 - 1 of every column are zero on entry, and the first radix-8
+// pass skips them.
+template <typename T, int N, int E, int C, int DIR, int MODE, bool ZB = false>
 __global__ __launch_bounds__(C*(N / E))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == 1 && E == 8 ? MLQ_ZI_WAVES : 1)))
 void k_strided(SArgs<T> a, int ncols) {
@@ -568,7 +571,7 @@ void k_strided(SArgs<T> a, int ncols) {
     }
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<T, N, E, DIR>(v, t, a.tw, L);
+  fft_core<T, N, E, DIR, LdsCol<T, C>, ZB>(v, t, a.tw, L);
   if (!valid) return;
   if (MODE == 1 && !kept) return;   // the pruned inverse y pass never reads it
   if (MODE == 2) {
