@@ -319,17 +319,18 @@ template <typename T, int N, int V> struct SVar {
 #ifndef MLQ_XINV_ZB
 #define MLQ_XINV_ZB 1   // band-limited inverse passes: zero-band first radix pass
 #endif
+#ifndef MLQ_FWD_ZB
+#define MLQ_FWD_ZB 1    // band-limited forward passes: zero-band last radix pass
+#endif
 template <typename T, int N, int V, int DIR, int MODE>
 void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, int planes) {
   using S = SVar<T, N, V>;
   const dim3 grid((ncols + S::C - 1) / S::C, planes), block(S::C * (N / S::E));
-  // inverse passes of a band-limited launch: zero-band first radix pass
-  if constexpr (DIR > 0) {
-    if (MLQ_XINV_ZB && sa.keep >= 0 && 8 * sa.keep < 3 * N) {
-      hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE, true>), grid, block, 0, st, sa,
-                         ncols);
-      return;
-    }
+  // band-limited launches: zero-band first (inverse) or last (forward) radix pass
+  if ((DIR > 0 ? MLQ_XINV_ZB : MLQ_FWD_ZB) && sa.keep >= 0 && 8 * sa.keep < 3 * N) {
+    hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE, true>), grid, block, 0, st, sa,
+                       ncols);
+    return;
   }
   hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE>), grid, block, 0, st, sa, ncols);
 }
@@ -382,7 +383,10 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   nfwd_ += nb;
   const long long pairs = (long long)nb * N * N / 2;
   prof_begin(PC_XFWD);
-  if (keep >= 0)
+  if (MLQ_FWD_ZB && keep >= 0 && 8 * keep < 3 * N)   // zero-band last radix pass
+    hipLaunchKernelGGL((k_xfwd<T, N, NF, true, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                       st_, xa);
+  else if (keep >= 0)
     hipLaunchKernelGGL((k_xfwd<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   else
     hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
@@ -428,7 +432,10 @@ void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool
     xa.keep = keep;
     const unsigned pairs = (unsigned)((long long)N * N / 2);
     prof_begin(PC_XFWD);
-    if (keep >= 0)
+    if (MLQ_FWD_ZB && keep >= 0 && 8 * keep < 3 * N)   // zero-band last radix pass
+      hipLaunchKernelGGL((k_xfwd_p<T, N, true, P, true>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod,
+                         xa);
+    else if (keep >= 0)
       hipLaunchKernelGGL((k_xfwd_p<T, N, true, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);
     else
       hipLaunchKernelGGL((k_xfwd_p<T, N, false, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);

@@ -191,8 +191,16 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
         x[r] = cmul(x[r], w);
       }
     }
-    if constexpr (ZB && P == 0 && R == 8) dft8_z34<double, DIR>(x);   // inputs 3N/8..5N/8-1 zero
-    else dftR<double, R, DIR>(x);
+    if constexpr (ZB && DIR > 0 && P == 0 && R == 8) {
+      dft8_z34<double, DIR>(x);   // inverse: inputs 3N/8 .. 5N/8 - 1 are zero
+    } else {
+      dftR<double, R, DIR>(x);
+    }
+    if constexpr (ZB && DIR < 0 && last && R == 8) {
+      // forward: outputs 3N/8 .. 5N/8 - 1 are never stored (their adds drop out)
+      x[3] = cmk<double>(0.0, 0.0);
+      x[4] = cmk<double>(0.0, 0.0);
+    }
     if constexpr (!last) {
       const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
@@ -204,15 +212,16 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
   }
   if constexpr (!last) {
     __syncthreads();
-    fft_pass<RT, N, E, DIR, P + 1, L>(v, t, tw, lds);
+    fft_pass<RT, N, E, DIR, P + 1, L, ZB>(v, t, tw, lds);
   }
 }
 
 // Full length-N transform; DIR = -1 forward (exp(-i...)), +1 inverse
 // (unnormalised).  tw[k] = exp(-2 pi i k / N) in FP64.
-// ZB: the caller guarantees that elements 3N/8 .. 5N/8 - 1 are zero (an
-// inverse transform of a band-limited input); the first radix-8 pass skips
-// them.
+// ZB, for a band-limited launch with keep < 3N/8: an inverse transform's
+// elements 3N/8 .. 5N/8 - 1 are zero on entry (the first radix-8 pass skips
+// them); a forward transform's outputs there are never stored (the last pass
+// leaves them zero).
 template <typename RT, int N, int E, int DIR, class L, bool ZB = false>
 __device__ __forceinline__ void fft_core(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
                                          const L& lds) {

@@ -248,12 +248,14 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
 // Transform of the packed pair in v (row view L of the thread's row pair) and
 // the split into the half spectra at d (row0) and d + PX (row0 + 1); only
 // kx <= kmax is stored.
-template <typename T, int N>
+template <typename T, int N, bool ZB = false>
 __device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t,
                                           const double2* __restrict__ tw, const LdsRow<T>& L,
                                           c2<T>* d, int kmax) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
-  fft_core<double, N, E, -1>(v, t, tw, L);
+  // ZB (kmax < 3N/8): entries 3N/8 .. 5N/8 - 1 of the packed row are never
+  // read by the split below
+  fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, tw, L);
 #pragma unroll
   for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
   __syncthreads();
@@ -272,7 +274,7 @@ __device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t
   }
 }
 
-template <typename T, int N, int NF, bool PR>
+template <typename T, int N, int NF, bool PR, bool ZB = false>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
     v[m] = cmk<double>((double)gld<nt_ld<PC_XF>()>(s + t + m * Tf), (double)gld<nt_ld<PC_XF>()>(s + N + t + m * Tf));
   LdsRow<T> L{sm + f * LP};
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
-  xfwd_pair<T, N>(v, t, a.tw, L, a.spec + b * a.sstride + row0 * PX, kmax);
+  xfwd_pair<T, N, ZB>(v, t, a.tw, L, a.spec + b * a.sstride + row0 * PX, kmax);
 }
 
 // Forward x pass of NB fields produced in the kernel (one row pair per
@@ -303,7 +305,7 @@ constexpr bool xfwd_p_fits() {
   return (long long)P::NB * LdsRow<T>::pitch(N) * (long long)sizeof(c2<T>) <= 160 * 1024 &&
          P::NB * FftShape<N>::Tf <= 1024;
 }
-template <typename T, int N, bool PR, class P>
+template <typename T, int N, bool PR, class P, bool ZB = false>
 __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs<T> a) {
   constexpr int NB = P::NB, E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
   __syncthreads();   // the transform rewrites the staged rows
   LdsRow<T> L{sm + f * LP};
   const int kmax = PR ? a.keep : N / 2;
-  xfwd_pair<T, N>(v, t, a.tw, L, a.spec + f * a.sstride + row0 * PX, kmax);
+  xfwd_pair<T, N, ZB>(v, t, a.tw, L, a.spec + f * a.sstride + row0 * PX, kmax);
 }
 
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
@@ -422,9 +424,9 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 #ifndef MLQ_ZI_WAVES
 #define MLQ_ZI_WAVES 8
 #endif
-// ZB (inverse passes of a band-limited launch with keep < 3N/8): rows
-// 3N/8 .. 5N/8 - 1 of every column are zero on entry, and the first radix-8
-// pass skips them.
+// ZB (band-limited launches with keep < 3N/8): an inverse pass's rows
+// 3N/8 .. 5N/8 - 1 are zero on entry and the first radix-8 pass skips them; a
+// forward pass never stores those rows (the last pass leaves them zero).
 template <typename T, int N, int E, int C, int DIR, int MODE, bool ZB = false>
 __global__ __launch_bounds__(C*(N / E))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == 1 && E == 8 ? MLQ_ZI_WAVES : 1)))
