@@ -418,9 +418,10 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
 // spectral factor (prologue), writing scratch.  MODE 2: forward z pass
 // reading scratch, writing factor * result to dst (epilogue).
 // ---------------------------------------------------------------------------
-// FP32 inverse z pass at E = 8: compiled for 8 waves per SIMD (64 VGPRs, a
-// 12-byte spill) instead of 72 VGPRs, which at 16 columns (1024 threads) let
-// only one workgroup per CU run; measured 0.4 ms faster per mixed iteration
+// FP32 inverse z pass at E = 8: compiled for 8 waves per SIMD (at most 64
+// VGPRs; the full-transform variant spills 12 bytes, the band-limited one
+// fits) instead of 72 VGPRs, which at 16 columns (1024 threads) let only one
+// workgroup per CU run; measured 0.4 ms faster per mixed iteration
 #ifndef MLQ_ZI_WAVES
 #define MLQ_ZI_WAVES 8
 #endif
