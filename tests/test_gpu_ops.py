@@ -240,3 +240,28 @@ def test_time_column_copy():
             assert by == 2 * 2 * 2 * 8 * (n // 2 + 1) * n * n
     with pytest.raises(RuntimeError):
         g.time_column_copy(3, 0)
+
+
+@pytest.mark.gpu
+def test_nan_reports_enan():
+    """A NaN in the field: fftfwd stops with MLQ_ENAN (fftwfunction.f90:78-89),
+    and so do R(u) (function_ops.f90:1843-1854 for e_u; the RCG's J) and the
+    solve; a finite field afterwards works again."""
+    from maxlq_amd.capi import MlqError, MLQ_ENAN
+    n = 16
+    g = _ctx(n)
+    u = np.random.default_rng(3).standard_normal((3, n, n, n))
+    bad = u.copy()
+    bad[1, 2, 3, 4] = np.nan
+    with pytest.raises(MlqError) as e:
+        g.fftfwd(bad[0] + 1j * bad[1])
+    assert e.value.code == MLQ_ENAN
+    with pytest.raises(MlqError) as e:
+        g.calc_global_dLqdt_inclParts(bad)
+    assert e.value.code == MLQ_ENAN
+    h = g.field(3)
+    g.upload(h, bad)
+    with pytest.raises(MlqError) as e:
+        g.maxdLqdt(h, max_iter=1, optim_tol=0.0)
+    assert e.value.code == MLQ_ENAN
+    assert np.isfinite(g.calc_global_dLqdt_inclParts(u)).all()
