@@ -14,7 +14,7 @@ program mlq_fsmoke
    type(mlq_params) :: p
    type(mlq_rcg_cfg) :: cfg
    type(mlq_rcg_result) :: res
-   real(c_double) :: nrm, r(3), lqq, x, y, z, c
+   real(c_double) :: nrm, r(3), lqq, x, y, z, c, ens
    integer(c_int) :: rc, h
    integer :: i, j, k, fails
 
@@ -77,6 +77,14 @@ program mlq_fsmoke
    rc = mlq_rcg_solve(ctx, h, cfg, res, c_null_funptr, c_null_ptr)
    if (rc /= MLQ_OK .or. res%iters < 1) then
       print*, "rcg solve rc", rc, "iters", res%iters
+      fails = fails + 1
+   end if
+   ! the result's enstrophy equals calc_global_summed_enstrophy of the result
+   rc = mlq_download(ctx, h, u)
+   rc = mlq_calc_global_summed_enstrophy(ctx, u, ens)
+   if (rc /= MLQ_OK .or. .not. (res%enstrophy > 0.0_c_double) .or. &
+       abs(res%enstrophy - ens) > 1.0e-10_c_double * ens) then
+      print*, "enstrophy", rc, res%enstrophy, ens
       fails = fails + 1
    end if
    rc = mlq_finalize(ctx)
