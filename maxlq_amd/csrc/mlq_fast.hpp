@@ -1031,8 +1031,12 @@ void Engine<T>::time_linesearch(int u, int d, const double* taus, int ntau, int 
 template <typename T, int N, int V>
 void launch_zls_v(hipStream_t st, const LsArgs<T>& la, int ncols, long long es, int nb) {
   using S = SVar<T, N, V>;
-  hipLaunchKernelGGL((k_zls<T, N, S::E, S::C>), dim3((ncols + S::C - 1) / S::C, nb),
-                     dim3(S::C * (N / S::E)), 0, st, la, ncols, es);
+  if (MLQ_FWD_ZB && la.keep >= 0 && 8 * la.keep < 3 * N)   // zero-band last radix pass
+    hipLaunchKernelGGL((k_zls<T, N, S::E, S::C, true>), dim3((ncols + S::C - 1) / S::C, nb),
+                       dim3(S::C * (N / S::E)), 0, st, la, ncols, es);
+  else
+    hipLaunchKernelGGL((k_zls<T, N, S::E, S::C>), dim3((ncols + S::C - 1) / S::C, nb),
+                       dim3(S::C * (N / S::E)), 0, st, la, ncols, es);
 }
 
 template <typename T> template <int N>
@@ -1069,8 +1073,12 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   la.remap = (var >> 2) & 1;
   la.part = lspart_;
   prof_begin(PC_XFWD);
-  hipLaunchKernelGGL((k_xfwd_ls<T, N, NF>), dim3((unsigned)((long long)N * N / 2 / NF)),
-                     dim3(NF * Tf), 0, st_, la);
+  if (MLQ_FWD_ZB && keep >= 0 && 8 * keep < 3 * N)   // zero-band last radix pass
+    hipLaunchKernelGGL((k_xfwd_ls<T, N, NF, true>), dim3((unsigned)((long long)N * N / 2 / NF)),
+                       dim3(NF * Tf), 0, st_, la);
+  else
+    hipLaunchKernelGGL((k_xfwd_ls<T, N, NF>), dim3((unsigned)((long long)N * N / 2 / NF)),
+                       dim3(NF * Tf), 0, st_, la);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_XFWD, 3.0 * real_bytes() + nb * spec_bytes());
   SArgs<T> sa{};

@@ -864,7 +864,8 @@ template <typename T> __device__ __forceinline__ T ls_gg(T s0, T s1, T s2, T t) 
   return gv * gv;
 }
 
-template <typename T, int N, int NF>
+// ZB (keep < 3N/8): the transform's entries 3N/8 .. 5N/8 - 1 are never read
+template <typename T, int N, int NF, bool ZB = false>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
@@ -884,7 +885,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
       v[m] = cmk<double>((double)ls_gg(r0[i], r1[i], r2[i], tb),
                          (double)ls_gg(r0[N + i], r1[N + i], r2[N + i], tb));
     }
-    fft_core<double, N, E, -1>(v, t, a.tw, L);
+    fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, a.tw, L);
 #pragma unroll
     for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
     __syncthreads();
@@ -901,7 +902,8 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
   }
 }
 
-template <typename T, int N, int E, int C>
+// ZB (keep < 3N/8): rows 3N/8 .. 5N/8 - 1 enter the sums with weight zero
+template <typename T, int N, int E, int C, bool ZB = false>
 __global__ __launch_bounds__(C*(N / E)) void k_zls(LsArgs<T> a, int ncols, long long estride) {
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   constexpr int NW = C * Tf / 64;
@@ -928,7 +930,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_zls(LsArgs<T> a, int ncols, long 
     for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<T, N, E, -1>(v, t, a.tw, L);
+  fft_core<T, N, E, -1, LdsCol<T, C>, ZB>(v, t, a.tw, L);
   double s0 = 0.0, s1 = 0.0;
   if (valid) {
     const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
