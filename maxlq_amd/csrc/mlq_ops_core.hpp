@@ -425,8 +425,9 @@ void Engine<T>::spectral_data_dev(const T* u, bool normalize, double* out) {
     tot += h[i];
   }
   if (normalize) {   // :2667-2671
-    double l2 = 0.0;
-    for (int c = 0; c < 3; ++c) l2 += dot(src[c], src[c]) / (double)nr_;
+    double l2 = 0.0;   // sum_c ||u_c||^2 / nr: one reduction over the 3 contiguous components
+    reduce<1>(R_dot<T>{u, u}, 3 * nr_, &l2);
+    l2 /= (double)nr_;
     const double spectral_ener = 2.0 * (4.0 * std::atan2(1.0, 1.0)) * tot;
     for (int i = 0; i < n; ++i) out[n + i] = l2 / spectral_ener * out[n + i];
   }
