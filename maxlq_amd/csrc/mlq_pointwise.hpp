@@ -197,13 +197,6 @@ template <typename T> struct R_sum {
   const T* a;
   __device__ void operator()(long long i, double (&acc)[1]) const { acc[0] += (double)a[i]; }
 };
-template <typename T> struct R_dot3v {  // sum over 3 components of a.b
-  const T* a; const T* b; long long nr;
-  __device__ void operator()(long long i, double (&acc)[1]) const {
-    acc[0] += (double)a[i] * (double)b[i] + (double)a[nr + i] * (double)b[nr + i] +
-              (double)a[2 * nr + i] * (double)b[2 * nr + i];
-  }
-};
 template <typename T> struct R_nan {
   const T* a;
   __device__ void operator()(long long i, double (&acc)[1]) const { if (a[i] != a[i]) acc[0] += 1.0; }
