@@ -423,7 +423,7 @@ int mlq_prof_get(mlq_ctx* ctx, int cls, long long* count, double* ms, double* by
 const char* mlq_prof_name(int cls) {
   static const char* names[PC_COUNT] = {"fft_x_fwd", "fft_y_fwd", "fft_z_fwd", "fft_z_inv",
                                         "fft_y_inv", "fft_x_inv", "pointwise", "spectral",
-                                        "reduce", "linesearch_pointwise"};
+                                        "reduce", "linesearch_pointwise", "fft_z_roundtrip"};
   return (cls >= 0 && cls < PC_COUNT) ? names[cls] : "?";
 }
 int mlq_mem_info(mlq_ctx* ctx, double* used, double* peak, double* dfree, double* dtotal) {

@@ -20,7 +20,7 @@ struct DevField {
 
 enum ProfClass : int {
   PC_XFWD = 0, PC_YFWD, PC_ZFWD, PC_ZINV, PC_YINV, PC_XINV,
-  PC_POINTWISE, PC_SPECTRAL, PC_REDUCE, PC_LS_POINT, PC_COUNT
+  PC_POINTWISE, PC_SPECTRAL, PC_REDUCE, PC_LS_POINT, PC_ZRT, PC_COUNT
 };
 
 struct LogSink {

@@ -574,6 +574,197 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   prof_end(PC_XINV, rb + nb * sb);
 }
 
+// Slot plan of a round-trip group whose inputs sit in scratch slots
+// 0..nin-1 (outputs' `in` index those slots).  z transforms: one per distinct
+// (input, factor without its i K_x / i K_y part), in place when it is its
+// input's only z transform, else into a fresh slot (the input slot is free
+// after the z launch).  y transforms: one per (z transform, i K_y or not),
+// the first in place, the others into free or fresh slots.  The i K_x part
+// is applied in the inverse x pass.  Returns the slots needed.
+template <typename T>
+int Engine<T>::rtrip_plan(int nin, int nout, const RtOut* outs, RtPlan& pl) const {
+  int zof[MAXB];
+  pl.nz = 0;
+  for (int o = 0; o < nout; ++o) {
+    SpecOp z = outs[o].op;
+    pl.oxik[o] = 0;
+    int yik = 0;
+    if ((z.flags & SP_IK) && z.dim < 2) {
+      (z.dim == 0 ? pl.oxik[o] : yik) = 1;
+      z.flags &= ~SP_IK;
+      z.dim = 0;
+    }
+    int j = 0;
+    for (; j < pl.nz; ++j) {
+      const SpecOp& p = pl.zop[j];
+      if (pl.zin[j] == outs[o].in && p.flags == z.flags && p.scale == z.scale &&
+          (!(p.flags & SP_MASK) || p.mcut == z.mcut) && (!(p.flags & SP_IK) || p.dim == z.dim) &&
+          (!(p.flags & SP_HSCALE) || p.hs == z.hs))
+        break;
+    }
+    if (j == pl.nz) {
+      if (pl.nz == MAXB) return 1 << 20;
+      pl.zin[j] = outs[o].in;
+      pl.zop[j] = z;
+      ++pl.nz;
+    }
+    zof[o] = j;
+    pl.oy[o] = yik;   // y key, resolved below
+  }
+  int next = nin;
+  int freel[MAXB], nfree = 0;
+  for (int i = 0; i < nin; ++i) {
+    int cnt = 0, last = -1;
+    for (int j = 0; j < pl.nz; ++j)
+      if (pl.zin[j] == i) { ++cnt; last = j; }
+    if (cnt == 1) {
+      pl.zout[last] = i;
+    } else {
+      for (int j = 0; j < pl.nz; ++j)
+        if (pl.zin[j] == i) pl.zout[j] = next++;
+      if (cnt > 1) freel[nfree++] = i;
+    }
+  }
+  pl.ny = 0;
+  for (int o = 0; o < nout; ++o) {
+    const int k = pl.oy[o], z = zof[o];
+    int p = 0;
+    while (p < pl.ny && !(pl.yz[p] == z && pl.yk[p] == k)) ++p;
+    if (p == pl.ny) {
+      bool first = true;
+      for (int r = 0; r < pl.ny; ++r) first = first && pl.yz[r] != z;
+      pl.yz[p] = z;
+      pl.yk[p] = k;
+      pl.yslot[p] = first ? pl.zout[z] : (nfree > 0 ? freel[--nfree] : next++);
+      ++pl.ny;
+    }
+    pl.oy[o] = p;
+  }
+  pl.need = next;
+  return next;
+}
+
+template <typename T> template <int N>
+void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep) {
+  constexpr int NF = xpass_NF<N>();
+  constexpr int Tf = FftShape<N>::Tf;
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  RtPlan pl;
+  rtrip_plan(nin, nout, outs, pl);
+  const double sb = (double)spec_bytes();
+  // forward x and y passes of the inputs into slots 0..nin-1
+  {
+    XArgs<T> xa{};
+    for (int b = 0; b < nin; ++b) xa.rsrc[b] = src[b];
+    xa.spec = scratch_;
+    xa.tw = tw_;
+    xa.sstride = ns_;
+    xa.nb = nin;
+    xa.scale = (T)1;
+    xa.keep = keep;
+    const long long pairs = (long long)nin * N * N / 2;
+    prof_begin(PC_XFWD);
+    if (MLQ_FWD_ZB && keep >= 0 && 8 * keep < 3 * N)
+      hipLaunchKernelGGL((k_xfwd<T, N, NF, true, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf),
+                         0, st_, xa);
+    else if (keep >= 0)
+      hipLaunchKernelGGL((k_xfwd<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                         st_, xa);
+    else
+      hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                         st_, xa);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_XFWD, nin * (real_bytes() + sb));
+    SArgs<T> sa{};
+    sa.spec = scratch_;
+    sa.tw = tw_;
+    sa.sstride = ns_;
+    sa.keep = keep;
+    prof_begin(PC_YFWD);
+    launch_strided<T, N, -1, 0>(svar_[keep >= 0 ? 4 : 0], st_, sa, keep >= 0 ? keep + 1 : PX,
+                                N * nin);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_YFWD, 2 * nin * sb);
+  }
+  nfwd_ += nin;
+  // fused z round trips
+  {
+    ZArgs<T> za{};
+    za.spec = scratch_;
+    za.tw = tw_;
+    za.sstride = ns_;
+    za.keep = keep;
+    for (int j = 0; j < pl.nz; ++j) {
+      za.zin[j] = pl.zin[j];
+      za.zout[j] = pl.zout[j];
+      za.op[j] = pl.zop[j];
+    }
+    constexpr int E = FftShape<N>::E, CZ = zrt_C<N>();
+    const int ncols = N * PP;
+    prof_begin(PC_ZRT);
+    hipLaunchKernelGGL((k_zrt<T, N, E, CZ>), dim3((ncols + CZ - 1) / CZ, pl.nz), dim3(CZ * (N / E)),
+                       0, st_, za, ncols);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_ZRT, 2 * pl.nz * sb);
+  }
+  // inverse y passes (out of place first: they read slots the in-place ones rewrite)
+  {
+    SArgs<T> ya{};
+    ya.spec = scratch_;
+    ya.tw = tw_;
+    ya.sstride = ns_;
+    ya.keep = keep;
+    ya.ymap = 1;
+    prof_begin(PC_YINV);
+    for (int pass = 0; pass < 2; ++pass) {
+      int nf = 0;
+      for (int p = 0; p < pl.ny; ++p) {
+        const int zs = pl.zout[pl.yz[p]];
+        if ((pl.yslot[p] == zs) != (pass == 1)) continue;
+        ya.yin[nf] = zs;
+        ya.yout[nf] = pl.yslot[p];
+        ya.yik[nf] = pl.yk[p];
+        ++nf;
+      }
+      if (nf == 0) continue;
+      launch_strided<T, N, 1, 0>(svar_[keep >= 0 ? 7 : 3], st_, ya, keep >= 0 ? keep + 1 : PX,
+                                 N * nf);
+      MLQ_HIP(hipGetLastError());
+    }
+    prof_end(PC_YINV, 2 * pl.ny * sb);
+  }
+  // inverse x pass (1/N^3 folded in)
+  {
+    XArgs<T> xa{};
+    for (int o = 0; o < nout; ++o) {
+      xa.rdst[o] = outs[o].dst;
+      xa.xin[o] = pl.yslot[pl.oy[o]];
+      xa.xik[o] = pl.oxik[o];
+    }
+    xa.xmap = 1;
+    xa.spec = scratch_;
+    xa.tw = tw_;
+    xa.sstride = ns_;
+    xa.nb = nout;
+    xa.keep = keep;
+    xa.scale = (T)(1.0 / ((double)N * N * N));
+    const long long pairs = (long long)nout * N * N / 2;
+    prof_begin(PC_XINV);
+    if (MLQ_XINV_ZB && keep >= 0 && 8 * keep < 3 * N)
+      hipLaunchKernelGGL((k_xinv<T, N, NF, true, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf),
+                         0, st_, xa);
+    else if (keep >= 0)
+      hipLaunchKernelGGL((k_xinv<T, N, NF, true>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                         st_, xa);
+    else
+      hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
+                         st_, xa);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_XINV, nout * (real_bytes() + sb));
+  }
+  ninv_ += nout;
+}
+
 #define MLQ_DISPATCH_N(FN, ...)                                              \
   switch (n_) {                                                              \
     case 8: FN<8>(__VA_ARGS__); break;                                       \
@@ -618,23 +809,55 @@ void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst,
 template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double order) {
   const int mc = mcut_of(order);
   if (mask_is_identity(mc)) return;   // identity mask: the round trip only adds round-off
-  // at most 3 spectra live at a time (bounded footprint at 1024^3)
-  if (nb > 3) {
-    for (int b0 = 0; b0 < nb; b0 += 3) dealias_dev(f + b0, std::min(3, nb - b0), order);
-    return;
+  std::vector<RtOut> outs(nb);
+  for (int b = 0; b < nb; ++b) outs[b] = RtOut{b, sp_mask(mc), f[b]};
+  rtrip(nb, f, nb, outs.data());
+}
+
+// Round trips in launch groups that fit the transform scratch: inputs are
+// added to a group while its slot plan fits; an input whose outputs alone do
+// not fit runs one output at a time.  A group's outputs are written by its
+// last pass, after every read of its inputs: an output may alias an input of
+// its own group (dealias in place), never an input of a later group.
+template <typename T>
+void Engine<T>::rtrip(int nin, const T* const* src, int nout, const RtOut* outs) {
+  int keep = -1;
+  for (int o = 0; o < nout; ++o) {
+    const SpecOp& op = outs[o].op;
+    if (!(op.flags & SP_MASK) || op.mcut >= n_ / 2) { keep = -1; break; }
+    keep = std::max(keep, op.mcut);
   }
-  std::vector<ST> sp;
-  std::vector<C*> dp;
-  std::vector<SpecOp> ops(nb, sp_mask(mc));
-  std::vector<InvSrc> is(nb);
-  std::vector<SpecOp> none(nb, sp_none());
-  for (int b = 0; b < nb; ++b) {
-    sp.push_back(stmp());
-    dp.push_back(sp.back().get());
-    is[b] = InvSrc{{dp[b], nullptr, nullptr}, {1.0, 0, 0}, 1};
+  for (int o = 0; o < nout; ++o)
+    if (outs[o].in < 0 || outs[o].in >= nin) throw Error(MLQ_EINVAL, "rtrip: bad input index");
+  int i0 = 0;
+  while (i0 < nin) {
+    // grow the group [i0, i1) while it fits
+    int i1 = i0;
+    std::vector<RtOut> g;
+    while (i1 < nin) {
+      std::vector<RtOut> t = g;
+      for (int o = 0; o < nout; ++o)
+        if (outs[o].in == i1) { RtOut r = outs[o]; r.in = i1 - i0; t.push_back(r); }
+      RtPlan pl;
+      if ((int)t.size() > MAXB || rtrip_plan(i1 - i0 + 1, (int)t.size(), t.data(), pl) > sb_) break;
+      g = std::move(t);
+      ++i1;
+    }
+    if (i1 == i0) {
+      // one input whose outputs do not fit together: one output per launch
+      for (int o = 0; o < nout; ++o) {
+        if (outs[o].in != i0) continue;
+        RtOut r = outs[o];
+        r.in = 0;
+        const T* s = src[i0];
+        MLQ_DISPATCH_N(rtrip_n, 1, &s, 1, &r, keep);
+      }
+      i0 += 1;
+      continue;
+    }
+    if (!g.empty()) MLQ_DISPATCH_N(rtrip_n, i1 - i0, src + i0, (int)g.size(), g.data(), keep);
+    i0 = i1;
   }
-  fwd(nb, f, dp.data(), ops.data());
-  inv(nb, is.data(), none.data(), f, mc);
 }
 
 template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms) {

@@ -177,6 +177,23 @@ template <typename T> class Engine final : public IEngine {
     inv(1, &s, &op, &dst);
   }
   void dealias_dev(T* const* f, int nb, double order);  // in place, batch
+  // Round trips dst = REAL(IFFT(op * FFT(src[in]))) of physical fields, nout
+  // outputs of nin inputs; the z transforms are fused (k_zrt), so no
+  // spectrum is stored.  An output may overwrite its own input only when it
+  // is that input's only output (dealias in place).
+  struct RtOut { int in; SpecOp op; T* dst; };
+  void rtrip(int nin, const T* const* src, int nout, const RtOut* outs);
+  template <int N>
+  void rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep);
+  // slot plan of one rtrip launch group; returns the scratch slots it needs
+  struct RtPlan {
+    int nz = 0, zin[MAXB], zout[MAXB];
+    SpecOp zop[MAXB];
+    int ny = 0, yz[MAXB], yk[MAXB], yslot[MAXB];
+    int oy[MAXB], oxik[MAXB];
+    int need = 0;
+  };
+  int rtrip_plan(int nin, int nout, const RtOut* outs, RtPlan& pl) const;
   // fused q = 4 line-search sums for a batch of steps (k_xfwd_ls, pruned y
   // pass, k_zls): out[2b] = sum_kept m |A_b|^2, out[2b+1] = sum_kept m
   // Re(conj(A_b) Q(tau_b)); requires 0 <= keep < n/2 and nb <= MAXB
@@ -203,6 +220,7 @@ template <typename T> class Engine final : public IEngine {
   RT chain_take(Chain& ch, double k);   // |u|^k, ownership moved out of the chain
   void spec3(const T* u, C* const* uh);                      // 3 forward transforms
   void grad9(C* const* uh, T* const* G);                     // G[i*3+j] = d_j u_i
+  void grad9_rt(const T* u, T* G);                           // the same by round trips
   void dLqdt_dev(const T* u, double q, double out[3]);
   void gradL2_dev(const T* u, T* const* V);
   double lq_norm_dev(const T* u);

@@ -104,7 +104,7 @@ template <typename T> class FastRcg {
       hk_ = 1.0;
     }
     G_ = e_.rtmp(9);
-    grad9(uh_, G_);
+    e_.grad9_rt(u_, G_);   // from u itself: no rounded spectrum in between
     s_ = e_.rtmp();
     e_.map(F_dot3self<T>{u_, nr_, s_}, nr_);
     if (poly_ok_) {

@@ -628,6 +628,74 @@ void k_strided(SArgs<T> a, int ncols) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused z round trip: REAL(IFFT(op * FFT(f))) along z for the columns that the
+// forward x and y passes left in the scratch spectra (kx, ky transformed, z
+// physical).  Each workgroup loads C adjacent columns, runs the forward z
+// transform, multiplies by the column's spectral factor, runs the inverse z
+// transform and stores the result for the inverse y and x passes.  The
+// spectrum of the round trip never leaves the workgroup: every butterfly,
+// twiddle product and LDS exchange of both transforms is FP64 and the column
+// is rounded to the field precision once, on its store.  In FP32 this removes
+// the rounding of fully transformed coefficients, whose errors are coherent
+// across the field: at 128^3, q = 6, it accounts for 1e-7 .. 2e-7 of the
+// relative error of R(u), against 1e-8 .. 2e-8 for the rounding after the x
+// and y passes (FP32-storage emulation of the oracle).
+// Plane p of the launch reads scratch slot zin[p] and writes zout[p] (equal
+// only when no other plane reads that slot).  With keep >= 0 every op masks
+// to the cube max_j |m_j| <= keep: columns outside it are never read by the
+// pruned inverse y pass and are skipped.
+// ---------------------------------------------------------------------------
+template <typename T> struct ZArgs {
+  c2<T>* spec;        // scratch spectra, stride sstride
+  const double2* tw;
+  long long sstride;
+  int keep = -1;
+  int remap = 0;
+  int zin[MAXB];
+  int zout[MAXB];
+  SpecOp op[MAXB];    // factors along z only (col_factor_ok)
+};
+// columns per workgroup: 16 (128-byte FP32 segments) while the FP64 column
+// tile N x C fits the 160 KB of LDS, else 8
+template <int N> constexpr int zrt_C() { return (long long)N * 16 * 16 <= 160 * 1024 ? 16 : 8; }
+
+template <typename T, int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
+  constexpr int Tf = N / E, PP = spec_pitch<T>(N);
+  constexpr int estride = N * PP;
+  __shared__ double2 sm[N * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  unsigned bx, by;
+  xcd_tile(a.remap != 0, bx, by);
+  const int col = (int)bx * C + c;
+  const int plane = (int)by;
+  const int ky = col / PP, kx = col - ky * PP;
+  const bool valid = col < ncols && kx <= N / 2;
+  const int keep = a.keep;
+  const bool ok = valid && (keep < 0 || (kx <= keep && mabs(ky, N) <= keep));
+  if (!__syncthreads_or(ok)) return;   // block-uniform: no column of the tile is read later
+  constexpr bool NTL = nt_ld<PC_ZI>(), NTS = nt_st<PC_ZI>();
+  const c2<T>* s = a.spec + a.zin[plane] * a.sstride + col;
+  c2<double> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    v[m] = ok ? ccvt<double>(gld<NTL>(s + (t + m * Tf) * estride)) : cmk<double>(0.0, 0.0);
+  LdsCol<double, C> L{reinterpret_cast<c2<double>*>(sm), c};
+  fft_core<double, N, E, -1, LdsCol<double, C>>(v, t, a.tw, L);
+  const SpecOp op = a.op[plane];
+  const ColFactor cf = col_factor_init(op, op.scale, kx, ky, N);
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmul(v[m], col_factor(cf, t + m * Tf, N));
+  // the last forward pass read its LDS rows before its barrier: the first
+  // inverse pass may overwrite them
+  fft_core<double, N, E, +1, LdsCol<double, C>>(v, t, a.tw, L);
+  if (!ok) return;
+  c2<T>* d = a.spec + a.zout[plane] * a.sstride + col;
+#pragma unroll
+  for (int m = 0; m < E; ++m) gst<NTS>(d + (t + m * Tf) * estride, cmk<T>((T)v[m].x, (T)v[m].y));
+}
+
 // Column copy with the memory access pattern of a strided pass and none of
 // its arithmetic: C adjacent columns per workgroup, E elements per thread at
 // the pass's element stride (AXIS 1: along y inside one z plane; AXIS 2:

@@ -83,6 +83,16 @@ template <typename T> void Engine<T>::grad9(C* const* uh, T* const* G) {
   inv(9, is, ops, G);
 }
 
+// G[i*3+j] = d_j u_i of a physical vector field by round trips (no stored
+// spectrum: d_j u_i = REAL(IFFT(i K_j FFT(u_i))) with one rounding per pass)
+template <typename T> void Engine<T>::grad9_rt(const T* u, T* G) {
+  const T* src[3] = {u, u + nr_, u + 2 * nr_};
+  RtOut o[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o[i * 3 + j] = RtOut{i, sp_ik(j), G + (i * 3 + j) * nr_};
+  rtrip(3, src, 9, o);
+}
+
 template <typename T> double Engine<T>::lq_norm_dev(const T* u) {
   Chain ch;
   chain_init(ch, u);
@@ -95,14 +105,9 @@ template <typename T> void Engine<T>::dLqdt_dev(const T* u, double q, double out
   const double nu = p_.visc, vc = p_.viscCoefficient, pc = p_.pressureCoefficient;
   const double dV = 1.0 / (double)nr_;
   const bool D = p_.toDealias != 0;
-  ST uh[3] = {stmp(), stmp(), stmp()};
-  C* uhp[3] = {uh[0], uh[1], uh[2]};
-  spec3(u, uhp);
   RT Gb = rtmp(9);                                               // G[i*3+j] = d_j u_i
   T* Gp = Gb.get();
-  T* G[9];
-  for (int a = 0; a < 9; ++a) G[a] = Gp + a * nr_;
-  grad9(uhp, G);
+  grad9_rt(u, Gp);
   RT s = rtmp();
   map(F_dot3self<T>{u, nr_, s}, nr_);
   if (D) { T* f = s; dealias_dev(&f, 1, 2.0); }
@@ -132,16 +137,16 @@ template <typename T> void Engine<T>::dLqdt_dev(const T* u, double q, double out
   // pressure: p = solve_poisson(-D(grad u : grad u^T))            :1339-1340
   RT Z = rtmp();
   map(F_gradgradT<T>{Gp, nr_, Z}, nr_);
-  ST ph = stmp();
+  RT pr = rtmp();
   {
     SpecOp op;
     op.flags = (D ? SP_MASK : 0) | SP_INVKSQ;
     op.mcut = mcut_of(2.0);
     op.scale = -1.0;
-    fwd1(Z, ph, op);
+    const T* z = Z;
+    const RtOut o{0, op, pr};
+    rtrip(1, &z, 1, &o);
   }
-  RT pr = rtmp();
-  inv1(ph, pr);
   RT P2 = rtmp();
   map(F_mul<T>{pr, Bq, P2}, nr_);                                 // :1343/:1349
   if (D) { T* f = P2; dealias_dev(&f, 1, 2.0); }

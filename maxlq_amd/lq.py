@@ -384,7 +384,7 @@ class MaxLq:
 
     def prof(self):
         out = {}
-        for cls in range(10):
+        for cls in range(11):
             cnt, ms, by = C.c_longlong(), C.c_double(), C.c_double()
             self._chk(self.lib.mlq_prof_get(self.ctx, cls, C.byref(cnt), C.byref(ms), C.byref(by)))
             out[self.lib.mlq_prof_name(cls).decode()] = (cnt.value, ms.value, by.value)
