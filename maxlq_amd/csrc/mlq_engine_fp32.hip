@@ -6,6 +6,10 @@
 
 namespace mlq {
 
+// the FP64 twin of the mixed line search is the FP64 engine, instantiated in
+// its own unit
+extern template class Engine<double>;
+
 template class Engine<float>;
 
 std::unique_ptr<IEngine> make_engine_fp32(int n, const mlq_params& p) {

@@ -42,6 +42,16 @@ template <typename T, typename P> class Tmp {
   size_t bytes_ = 0;
 };
 
+// Coefficient fields of the generic line search for phi = u + tau d1, d1 =
+// d + gam u: D(phi.phi) = S0 + tau S1 + tau^2 S2, the pressure P0 + tau P1 +
+// tau^2 P2 (P: 3 contiguous fields), |grad phi|^2 = Q0 + tau Q1 + tau^2 Q2,
+// w_j = D(sum_i phi_i d_j phi_i) = a_j + tau b_j + tau^2 e_j, u and d (3
+// contiguous fields each).
+template <typename T> struct LsGen {
+  const T *S0, *S1, *S2, *P, *Q0, *Q1, *Q2, *a, *b, *e, *u, *d;
+  double gam;
+};
+
 template <typename T> class Engine final : public IEngine {
  public:
   using C = c2<T>;
@@ -130,6 +140,16 @@ template <typename T> class Engine final : public IEngine {
   // selects the pointwise-kernel + transform form instead (same results up to
   // rounding; kept for tests of that form)
   bool fused_ = true;
+
+  // The sums of R(phi) at one step size from the generic line-search fields
+  // (the |phi|^k chain of calc_uk on D(phi.phi)): out = {sum |phi|^(q-2)
+  // |grad phi|^2, sum D(|phi|^(q-4)) w.w, sum D(p |phi|^(q-4)) phi.w,
+  // sum (|phi|^(q/2))^2}, unnormalised (function_ops.f90:1304-1358, 1675-1692)
+  void ls_generic(const LsGen<T>& f, double t, double out[4]);
+  // an FP64 engine on the same grid and parameters (created on first use):
+  // the mixed path evaluates its line-search J(tau) chain there, so that J is
+  // a smooth function of tau (brent resolves tau to TOL = 1e-6, J to 1e-12)
+  Engine<double>* fp64_twin();
 
   // pool of device temporaries, keyed by size
   RT rtmp(int k = 1);
@@ -251,6 +271,7 @@ template <typename T> class Engine final : public IEngine {
   double* red_dev_ = nullptr;
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;
+  std::unique_ptr<IEngine> twin_;   // fp64_twin()
   std::map<size_t, std::vector<void*>> pool_;
   size_t used_ = 0, peak_ = 0;
   // profiling
@@ -277,7 +298,11 @@ template <typename T> class Engine final : public IEngine {
   double fast_path_bytes() const {
     const bool q4 = std::fabs(p_.q - 4.0) < 1e-14;
     // + 9 fields: the velocity-gradient operand of the deferred state update
-    return (q4 ? 57.0 : 65.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+    double b = (q4 ? 57.0 : 65.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+    // FP32 fields, q != 4: the FP64 twin's copy of the 24 line-search fields,
+    // its chain temporaries (8 fields) and its transform scratch (12 spectra)
+    if (!q4 && sizeof(T) == 4) b += 32.0 * 8.0 * (double)nr_ + 12.0 * 16.0 * (double)n_ * n_ * (n_ / 2 + 8);
+    return b;
   }
  public:
   bool uses_fast_path() const override { return fast_path_ok(); }

@@ -422,6 +422,7 @@ template <typename T> class FastRcg {
       }
     }
     cache_.clear();
+    g64_fresh_ = false;
   }
 
   // ---- J_LineMin(tau) = -R(B phi/||phi||_q) for a batch of step sizes ---------------
@@ -750,34 +751,43 @@ template <typename T> class FastRcg {
     }
   }
 
+  // generic line search (q != 4): the sums of R(phi) at tau from the
+  // coefficient fields.  FP32 fields: the |phi|^k chain runs on the FP64 twin
+  // engine from exact FP64 copies of the fields (made once per precompute),
+  // so J(tau) carries the FP32 rounding of the fields but no rounding that
+  // depends on tau; brent then sees a smooth J and stops after as many
+  // requests as in FP64.
   void eval_generic(double t, double* J, double* Lout) {
-    RT s1 = e_.rtmp();
-    e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)t, (T)1}, nr_);
-    typename E_::Chain ch;
-    e_.chain_init(ch, nullptr, s1);
-    const T* A = e_.chain_pow(ch, q_ - 2.0);
-    const T* Bq = e_.chain_pow(ch, q_ - 4.0);
-    const T* Lh = e_.chain_pow(ch, q_ / 2.0);
-    // D(Bq) unless |phi|^(q-4) is already band-limited by the 2/3 mask
-    RT Bw;
-    const T* Bwp = Bq;
-    if (q_ - 4.0 + 2.0e-16 < 2.0) {
-      Bw = e_.rtmp();
-      e_.map(F_copy<T>{Bq, Bw}, nr_);
-      dealias1(Bw);
-      Bwp = Bw;
-    }
-    RT Pd = e_.rtmp();
-    e_.map(F_pB<T>{P_, P_ + nr_, P_ + 2 * nr_, Bq, Pd, (T)t}, nr_);
-    dealias1(Pd);
-    // physical |grad phi|^2 coefficients: Q0 (state), Q1, Q2
     double acc[4];
-    e_.template reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, nr_, t,
-                                          gam_}, nr_, acc);
+    if constexpr (std::is_same<T, double>::value) {
+      e_.ls_generic(LsGen<T>{s_, S1_, S2_, P_, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, gam_}, t, acc);
+    } else {
+      Engine<double>* tw = e_.fp64_twin();
+      if (!g64_fresh_) load_twin(tw);
+      tw->ls_generic(gen64_, t, acc);
+    }
     const double R1 = -nuvc_ * acc[0] / N3_;
     const double R2 = -nuvc_ * (q_ - 2.0) * acc[1] / N3_;
     const double R3 = pc_ * (q_ - 2.0) * acc[2] / N3_;
     finish_J(t, R1 + R2, R3, acc[3] / N3_, J, Lout);
+  }
+  // exact FP64 copies of the generic line-search fields on the twin
+  template <typename W = T> void load_twin(Engine<double>* tw) {
+    static_assert(std::is_same<W, float>::value, "FP32 fields only");
+    if (!g64_.get()) g64_ = tw->rtmp(24);
+    const W* src[12] = {s_, S1_, S2_, P_, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_};
+    const int nc[12] = {1, 1, 1, 3, 1, 1, 1, 3, 3, 3, 3, 3};
+    const double* p[12];
+    double* dst = g64_;
+    for (int k = 0; k < 12; ++k) {
+      e_.map(F_widen{src[k], dst}, nc[k] * nr_);
+      p[k] = dst;
+      dst += nc[k] * nr_;
+    }
+    e_.sync();   // the twin runs on its own stream
+    gen64_ = LsGen<double>{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11],
+                           gam_};
+    g64_fresh_ = true;
   }
 
   E_& e_;
@@ -840,6 +850,10 @@ template <typename T> class FastRcg {
     return tot > 0.0 && m <= tol * tot;
   }
   std::map<double, double> cache_;
+  // FP64 copies of the generic line-search fields (FP32 engine, q != 4)
+  Tmp<double, double> g64_;
+  LsGen<double> gen64_{};
+  bool g64_fresh_ = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -974,6 +988,46 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
 }  // namespace mlq
 
 namespace mlq {
+
+template <typename T> void Engine<T>::ls_generic(const LsGen<T>& f, double t, double out[4]) {
+  const double q = p_.q;
+  RT s1 = rtmp();
+  map(F_quad<T>{f.S0, f.S1, f.S2, s1, (T)t, (T)1}, nr_);
+  Chain ch;
+  chain_init(ch, nullptr, s1);
+  const T* A = chain_pow(ch, q - 2.0);
+  const T* Bq = chain_pow(ch, q - 4.0);
+  const T* Lh = chain_pow(ch, q / 2.0);
+  // D(Bq) unless |phi|^(q-4) is already band-limited by the 2/3 mask
+  RT Bw;
+  const T* Bwp = Bq;
+  if (q - 4.0 + 2.0e-16 < 2.0) {
+    Bw = rtmp();
+    map(F_copy<T>{Bq, Bw}, nr_);
+    T* b = Bw;
+    dealias_dev(&b, 1, 2.0);
+    Bwp = Bw;
+  }
+  RT Pd = rtmp();
+  map(F_pB<T>{f.P, f.P + nr_, f.P + 2 * nr_, Bq, Pd, (T)t}, nr_);
+  {
+    T* pd = Pd;
+    dealias_dev(&pd, 1, 2.0);
+  }
+  reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, f.Q0, f.Q1, f.Q2, f.a, f.b, f.e, f.u, f.d, nr_, t,
+                            f.gam}, nr_, out);
+}
+
+template <typename T> Engine<double>* Engine<T>::fp64_twin() {
+  if constexpr (std::is_same<T, double>::value) {
+    return this;
+  } else {
+    if (!twin_) twin_.reset(new Engine<double>(n_, p_));
+    Engine<double>* e = static_cast<Engine<double>*>(twin_.get());
+    e->set_params(p_);
+    return e;
+  }
+}
 
 template <typename T>
 void Engine<T>::rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) {

@@ -17,6 +17,19 @@ template <class F> __global__ __launch_bounds__(MAP_THREADS) void k_map(F f, lon
 
 // ---- physical-space functors -------------------------------------------------
 template <typename T> struct F_fill { T* o; T v; __device__ void operator()(long long i) const { o[i] = v; } };
+// exact widening of FP32 values to FP64 (4 points per call)
+struct F_widen {
+  static constexpr int vec = 4;
+  const float* a; double* o;
+  __device__ void operator()(long long i) const {
+    float v[4];
+    ld16(a, i, v);
+    const double w0[2] = {(double)v[0], (double)v[1]}, w1[2] = {(double)v[2], (double)v[3]};
+    st16(o, 2 * i, w0);
+    st16(o, 2 * i + 1, w1);
+  }
+};
+
 template <typename T> struct F_copy {   // 16-byte vectors
   static constexpr int vec = V16<T>::w;
   const T* a; T* o;

@@ -88,10 +88,11 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
     calc0thFourierModes(u, m0);
     if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
-    // line search on J(tau) = eval_J(u + tau d1, "LineMin")
+    // line search on J(tau) = eval_J(u + tau d1, "LineMin"), every value
+    // computed from the field as eval_J does (optimization.f90:531-539)
     JFun J = [&](double tau) {
       double v;
-      eval_J_batch(u, d1, &tau, 1, &v);
+      eval_J_batch_ref(u, d1, &tau, 1, &v);
       ++nJ; ++nJc;
       if (std::isnan(v)) throw Error(MLQ_ENAN, "nan in J");
       return v;
