@@ -36,7 +36,8 @@ EXPORTS = [
     "mlq_eval_J", "mlq_field_alloc", "mlq_field_free", "mlq_upload", "mlq_download",
     "mlq_seeded_field", "mlq_eval_J_batch", "mlq_time_linesearch", "mlq_kappa_test", "mlq_rcg_path", "mlq_rcg_solve", "mlq_time_fft", "mlq_time_copy", "mlq_time_column_copy",
     "mlq_autotune", "mlq_sizes",
-    "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info", "mlq_transform_counts",
+    "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info", "mlq_mem_live",
+    "mlq_transform_counts",
 ]
 
 
@@ -132,6 +133,7 @@ def load(path: str = LIB_PATH):
         "mlq_prof_get": (ci, [vp, ci, C.POINTER(C.c_longlong), dp, dp]),
         "mlq_prof_name": (C.c_char_p, [ci]),
         "mlq_mem_info": (ci, [vp, dp, dp, dp, dp]),
+        "mlq_mem_live": (ci, [vp, dp, dp]),
         "mlq_transform_counts": (ci, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():

@@ -426,6 +426,9 @@ const char* mlq_prof_name(int cls) {
                                         "reduce", "linesearch_pointwise", "fft_z_roundtrip"};
   return (cls >= 0 && cls < PC_COUNT) ? names[cls] : "?";
 }
+int mlq_mem_live(mlq_ctx* ctx, double* live, double* peak) {
+  return guarded(ctx, [&] { ctx->eng->mem_live(live, peak); });
+}
 int mlq_mem_info(mlq_ctx* ctx, double* used, double* peak, double* dfree, double* dtotal) {
   return guarded(ctx, [&] {
     ctx->eng->mem_info(used, peak);

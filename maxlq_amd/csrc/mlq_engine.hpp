@@ -103,6 +103,8 @@ class IEngine {
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;
   virtual void mem_info(double* used, double* peak) = 0;
+  // device bytes in use (allocated minus cached in the pool) and their peak
+  virtual void mem_live(double* live, double* peak) = 0;
   virtual void transform_counts(long long* fwd, long long* inv) const = 0;
 };
 

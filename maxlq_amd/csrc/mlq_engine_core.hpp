@@ -89,6 +89,7 @@ template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
     for (auto& kv : pool_)
       for (void* q : kv.second) dfree(q, kv.first);
     pool_.clear();
+    pooled_ = 0;
     e = hipMalloc(&p, bytes);
   }
   if (e != hipSuccess) {
@@ -97,6 +98,7 @@ template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
   }
   used_ += bytes;
   peak_ = std::max(peak_, used_);
+  note_live();
   return p;
 }
 template <typename T> void Engine<T>::dfree(void* p, size_t bytes) {
@@ -110,7 +112,7 @@ template <typename T> typename Engine<T>::RT Engine<T>::rtmp(int k) {
   const size_t bytes = sizeof(T) * nr_ * k;
   auto& v = pool_[bytes];
   void* p;
-  if (!v.empty()) { p = v.back(); v.pop_back(); }
+  if (!v.empty()) { p = v.back(); v.pop_back(); pooled_ -= bytes; note_live(); }
   else p = dmalloc(bytes);
   return RT(this, (T*)p, bytes);
 }
@@ -118,7 +120,7 @@ template <typename T> typename Engine<T>::ST Engine<T>::stmp(int k) {
   const size_t bytes = sizeof(C) * ns_ * k + 64;  // distinct key from real blocks
   auto& v = pool_[bytes];
   void* p;
-  if (!v.empty()) { p = v.back(); v.pop_back(); }
+  if (!v.empty()) { p = v.back(); v.pop_back(); pooled_ -= bytes; note_live(); }
   else p = dmalloc(bytes);
   return ST(this, (C*)p, bytes);
 }

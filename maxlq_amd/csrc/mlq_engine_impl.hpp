@@ -131,6 +131,10 @@ template <typename T> class Engine final : public IEngine {
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
   void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }
+  void mem_live(double* live, double* peak) override {
+    *live = (double)(used_ - pooled_);
+    *peak = (double)live_peak_;
+  }
   void transform_counts(long long* fwd, long long* inv) const override {
     *fwd = nfwd_;
     *inv = ninv_;
@@ -154,7 +158,10 @@ template <typename T> class Engine final : public IEngine {
   // pool of device temporaries, keyed by size
   RT rtmp(int k = 1);
   ST stmp(int k = 1);
-  void pool_put(void* p, size_t bytes) { pool_[bytes].push_back(p); }
+  void pool_put(void* p, size_t bytes) {
+    pool_[bytes].push_back(p);
+    pooled_ += bytes;
+  }
 
  private:
   // ---- set-up ----
@@ -274,6 +281,10 @@ template <typename T> class Engine final : public IEngine {
   std::unique_ptr<IEngine> twin_;   // fp64_twin()
   std::map<size_t, std::vector<void*>> pool_;
   size_t used_ = 0, peak_ = 0;
+  // bytes held by the pool (allocated, not in use) and the high-water mark
+  // of the bytes in use (allocated minus pooled)
+  size_t pooled_ = 0, live_peak_ = 0;
+  void note_live() { live_peak_ = std::max(live_peak_, used_ - pooled_); }
   // profiling
   bool prof_ = false;
   struct ProfRec { int cls; hipEvent_t a, b; double bytes; };

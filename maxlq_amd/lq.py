@@ -395,6 +395,12 @@ class MaxLq:
         self._chk(self.lib.mlq_mem_info(self.ctx, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
+    def mem_live(self):
+        """(bytes in use, their peak): allocations minus cached temporaries."""
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.mlq_mem_live(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def transform_counts(self):
         """(forward, inverse) fields transformed since the context was created."""
         f, i = C.c_longlong(), C.c_longlong()
