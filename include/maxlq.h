@@ -89,6 +89,17 @@ int mlq_fftfwd(mlq_ctx* ctx, const double* u_cplx, double* fu_cplx);
 int mlq_fftbwd(mlq_ctx* ctx, const double* fu_cplx, double* u_cplx);
 /* fftfwdr(u, fu): real -> (n1/2+1, n2, n3) half spectrum, :118-134 */
 int mlq_fftfwdr(mlq_ctx* ctx, const double* u_real, double* fu_half);
+/* mlq_fftfwdr on device arrays, allocating no device memory.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  u_dev: n^3
+ * FP64 values, x fastest; fu_dev: (n/2+1, n, n) complex FP64, interleaved,
+ * the layout mlq_fftfwdr returns.  FP64 whatever the context's precision.
+ * The intermediate spectrum lives in a per-device static array of 128 x 128
+ * x 72 complex doubles (18.9 MB, held by every GPU the library runs on until
+ * the process ends), so n <= 128; larger grids return MLQ_EINVAL.  Calls on
+ * one GPU share that array: calls that overlap on one GPU (from any context)
+ * race.  Synchronous on the context's stream. */
+int mlq_fftfwdr_noalloc(mlq_ctx* ctx, const double* u_dev, double* fu_dev);
 /* fftbwdr(fu, u): half spectrum -> real, divided by n1*n2*n3, :139-165 */
 int mlq_fftbwdr(mlq_ctx* ctx, const double* fu_half, double* u_real);
 /* fftfwdV / fftbwdV: the vector forms, :24-57 */

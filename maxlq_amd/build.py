@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmaxlq.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["mlq_engine_fp64.hip", "mlq_engine_fp32.hip", "mlq_capi.cpp"]
+SOURCES = ["mlq_engine_fp64.hip", "mlq_engine_fp32.hip", "mlq_noalloc.hip", "mlq_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MLQ_OFFLOAD_ARCH", "gfx950")

@@ -25,7 +25,7 @@ MLQ_RCG_REFERENCE = 1   # mlq_rcg_cfg.flags: reference-structured iteration
 EXPORTS = [
     "mlq_default_params", "mlq_init", "mlq_finalize", "mlq_set_params", "mlq_last_error",
     "mlq_local_size_3d", "mlq_sync", "mlq_fftfwd", "mlq_fftbwd", "mlq_fftfwdr", "mlq_fftbwdr",
-    "mlq_fftfwdV", "mlq_fftbwdV", "mlq_gradient", "mlq_divergence", "mlq_laplacian",
+    "mlq_fftfwdV", "mlq_fftbwdV", "mlq_fftfwdr_noalloc", "mlq_gradient", "mlq_divergence", "mlq_laplacian",
     "mlq_solve_poisson", "mlq_dealias_scalar", "mlq_divAvg_free", "mlq_HilbertGradient",
     "mlq_calc0thFourierModes", "mlq_calc_uk", "mlq_calc_gk_order2", "mlq_calc_global_Lq_norm",
     "mlq_calculate_spectral_data", "mlq_spectral_data_field", "mlq_calc_global_summed_enstrophy",
@@ -97,6 +97,7 @@ def load(path: str = LIB_PATH):
         "mlq_fftfwd": (ci, [vp, dp, dp]), "mlq_fftbwd": (ci, [vp, dp, dp]),
         "mlq_fftfwdr": (ci, [vp, dp, dp]), "mlq_fftbwdr": (ci, [vp, dp, dp]),
         "mlq_fftfwdV": (ci, [vp, dp, dp]), "mlq_fftbwdV": (ci, [vp, dp, dp]),
+        "mlq_fftfwdr_noalloc": (ci, [vp, vp, vp]),
         "mlq_gradient": (ci, [vp, dp, dp]), "mlq_divergence": (ci, [vp, dp, dp]),
         "mlq_laplacian": (ci, [vp, dp]), "mlq_solve_poisson": (ci, [vp, dp, cd, dp]),
         "mlq_dealias_scalar": (ci, [vp, dp, cd]), "mlq_divAvg_free": (ci, [vp, dp]),

@@ -83,6 +83,7 @@ int mlq_init(mlq_ctx** ctx, int n1, int n2, int n3, const mlq_params* p) {
   c->n = n;
   int rc = guarded(c, [&] {
     c->eng = p->precision == MLQ_MIXED ? make_engine_fp32(n, *p) : make_engine_fp64(n, *p);
+    noalloc_bind();
     const char* at = std::getenv("MLQ_AUTOTUNE");
     if (n >= 128 && !(at && at[0] == '0')) c->eng->autotune(3, nullptr, nullptr);
   });
@@ -135,6 +136,11 @@ int mlq_fftfwdV(mlq_ctx* ctx, const double* v, double* fv) {
 }
 int mlq_fftbwdV(mlq_ctx* ctx, const double* fv, double* v) {
   return guarded(ctx, [&] { ctx->eng->fft_c2c(fv, v, true, 3); });
+}
+int mlq_fftfwdr_noalloc(mlq_ctx* ctx, const double* u_dev, double* fu_dev) {
+  return guarded(ctx, [&] {
+    fftfwdr_noalloc(ctx->n, ctx->eng->stream(), ctx->eng->twiddles(), u_dev, fu_dev);
+  });
 }
 int mlq_fftfwdr(mlq_ctx* ctx, const double* u, double* fu) {
   return guarded(ctx, [&] { ctx->eng->fft_r2c_host(u, fu); });

@@ -106,9 +106,15 @@ class IEngine {
   // device bytes in use (allocated minus cached in the pool) and their peak
   virtual void mem_live(double* live, double* peak) = 0;
   virtual void transform_counts(long long* fwd, long long* inv) const = 0;
+  virtual hipStream_t stream() const = 0;          // the context's HIP stream
+  virtual const double2* twiddles() const = 0;     // exp(-2 pi i k/n), k < n, on the device
 };
 
 std::unique_ptr<IEngine> make_engine_fp64(int n, const mlq_params& p);
+// mlq_noalloc.hip: device address of the no-allocation scratch on the
+// current device (once per device), and fftfwdr through it
+void noalloc_bind();
+void fftfwdr_noalloc(int n, hipStream_t st, const double2* tw, const double* u, double* fu);
 std::unique_ptr<IEngine> make_engine_fp32(int n, const mlq_params& p);
 
 }  // namespace mlq

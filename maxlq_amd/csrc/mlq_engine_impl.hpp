@@ -135,6 +135,8 @@ template <typename T> class Engine final : public IEngine {
     *live = (double)(used_ - pooled_);
     *peak = (double)live_peak_;
   }
+  hipStream_t stream() const override { return st_; }
+  const double2* twiddles() const override { return tw_; }
   void transform_counts(long long* fwd, long long* inv) const override {
     *fwd = nfwd_;
     *inv = ninv_;
