@@ -449,3 +449,173 @@ def test_denstrophy_vortex_stretching_identity():
     lap2 = sum(float(np.sum(np.abs(ksq * uh[c]) ** 2)) for c in range(3)) / n ** 6
     got = orc.calc_global_dEnstrophydt(u)
     assert rel(got, -lap2 + stretch) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# The 2/3 mask active inside the nonlinear chain.  For fields whose dealiased
+# products stay alias-free (band-limited inside the cube, n a power of two)
+# every dealiased product has an exact value -- the cube mask of the
+# unaliased product, formed on a 2n grid (tests/fields.py) -- which fixes
+# calc_uk, calc_gk_order2, the gradient products and R(u) at q = 4 and 6
+# independently of the oracle's transforms and dealias code.
+# ---------------------------------------------------------------------------
+
+from tests.fields import (cube_mcut, exact_R, mask_down, masked_chain_field,  # noqa: E402
+                          spec_deriv, up2)
+
+
+def _scalar_bandlimited(n, seed=11, c=1.5, eps=0.4):
+    """c + eps w, w band-limited to max_j |m_j| <= mcut - 1, max |w| = 1."""
+    M = cube_mcut(n) - 1
+    rng = np.random.default_rng(seed)
+    w = bandlimit(rng.standard_normal((n, n, n)), M)
+    return c + eps * w / np.max(np.abs(w))
+
+
+def _exact_gk(g, k):
+    """calc_gk_order2(g, k) for a band-limited g (function_ops.f90:2281-2319):
+    gk(0) = 1, gk(1) = g, gk(k) = D(D(g g) gk(k-2)); every product alias-free."""
+    n = g.shape[-1]
+    mc = cube_mcut(n)
+    G2 = mask_down(up2(g) * up2(g), n, mc)
+    if k == 0:
+        return np.ones_like(g)
+    if k == 1:
+        return g
+    return mask_down(up2(G2) * up2(_exact_gk(g, k - 2)), n, mc)
+
+
+_CHAIN_CACHE = {}
+
+
+def _chain(n):
+    if n not in _CHAIN_CACHE:
+        u = masked_chain_field(n)
+        _CHAIN_CACHE[n] = (u, exact_R(u, 4.0)[1], _scalar_bandlimited(n))
+    return _CHAIN_CACHE[n]
+
+
+def _chain_pins(orc, n):
+    """(name, oracle value, exact value) of every chain quantity pinned."""
+    u, E, g = _chain(n)
+    out = [("calc_uk(u,1)", orc.calc_uk(u, 1.0), np.sqrt(E["S"])),
+           ("calc_uk(u,2)", orc.calc_uk(u, 2.0), E["S"]),
+           ("calc_uk(u,4)", orc.calc_uk(u, 4.0), E["SS"]),
+           ("calc_vgraduvgradu(u,u)", orc.calc_vgraduvgradu(u, u), E["vgraduvgradu"]),
+           ("calc_vvgradu(u,u)", orc.calc_vvgradu(u, u), E["vvgradu"]),
+           ("calc_nablaModSquared(u)", orc.calc_nablaModSquared(u), E["gradsq"]),
+           ("calc_nablaUnablaUt(u)", orc.calc_nablaUnablaUt(u), E["ggT"])]
+    for k in (2, 3, 4, 5):
+        out.append((f"calc_gk_order2(g,{k})", orc.calc_gk_order2(g, float(k)), _exact_gk(g, k)))
+    return out
+
+
+def test_masked_chain_field_exercises_the_mask():
+    """The pin fields: D(u.u) > 0 (no clipping), and u.u has modes beyond the
+    cube (the mask truncates) and beyond n/2 (the n-grid product aliases)."""
+    n = 32
+    u, E, g = _chain(n)
+    mc = cube_mcut(n)
+    assert E["S"].min() > 0.5
+    m = np.abs(mode_index(2 * n))
+    mm = np.maximum(np.maximum(m[:, None, None], m[None, :, None]), m[None, None, :])
+    for f2 in (sum(up2(u[i]) ** 2 for i in range(3)), up2(g) ** 2):
+        F = np.abs(np.fft.fftn(f2))
+        F[0, 0, 0] = 0.0                                   # fractions of the fluctuation
+        tot = np.sqrt(np.sum(F ** 2))
+        assert np.sqrt(np.sum(F[mm > mc] ** 2)) > 0.02 * tot        # truncated by the mask
+        assert np.sqrt(np.sum(F[mm > n // 2] ** 2)) > 0.005 * tot   # aliased on the n grid
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_chain_quantities_exact_with_active_mask(n):
+    orc = Oracle(Params(n=n, q=4.0))
+    for name, got, ref in _chain_pins(orc, n):
+        assert rel(got, ref) < 1e-12, name
+
+
+@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("q", [4.0, 6.0])
+def test_R_exact_with_active_mask(n, q):
+    """R, R1+R2 and R3 of calc_local_dLqdt_inclParts with every dealiased
+    product evaluated exactly (|u|^(q-2) = D(u.u) at q = 4, D(S S) at q = 6)."""
+    u = masked_chain_field(n)
+    ref, _ = exact_R(u, q)
+    got = Oracle(Params(n=n, q=q)).calc_local_dLqdt_inclParts(u)
+    assert abs(got[0] - ref[0]) <= 1e-12 * abs(ref[0])
+    assert abs(got[1] - ref[1]) <= 1e-12 * abs(ref[1])
+    assert abs(got[2] - ref[2]) <= 1e-11 * abs(ref[2])
+
+
+def test_R_q4_closed_forms_with_active_mask():
+    """q = 4, S = D(u.u): R1 = -nu <S, |grad u|^2> and R2 = -(nu/2) ||grad S||^2
+    = -(nu/2) sum_k |K|^2 |S_hat_k|^2 / N^6 (D is an orthogonal projection and
+    D(u_i d_j u_i) = d_j S / 2)."""
+    n = 32
+    u = masked_chain_field(n)
+    _, E = exact_R(u, 4.0)
+    S = E["S"]
+    gsq = sum(spec_deriv(u[i], j) ** 2 for i in range(3) for j in range(3))
+    R1 = -np.sum(S * gsq) / n ** 3
+    K = TP * mode_index(n)
+    ksq = K[None, None, :] ** 2 + K[None, :, None] ** 2 + K[:, None, None] ** 2
+    R2 = -0.5 * np.sum(ksq * np.abs(np.fft.fftn(S)) ** 2) / n ** 6
+    got = Oracle(Params(n=n, q=4.0)).calc_local_dLqdt_inclParts(u)
+    assert abs(got[1] - (R1 + R2)) <= 1e-12 * abs(R1 + R2)
+
+
+def _dealias_sites(orc, run):
+    """Source lines of the oracle that call dealias_scalar with a mask that is
+    not the identity, during run()."""
+    import sys
+    sites = set()
+    orig = orc.dealias_scalar
+
+    def spy(f, nonlinOrder=2.0):
+        if not orc.dealias_mask(nonlinOrder).all():
+            fr = sys._getframe(1)
+            while fr.f_code.co_name in ("_D", "dealias_scalar"):
+                fr = fr.f_back
+            sites.add((fr.f_code.co_name, fr.f_lineno))
+        return orig(f, nonlinOrder)
+
+    orc.dealias_scalar = spy
+    try:
+        run()
+    finally:
+        del orc.dealias_scalar
+    return sites
+
+
+def test_every_chain_dealias_is_pinned():
+    """Mutations planted in the oracle on the CPU: deleting any one dealias
+    call (every call from one source line) of calc_uk, calc_gk_order2,
+    calc_vgraduvgradu, calc_vvgradu, calc_nablaModSquared or
+    calc_nablaUnablaUt changes at least one pinned quantity by far more than
+    round-off."""
+    import sys
+    n = 32
+    orc = Oracle(Params(n=n, q=4.0))
+    sites = _dealias_sites(orc, lambda: _chain_pins(orc, n))
+    targets = {"calc_uk", "calc_gk_order2", "calc_vgraduvgradu", "calc_vvgradu",
+               "calc_nablaModSquared", "calc_nablaUnablaUt"}
+    assert {s[0] for s in sites} >= targets - {"calc_nablaModSquared", "calc_nablaUnablaUt"}
+    orig = orc.dealias_scalar
+    missed = []
+    for site in sorted(sites):
+        def mutant(f, nonlinOrder=2.0, site=site):
+            fr = sys._getframe(1)
+            while fr.f_code.co_name in ("_D", "dealias_scalar"):
+                fr = fr.f_back
+            if (fr.f_code.co_name, fr.f_lineno) == site:
+                return f                                   # the deleted dealias
+            return orig(f, nonlinOrder)
+        orc.dealias_scalar = mutant
+        try:
+            worst = max(rel(got, ref) for _, got, ref in _chain_pins(orc, n))
+        finally:
+            del orc.dealias_scalar
+        if worst < 1e-9:
+            missed.append(site)
+    print("dealias sites pinned:", sorted(sites))
+    assert not missed, missed
