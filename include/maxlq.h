@@ -18,6 +18,8 @@
 #ifndef MAXLQ_H
 #define MAXLQ_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -80,6 +82,12 @@ const char* mlq_last_error(const mlq_ctx* ctx);
 /* single-rank slab: local_N = n3, offset 0 (replaces fftw_mpi_local_size_3d,
  * initialize.f90:149) */
 int mlq_local_size_3d(const mlq_ctx* ctx, int* local_n, int* local_k_offset);
+/* fftw_mpi_local_size_3d(n0, n1, n2, comm, local_n0, local_0_start) of FFTW's
+ * MPI interface on one rank, which initialize.f90:149 calls through
+ * maxlq_amd/fortran/fftw3-mpi.f03: local_n0 = n0, local_0_start = 0, returns
+ * n0 n1 n2 (no context needed) */
+intptr_t mlq_fftw_mpi_local_size_3d(intptr_t n0, intptr_t n1, intptr_t n2, int32_t comm,
+                                    intptr_t* local_n0, intptr_t* local_0_start);
 int mlq_sync(mlq_ctx* ctx);
 
 /* ---- fftwfunction (fftwfunction.f90:62-165) ---- */
@@ -213,6 +221,9 @@ typedef struct mlq_rcg_cfg {
                             this many iterations the solve delivers MLQ_LOG_SPECTRUM
                             records of uvec and d (optimization.f90:119-121, 377-381);
                             < 1 never */
+  int save_every;        /* save_uvecEveryXiteration (global_variables.f90:70): every this
+                            many iterations the solve delivers the iterate as an
+                            MLQ_LOG_FIELD record (optimization.f90:388-390); < 1 never */
 } mlq_rcg_cfg;
 /* cfg.flags: run every operator from the field as the reference orders it
  * (no resident state, no tau-polynomial line search); validation path */
@@ -250,10 +261,41 @@ typedef struct mlq_rcg_result {
                               1 d; d[0..2n1-1] = spectral_data(1:n1,1:2) as
                               mlq_calculate_spectral_data returns it (save_spectral_data,
                               data_ops.f90:227-261, writes it) */
+#define MLQ_LOG_FIELD 7    /* save_field(Uvec, "u_result") every save_every iterations
+                              (optimization.f90:388-390): i0 = iter, d = the iterate, 3 n^3
+                              doubles in the reference layout (valid during the call) */
 typedef void (*mlq_log_cb)(void* user, int kind, int iter, int i0, int i1, const double* d);
 
 int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_result* res,
                   mlq_log_cb cb, void* user);
+
+/* ---- field files and result-file names (host only, no context) ---- */
+/* save_field / save_field_R3toR3_ncdf (data_ops.f90:9-41, 390-447): the vector
+ * field (3 component blocks of n1*n2*n3 doubles, x fastest) as a NetCDF classic
+ * file with dimensions x, y, z and double variables Ux, Uy, Uz; CDF-1 while
+ * its offsets fit 32 bits, else the 64-bit-offset format, with z the record
+ * dimension when one variable exceeds 4 GiB (1024^3) */
+int mlq_save_field_nc(const char* path, int n1, int n2, int n3, const double* field);
+/* read_field_R3toR3_ncdf (data_ops.f90:1198-1397) for the names Ux, Uy, Uz:
+ * either format and layout; MLQ_EINVAL when the file, its variables or its
+ * dimensions do not match the grid */
+int mlq_read_field_nc(const char* path, int n1, int n2, int n3, double* field);
+/* the reference's field-file name (data_ops.f90:31):
+ * dir + field_name + "_q" + lebesgueQTxt + "_n" + n + "_B" + i3.3(b_iter) +
+ * "_iter" + (i5.5(iter), or "end" for iter < 0) + ".nc"; lebesgueQTxt as
+ * setLebesgueQtext writes it (initialize.f90:49-70: i1 or i2 for integer q
+ * below 100, else ES9.2) */
+int mlq_result_filename(char* buf, int buflen, const char* dir, const char* field_name, double q,
+                        int n, int b_iter, int iter);
+int mlq_q_text(double q, char* buf, int buflen);
+/* set_q_resol_bIterOffset_optimIterOffsets (function_ops.f90:4635-4674, with
+ * getQfromFileName / getBiterfromFileName :4599-4632): q after the first
+ * "_q", resol after the first "_n", the B index after the first "_B", and the
+ * iteration after the first "_iter"; a name ending in "iterend" resumes after
+ * that B solve (b_iter_offset = B index, optim_iter_offset = 0), any other
+ * inside it (b_iter_offset = B index - 1, optim_iter_offset = the iteration) */
+int mlq_parse_result_filename(const char* name, double* q, int* resol, int* b_iter_offset,
+                              int* optim_iter_offset);
 
 /* ---- measurement helpers ---- */
 /* time `reps` launches of the 3-D real transform pair of a batch of `batch`

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmaxlq.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["mlq_engine_fp64.hip", "mlq_engine_fp32.hip", "mlq_noalloc.hip", "mlq_capi.cpp"]
+SOURCES = ["mlq_engine_fp64.hip", "mlq_engine_fp32.hip", "mlq_noalloc.hip", "mlq_capi.cpp",
+           "mlq_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MLQ_OFFLOAD_ARCH", "gfx950")
@@ -58,24 +59,49 @@ FSMOKE = os.path.join(FDIR, "mlq_fsmoke")
 FC = os.environ.get("MLQ_FC", "/opt/rocm/bin/amdflang")
 
 
+def _fc(args, cwd):
+    r = subprocess.run([FC] + args, capture_output=True, text=True, cwd=cwd)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran build failed: " + " ".join(args[-4:]))
+
+
+REF_GLOBALS = "/root/reference/global_variables.f90"
+
+
 def build_fortran(verbose: bool = True) -> str | None:
-    """The Fortran interface module and the Fortran smoke caller (amdflang)."""
+    """The Fortran interface module and its callers (amdflang): the smoke
+    caller, the B-continuation driver, the fftw3-mpi.f03 stand-in check and,
+    where the reference's sources are mounted, the check of the drop-in
+    modules against the reference's own global_variables module (compiled
+    where it lies; objects stay in the build directory)."""
     if not os.path.exists(FC):
         if verbose:
-            print("amdflang not found: Fortran caller not built")
+            print("amdflang not found: Fortran callers not built")
         return None
     mod = os.path.join(OBJ, "fortran")
     os.makedirs(mod, exist_ok=True)
-    cmd = [FC, "-O2", "-J", mod, os.path.join(FDIR, "maxlq_c.f90"),
-           os.path.join(FDIR, "mlq_fsmoke.f90"), "-o", FSMOKE + ".tmp",
-           "-L", HERE, "-lmaxlq", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mod)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("Fortran build failed")
-    os.replace(FSMOKE + ".tmp", FSMOKE)
-    if verbose:
-        print(f"built {FSMOKE}")
+    link = ["-L", HERE, "-lmaxlq", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."]
+    iface = os.path.join(FDIR, "maxlq_c.f90")
+    progs = {"mlq_fsmoke": [iface, os.path.join(FDIR, "mlq_fsmoke.f90")],
+             "mlq_continuation": [iface, os.path.join(FDIR, "mlq_continuation.f90")],
+             "mlq_fftwmpi_check": [os.path.join(FDIR, "mlq_fftwmpi_check.f90")]}
+    for name, srcs in progs.items():
+        out = os.path.join(FDIR, name)
+        _fc(["-O2", "-J", mod, "-I", FDIR] + srcs + ["-o", out + ".tmp"] + link, mod)
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print(f"built {out}")
+    if os.path.exists(REF_GLOBALS):
+        rmod = os.path.join(OBJ, "fortran_ref")
+        os.makedirs(rmod, exist_ok=True)
+        srcs = [REF_GLOBALS, iface, os.path.join(FDIR, "fftwfunction_gpu.f90"),
+                os.path.join(FDIR, "mlq_field_ops.f90"), os.path.join(FDIR, "mlq_dropin_check.f90")]
+        out = os.path.join(FDIR, "mlq_dropin_check")
+        _fc(["-O2", "-J", rmod] + srcs + ["-o", out + ".tmp"] + link, rmod)
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print(f"built {out}")
     return FSMOKE
 
 

@@ -19,6 +19,7 @@ MLQ_IP_L2, MLQ_IP_HL, MLQ_IP_H, MLQ_IP_OLD_H, MLQ_IP_OLD_HL = range(5)
 MLQ_J_MAXDLQDT, MLQ_J_LINEMIN, MLQ_J_LQNORM = range(3)
 MLQ_LOG_ITER, MLQ_LOG_LINEMIN, MLQ_LOG_BRENT, MLQ_LOG_MSG, MLQ_LOG_ERR = 1, 2, 3, 4, 5
 MLQ_LOG_SPECTRUM = 6
+MLQ_LOG_FIELD = 7
 MLQ_RCG_REFERENCE = 1   # mlq_rcg_cfg.flags: reference-structured iteration
 
 # symbols the header declares (checked by the CPU test suite)
@@ -37,7 +38,8 @@ EXPORTS = [
     "mlq_seeded_field", "mlq_eval_J_batch", "mlq_time_linesearch", "mlq_kappa_test", "mlq_rcg_path", "mlq_rcg_solve", "mlq_time_fft", "mlq_time_copy", "mlq_time_column_copy",
     "mlq_autotune", "mlq_sizes",
     "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info", "mlq_mem_live",
-    "mlq_transform_counts",
+    "mlq_transform_counts", "mlq_save_field_nc", "mlq_read_field_nc", "mlq_result_filename",
+    "mlq_q_text", "mlq_parse_result_filename", "mlq_fftw_mpi_local_size_3d",
 ]
 
 
@@ -58,7 +60,7 @@ class Params(C.Structure):
 class RcgCfg(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("optim_tol", C.c_double), ("start_iter", C.c_int),
                 ("batch_linesearch", C.c_int), ("reset_every", C.c_int), ("rolling", C.c_int),
-                ("flags", C.c_int), ("spectra_every", C.c_int)]
+                ("flags", C.c_int), ("spectra_every", C.c_int), ("save_every", C.c_int)]
 
 
 class RcgResult(C.Structure):
@@ -135,6 +137,12 @@ def load(path: str = LIB_PATH):
         "mlq_prof_name": (C.c_char_p, [ci]),
         "mlq_mem_info": (ci, [vp, dp, dp, dp, dp]),
         "mlq_mem_live": (ci, [vp, dp, dp]),
+        "mlq_save_field_nc": (ci, [C.c_char_p, ci, ci, ci, dp]),
+        "mlq_read_field_nc": (ci, [C.c_char_p, ci, ci, ci, dp]),
+        "mlq_result_filename": (ci, [C.c_char_p, ci, C.c_char_p, C.c_char_p, cd, ci, ci, ci]),
+        "mlq_q_text": (ci, [cd, C.c_char_p, ci]),
+        "mlq_parse_result_filename": (ci, [C.c_char_p, dp, C.POINTER(ci), C.POINTER(ci),
+                                           C.POINTER(ci)]),
         "mlq_transform_counts": (ci, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():

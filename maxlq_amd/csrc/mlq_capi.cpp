@@ -116,6 +116,14 @@ int mlq_local_size_3d(const mlq_ctx* ctx, int* local_n, int* local_k_offset) {
   return MLQ_OK;
 }
 
+intptr_t mlq_fftw_mpi_local_size_3d(intptr_t n0, intptr_t n1, intptr_t n2, int32_t comm,
+                                    intptr_t* local_n0, intptr_t* local_0_start) {
+  (void)comm;
+  if (local_n0) *local_n0 = n0;
+  if (local_0_start) *local_0_start = 0;
+  return n0 * n1 * n2;
+}
+
 int mlq_sync(mlq_ctx* ctx) { return guarded(ctx, [&] { ctx->eng->sync(); }); }
 
 // ---- fftwfunction ----

@@ -114,6 +114,13 @@ template <typename T> class Engine final : public IEngine {
     spectrum_of(cfg, u, s);
     log.emit(MLQ_LOG_SPECTRUM, iter, which, 0, s.data());
   }
+  // the iterate as an MLQ_LOG_FIELD record (save_field, optimization.f90:388-390)
+  void emit_field(const LogSink& log, int iter, int u) {
+    field_host_.resize(3 * (size_t)nr_);
+    download(u, field_host_.data());
+    log.emit(MLQ_LOG_FIELD, iter, iter, 0, field_host_.data());
+  }
+  std::vector<double> field_host_;
   void grad_L2(int u_vec, int out_vec) override;
   double eval_J(int f_vec, int system) override;
   void eval_J_batch(int u, int d, const double* taus, int ntau, double* J) override;

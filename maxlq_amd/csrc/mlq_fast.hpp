@@ -960,6 +960,10 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
       e.emit_spectrum(log, cfg, iter, 0, F.u_field());
       log.emit(MLQ_LOG_SPECTRUM, iter, 1, 0, dspec.data());
     }
+    if (cfg.save_every > 0 && iter % cfg.save_every == 0 && log.cb) {   // :388-390
+      F.download_u(e.fphys(u));
+      e.emit_field(log, iter, u);
+    }
     J0 = J1;
     tau0 = tau1;
     for (int i = nroll - 1; i > 0; --i) rel[i] = rel[i - 1];

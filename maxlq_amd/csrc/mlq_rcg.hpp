@@ -159,6 +159,8 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
       emit_spectrum(log, cfg, iter, 0, fphys(u));
       emit_spectrum(log, cfg, iter, 1, fphys(d1));
     }
+    if (cfg.save_every > 0 && iter % cfg.save_every == 0 && log.cb)   // :388-390
+      emit_field(log, iter, u);
     J0 = J1;                                                        // :432-436
     gu0_norm = inner(gu1, gu1, HL);
     tau0 = tau1;

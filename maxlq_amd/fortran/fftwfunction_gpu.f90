@@ -11,6 +11,10 @@ module fftwfunction
    implicit none
 
    type(c_ptr), save :: mlq_ctx_ptr = c_null_ptr
+   ! precision of the device context: MLQ_FP64 (the parity path) or MLQ_MIXED
+   ! (FP32 fields and transforms, FP64 reductions and scalars); the caller may
+   ! set it before init_fft
+   integer(c_int), save :: mlq_precision = MLQ_FP64
 
 contains
 
@@ -46,7 +50,7 @@ contains
       p%toDealias = merge(1_c_int, 0_c_int, toDealias)
       p%use_e_u = merge(1_c_int, 0_c_int, use_e_u_instead_of_uqMinus4)
       p%dealiase_if_mult_by_e_u = merge(1_c_int, 0_c_int, dealiase_if_mult_by_e_u)
-      p%precision = MLQ_FP64
+      p%precision = mlq_precision
    end subroutine fill_params
 
    subroutine init_fft()                               ! fftwfunction.f90:169-185

@@ -15,7 +15,8 @@ module maxlq_c
                                 MLQ_IP_OLD_HL = 4
    integer(c_int), parameter :: MLQ_J_MAXDLQDT = 0, MLQ_J_LINEMIN = 1, MLQ_J_LQNORM = 2
    integer(c_int), parameter :: MLQ_LOG_ITER = 1, MLQ_LOG_LINEMIN = 2, MLQ_LOG_BRENT = 3, &
-                                MLQ_LOG_MSG = 4, MLQ_LOG_ERR = 5
+                                MLQ_LOG_MSG = 4, MLQ_LOG_ERR = 5, MLQ_LOG_SPECTRUM = 6, &
+                                MLQ_LOG_FIELD = 7
    integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1
 
    type, bind(C) :: mlq_params
@@ -27,6 +28,7 @@ module maxlq_c
       integer(c_int) :: max_iter
       real(c_double) :: optim_tol
       integer(c_int) :: start_iter, batch_linesearch, reset_every, rolling, flags, spectra_every
+      integer(c_int) :: save_every
    end type mlq_rcg_cfg
 
    type, bind(C) :: mlq_rcg_result
@@ -403,6 +405,43 @@ module maxlq_c
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: used_bytes, peak_bytes, device_free, device_total
       end function
+      ! field files and result-file names (host only): strings are
+      ! NUL-terminated character(kind=c_char) arrays
+      integer(c_int) function mlq_save_field_nc(path, n1, n2, n3, field) &
+            bind(C, name="mlq_save_field_nc")
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: n1, n2, n3
+         real(c_double), intent(in) :: field(*)
+      end function
+      integer(c_int) function mlq_read_field_nc(path, n1, n2, n3, field) &
+            bind(C, name="mlq_read_field_nc")
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: n1, n2, n3
+         real(c_double), intent(out) :: field(*)
+      end function
+      integer(c_int) function mlq_result_filename(buf, buflen, dir, field_name, q, n, b_iter, iter) &
+            bind(C, name="mlq_result_filename")
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_int), value :: buflen, n, b_iter, iter
+         character(kind=c_char), intent(in) :: dir(*), field_name(*)
+         real(c_double), value :: q
+      end function
+      integer(c_int) function mlq_q_text(q, buf, buflen) bind(C, name="mlq_q_text")
+         import :: c_int, c_double, c_char
+         real(c_double), value :: q
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_int), value :: buflen
+      end function
+      integer(c_int) function mlq_parse_result_filename(name, q, resol, b_iter_offset, &
+            optim_iter_offset) bind(C, name="mlq_parse_result_filename")
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(out) :: q
+         integer(c_int), intent(out) :: resol, b_iter_offset, optim_iter_offset
+      end function
       integer(c_int) function mlq_transform_counts(ctx, fwd_fields, inv_fields) &
             bind(C, name="mlq_transform_counts")
          import :: c_ptr, c_int, c_long_long
@@ -410,5 +449,34 @@ module maxlq_c
          integer(c_long_long), intent(out) :: fwd_fields, inv_fields
       end function
    end interface
+
+contains
+
+   ! a Fortran string as a NUL-terminated C string
+   function cstr(s) result(c)
+      character(len=*), intent(in) :: s
+      character(kind=c_char) :: c(len_trim(s) + 1)
+      integer :: i
+      do i = 1, len_trim(s)
+         c(i) = s(i:i)
+      end do
+      c(len_trim(s) + 1) = c_null_char
+   end function cstr
+
+   ! a NUL-terminated C string as a Fortran string
+   function fstr(c) result(s)
+      character(kind=c_char), intent(in) :: c(:)
+      character(len=:), allocatable :: s
+      integer :: i, n
+      n = 0
+      do i = 1, size(c)
+         if (c(i) == c_null_char) exit
+         n = i
+      end do
+      allocate(character(len=n) :: s)
+      do i = 1, n
+         s(i:i) = c(i)
+      end do
+   end function fstr
 
 end module maxlq_c

@@ -41,6 +41,7 @@ class RcgTrace:
     msgs: list = field(default_factory=list)             # (iter, code)
     errors: list = field(default_factory=list)           # (iter, code)
     spectra: list = field(default_factory=list)          # (iter, "uvec" | "d", (n, 2) array)
+    fields: list = field(default_factory=list)           # (iter, (3, n, n, n) array): save_every
 
 
 class MaxLq:
@@ -316,13 +317,14 @@ class MaxLq:
         return tab.reshape(16, 5), frac.value, best.value, row.value
 
     def maxdLqdt(self, hu, max_iter=99999, optim_tol=None, start_iter=0, batch_linesearch=0,
-                 reset_every=25, rolling=10, trace=True, spectra_every=0, flags=0):
+                 reset_every=25, rolling=10, trace=True, spectra_every=0, flags=0, save_every=0):
         """One RCG solve for the current B on the device-resident field ``hu``
         (optimization.f90:7-514).  Returns (RcgResult, RcgTrace)."""
         if optim_tol is None:
             optim_tol = 1.0e-4 if self.n >= 1024 else 1.0e-5       # initialize.f90:95-100
         cfg = capi.RcgCfg(int(max_iter), float(optim_tol), int(start_iter), int(batch_linesearch),
-                          int(reset_every), int(rolling), int(flags), int(spectra_every))
+                          int(reset_every), int(rolling), int(flags), int(spectra_every),
+                          int(save_every))
         res = capi.RcgResult()
         tr = RcgTrace()
 
@@ -337,6 +339,9 @@ class MaxLq:
                 tr.msgs.append((it, i0))
             elif kind == capi.MLQ_LOG_ERR:
                 tr.errors.append((it, i0))
+            elif kind == capi.MLQ_LOG_FIELD:
+                a = np.ctypeslib.as_array(d, shape=(3 * self.n ** 3,)).copy()
+                tr.fields.append((i0, a.reshape(self.vshape)))
             elif kind == capi.MLQ_LOG_SPECTRUM:
                 sp = np.ctypeslib.as_array(d, shape=(2 * self.n,)).copy()
                 tr.spectra.append((it, "uvec" if i0 == 0 else "d", sp.reshape(2, self.n).T.copy()))
@@ -406,3 +411,61 @@ class MaxLq:
         f, i = C.c_longlong(), C.c_longlong()
         self._chk(self.lib.mlq_transform_counts(self.ctx, C.byref(f), C.byref(i)))
         return f.value, i.value
+
+
+# -- field files and result-file names (data_ops.f90, function_ops.f90:4590-4674) ------------
+
+def save_field_nc(path, u):
+    """save_field_R3toR3_ncdf (data_ops.f90:390-447): Ux, Uy, Uz of a vector
+    field (3, n3, n2, n1) to a NetCDF classic file (library code, no GPU)."""
+    lib = capi.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if u.ndim != 4 or u.shape[0] != 3:
+        raise ValueError(f"expected a vector field (3, n3, n2, n1), got {u.shape}")
+    n3, n2, n1 = u.shape[1:]
+    rc = lib.mlq_save_field_nc(str(path).encode(), n1, n2, n3, _p(u))
+    if rc != capi.MLQ_OK:
+        raise MlqError(rc, f"cannot write {path}")
+
+
+def read_field_nc(path, n):
+    """read_field_R3toR3_ncdf (data_ops.f90:1198-1397) of an n^3 grid."""
+    lib = capi.load()
+    out = np.empty((3, n, n, n))
+    rc = lib.mlq_read_field_nc(str(path).encode(), n, n, n, _p(out))
+    if rc != capi.MLQ_OK:
+        raise MlqError(rc, f"cannot read a {n}^3 Ux/Uy/Uz field from {path}")
+    return out
+
+
+def result_filename(q, n, b_iter, iter_, field_name="u_result", dir_=""):
+    """The reference's field-file name (data_ops.f90:31); iter_ < 0 is "end"."""
+    lib = capi.load()
+    buf = C.create_string_buffer(4096)
+    rc = lib.mlq_result_filename(buf, len(buf), dir_.encode(), field_name.encode(), float(q),
+                                 int(n), int(b_iter), int(iter_))
+    if rc != capi.MLQ_OK:
+        raise MlqError(rc, "result_filename")
+    return buf.value.decode()
+
+
+def q_text(q):
+    """lebesgueQTxt (setLebesgueQtext, initialize.f90:49-70)."""
+    lib = capi.load()
+    buf = C.create_string_buffer(64)
+    if lib.mlq_q_text(float(q), buf, len(buf)) != capi.MLQ_OK:
+        raise MlqError(capi.MLQ_EINVAL, "q_text")
+    return buf.value.decode()
+
+
+def parse_result_filename(name):
+    """set_q_resol_bIterOffset_optimIterOffsets (function_ops.f90:4635-4674):
+    (q, resol, bIterOffset, optimizationIterOffset)."""
+    lib = capi.load()
+    q = C.c_double()
+    r, b, i = C.c_int(), C.c_int(), C.c_int()
+    rc = lib.mlq_parse_result_filename(str(name).encode(), C.byref(q), C.byref(r), C.byref(b),
+                                       C.byref(i))
+    if rc != capi.MLQ_OK:
+        raise MlqError(rc, f"cannot parse {name}")
+    return q.value, r.value, b.value, i.value
