@@ -191,6 +191,47 @@ int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus,
 int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,
                         int batch, double* J, double* t_ms);
 
+/* diagnosticScalars(U), function_ops.f90:2503-2602: the values it passes to
+ * save_diagnosticFields_global (data_ops.f90:125-155) with w = vel2vort(u):
+ * Energy(U) and Energy(W) (:2394-2412), the largest value of each component,
+ * vorCore = calculate_geometric_data(U, W, aux, "Q") (:2681-2832: volume
+ * where the Q criterion is positive, its maximum, and a third entry the
+ * reference leaves unset, 0 here), max |u|, max |w|, Helicity (:2479-2497)
+ * and the extrema of u.w; divU_L2 = inner_product(div u, div u, "L2") */
+typedef struct mlq_diag {
+  double K[3], E[3], Umax[3], Wmax[3], vorCore[3];
+  double magUmax, magWmax, H, maxHel, minHel, divU_L2;
+} mlq_diag;
+/* fields (may be null): 4 n^3 doubles |u|, |w|, u.w and Q / max Q
+ * (save_diagnosticScalars' magU, magW, Helicity, VortexCore); ring_xyz (may
+ * be null): the coordinates ((i-1)/n, (j-1)/n, (k-1)/n) of the points with
+ * |w| / max |w| > 0.95 in the reference's loop order, at most ring_cap of
+ * them (calculate_ring_data :2837-2900); *ring_points counts them all */
+int mlq_diagnosticScalars(mlq_ctx* ctx, const double* u, mlq_diag* out, double* fields,
+                          double* ring_xyz, long long ring_cap, long long* ring_points);
+
+/* kappa_test_pert(phi_pert, mytype, m1, m2, m3), function_ops.f90:613-977, into
+ * the device field `out`.  The "load-*" kinds take the loaded field (3 n^3
+ * doubles, e.g. from mlq_read_field_nc) where the reference reads its input
+ * file; MLQ_PERT_RANDOM_POLY ("create-random-poly-b", which kappa_test falls
+ * back to, optimization.f90:1024-1035) draws its uniform values from the
+ * project's seeded generator (SplitMix64 keyed by seed, component and point)
+ * where the reference calls random_number. */
+#define MLQ_PERT_SINE 0             /* "sine" (:660-673) */
+#define MLQ_PERT_DIVFREE_SINE 1     /* "divfree-sine" (:675-689) */
+#define MLQ_PERT_LOAD_RAW 2         /* "load-te0080": the loaded field as is (:655-658) */
+#define MLQ_PERT_LOAD 3             /* "load-random-a/b" (:692-702) */
+#define MLQ_PERT_LOAD_SMOOTH 4      /* "load-random-smooth-a" (:704-713) */
+#define MLQ_PERT_LOAD_SMOOTH_RAW 5  /* "load-random-smooth-b", no projection (:927-935) */
+#define MLQ_PERT_LOAD_K 6           /* "load-k-random-a/b", factor m1 (:715-767) */
+#define MLQ_PERT_LOAD_POLY 7        /* "load-random-poly-a/b" (:769-795, 846-871) */
+#define MLQ_PERT_LOAD_EXP 8         /* "load-random-exp-a/b" (:873-925) */
+#define MLQ_PERT_RANDOM_POLY 9      /* "create-random-poly-b" (:797-844) */
+#define MLQ_PERT_RANDOM 10          /* "save-random": the field it saves (:937-972); the
+                                       caller writes the file */
+int mlq_kappa_test_pert(mlq_ctx* ctx, mlq_field out, int kind, double m1, double m2, double m3,
+                        unsigned long long seed, const double* loaded);
+
 /* kappa_test (optimization.f90:957-1129): J0 = eval_J(phi), then for
  * ii = 1..16, eps = 10^(ii-15): kappa = (eval_J(phi + eps pert) - J0) /
  * (eps <grad, pert>_kind).  The perturbation is the caller's (the reference
@@ -224,6 +265,14 @@ typedef struct mlq_rcg_cfg {
   int save_every;        /* save_uvecEveryXiteration (global_variables.f90:70): every this
                             many iterations the solve delivers the iterate as an
                             MLQ_LOG_FIELD record (optimization.f90:388-390); < 1 never */
+  int kappa_every;       /* kappaTestEveryXiteration (global_variables.f90:75): every this
+                            many iterations a kappa test of the L2 gradient
+                            (optimization.f90:147-149); < 1 never */
+  unsigned long long kappa_seed;  /* the kappa tests' perturbation when kappa_field is null:
+                            "create-random-poly-b" (m1 = -5) drawn with this seed */
+  const double* kappa_field;      /* the contents of n<N>_random_b.nc (3 n^3 doubles) for
+                            "load-random-poly-b" (m1 = -5, optimization.f90:1019-1031),
+                            or null */
 } mlq_rcg_cfg;
 /* cfg.flags: run every operator from the field as the reference orders it
  * (no resident state, no tau-polynomial line search); validation path */
@@ -231,6 +280,11 @@ typedef struct mlq_rcg_cfg {
 /* cfg.flags: spectra without the L2 normalisation (normalizeSpectrumByL2Norm
  * = .false., global_variables.f90:76) */
 #define MLQ_RCG_SPECTRA_RAW 2
+/* cfg.flags: kappaTest (global_variables.f90:67): kappa tests of the L2
+ * gradient at the start of the solve (optimization.f90:69-72) and, at its
+ * end, of the H_l gradient (and, if the solve did not converge, of the
+ * direction d1) at uvec0 (:457-469); results as MLQ_LOG_KAPPA records */
+#define MLQ_RCG_KAPPA 4
 
 typedef struct mlq_rcg_result {
   double J1;             /* final R(u) */
@@ -264,6 +318,12 @@ typedef struct mlq_rcg_result {
 #define MLQ_LOG_FIELD 7    /* save_field(Uvec, "u_result") every save_every iterations
                               (optimization.f90:388-390): i0 = iter, d = the iterate, 3 n^3
                               doubles in the reference layout (valid during the call) */
+#define MLQ_LOG_KAPPA 8    /* a kappa test (optimization.f90:957-1129, written by
+                              save_kappa_test, data_ops.f90:1448-1491): i0 = 0
+                              "initial_grad_L2", 1 "gradL2_iter<iter>", 2 "end_gradJ",
+                              3 "end_d1"; i1 = useAdjustedKappaTest; d[0..79] the table
+                              mlq_kappa_test returns, d[80] frac, d[81] best kappa,
+                              d[82] best row */
 typedef void (*mlq_log_cb)(void* user, int kind, int iter, int i0, int i1, const double* d);
 
 int mlq_rcg_solve(mlq_ctx* ctx, mlq_field u, const mlq_rcg_cfg* cfg, mlq_rcg_result* res,

@@ -20,6 +20,11 @@ MLQ_J_MAXDLQDT, MLQ_J_LINEMIN, MLQ_J_LQNORM = range(3)
 MLQ_LOG_ITER, MLQ_LOG_LINEMIN, MLQ_LOG_BRENT, MLQ_LOG_MSG, MLQ_LOG_ERR = 1, 2, 3, 4, 5
 MLQ_LOG_SPECTRUM = 6
 MLQ_LOG_FIELD = 7
+MLQ_LOG_KAPPA = 8
+MLQ_RCG_KAPPA = 4
+MLQ_PERT_SINE, MLQ_PERT_DIVFREE_SINE, MLQ_PERT_LOAD_RAW, MLQ_PERT_LOAD = 0, 1, 2, 3
+MLQ_PERT_LOAD_SMOOTH, MLQ_PERT_LOAD_SMOOTH_RAW, MLQ_PERT_LOAD_K = 4, 5, 6
+MLQ_PERT_LOAD_POLY, MLQ_PERT_LOAD_EXP, MLQ_PERT_RANDOM_POLY, MLQ_PERT_RANDOM = 7, 8, 9, 10
 MLQ_RCG_REFERENCE = 1   # mlq_rcg_cfg.flags: reference-structured iteration
 
 # symbols the header declares (checked by the CPU test suite)
@@ -40,6 +45,7 @@ EXPORTS = [
     "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info", "mlq_mem_live",
     "mlq_transform_counts", "mlq_save_field_nc", "mlq_read_field_nc", "mlq_result_filename",
     "mlq_q_text", "mlq_parse_result_filename", "mlq_fftw_mpi_local_size_3d",
+    "mlq_diagnosticScalars", "mlq_kappa_test_pert",
 ]
 
 
@@ -60,7 +66,9 @@ class Params(C.Structure):
 class RcgCfg(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("optim_tol", C.c_double), ("start_iter", C.c_int),
                 ("batch_linesearch", C.c_int), ("reset_every", C.c_int), ("rolling", C.c_int),
-                ("flags", C.c_int), ("spectra_every", C.c_int), ("save_every", C.c_int)]
+                ("flags", C.c_int), ("spectra_every", C.c_int), ("save_every", C.c_int),
+                ("kappa_every", C.c_int), ("kappa_seed", C.c_ulonglong),
+                ("kappa_field", C.POINTER(C.c_double))]
 
 
 class RcgResult(C.Structure):
@@ -138,6 +146,9 @@ def load(path: str = LIB_PATH):
         "mlq_mem_info": (ci, [vp, dp, dp, dp, dp]),
         "mlq_mem_live": (ci, [vp, dp, dp]),
         "mlq_save_field_nc": (ci, [C.c_char_p, ci, ci, ci, dp]),
+        "mlq_diagnosticScalars": (ci, [vp, dp, dp, dp, dp, C.c_longlong,
+                                       C.POINTER(C.c_longlong)]),
+        "mlq_kappa_test_pert": (ci, [vp, ci, ci, cd, cd, cd, C.c_ulonglong, dp]),
         "mlq_read_field_nc": (ci, [C.c_char_p, ci, ci, ci, dp]),
         "mlq_result_filename": (ci, [C.c_char_p, ci, C.c_char_p, C.c_char_p, cd, ci, ci, ci]),
         "mlq_q_text": (ci, [cd, C.c_char_p, ci]),

@@ -124,6 +124,23 @@ intptr_t mlq_fftw_mpi_local_size_3d(intptr_t n0, intptr_t n1, intptr_t n2, int32
   return n0 * n1 * n2;
 }
 
+int mlq_diagnosticScalars(mlq_ctx* ctx, const double* u, mlq_diag* out, double* fields,
+                          double* ring_xyz, long long ring_cap, long long* ring_points) {
+  return guarded(ctx, [&] {
+    if (!u || !out) throw Error(MLQ_EINVAL, "null argument");
+    IEngine& e = *ctx->eng;
+    e.upload(vec(ctx, 0), u);
+    long long nr = 0;
+    e.diagnostic_scalars(vec(ctx, 0), reinterpret_cast<double*>(out), fields, ring_xyz, ring_cap,
+                         ring_points ? &nr : nullptr);
+    if (ring_points) *ring_points = nr;
+  });
+}
+int mlq_kappa_test_pert(mlq_ctx* ctx, mlq_field out, int kind, double m1, double m2, double m3,
+                        unsigned long long seed, const double* loaded) {
+  return guarded(ctx, [&] { ctx->eng->kappa_test_pert(out, kind, m1, m2, m3, seed, loaded); });
+}
+
 int mlq_sync(mlq_ctx* ctx) { return guarded(ctx, [&] { ctx->eng->sync(); }); }
 
 // ---- fftwfunction ----

@@ -85,6 +85,10 @@ class IEngine {
   virtual void kappa_test(int phi, int grad, int pert, int system, int kind, double* table,
                           double* frac, double* best_kappa, int* best_row) = 0;
   virtual void seeded(int f_vec, unsigned long long seed, int kind, double a) = 0;
+  virtual void diagnostic_scalars(int u_vec, double* out, double* fields, double* ring,
+                                  long long cap, long long* nring) = 0;
+  virtual void kappa_test_pert(int out, int kind, double m1, double m2, double m3,
+                               unsigned long long seed, const double* loaded) = 0;
 
   // whether rcg_solve / eval_J_batch take the resident-state iteration
   virtual bool uses_fast_path() const = 0;

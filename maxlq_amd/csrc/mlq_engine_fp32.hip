@@ -3,6 +3,7 @@
 #include "mlq_ops_core.hpp"
 #include "mlq_rcg.hpp"
 #include "mlq_fast.hpp"
+#include "mlq_diag.hpp"
 
 namespace mlq {
 

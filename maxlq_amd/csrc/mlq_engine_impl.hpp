@@ -154,6 +154,23 @@ template <typename T> class Engine final : public IEngine {
   // rounding; kept for tests of that form)
   bool fused_ = true;
 
+  // diagnosticScalars (function_ops.f90:2503-2602): out[21] in the mlq_diag
+  // layout; fields (4 n^3: |u|, |w|, u.w, Q / max Q) and the ring points
+  // (up to cap, 3 coordinates each; *nring counts them all) when not null
+  void diag_scalars(const T* u, double* out, double* fields, double* ring, long long cap,
+                    long long* nring);
+  void diagnostic_scalars(int u_vec, double* out, double* fields, double* ring, long long cap,
+                          long long* nring) override {
+    if (fld(u_vec).ncomp != 3) throw Error(MLQ_EINVAL, "diagnosticScalars of a vector field");
+    diag_scalars(fphys(u_vec), out, fields, ring, cap, nring);
+  }
+  // kappa_test_pert (function_ops.f90:613-977) into the vector field `out`
+  void kappa_pert(int out, int kind, double m1, double m2, double m3, unsigned long long seed,
+                  const double* loaded);
+  void kappa_test_pert(int out, int kind, double m1, double m2, double m3, unsigned long long seed,
+                       const double* loaded) override {
+    kappa_pert(out, kind, m1, m2, m3, seed, loaded);
+  }
   // The sums of R(phi) at one step size from the generic line-search fields
   // (the |phi|^k chain of calc_uk on D(phi.phi)): out = {sum |phi|^(q-2)
   // |grad phi|^2, sum D(|phi|^(q-4)) w.w, sum D(p |phi|^(q-4)) phi.w,
@@ -319,6 +336,7 @@ template <typename T> class Engine final : public IEngine {
     const bool q4 = std::fabs(p_.q - 4.0) < 1e-14;
     // + 9 fields: the velocity-gradient operand of the deferred state update
     double b = (q4 ? 57.0 : 65.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+    b += 3.0 * kappa_fields_ * real_bytes();
     // FP32 fields, q != 4: the FP64 twin's copy of the 24 line-search fields,
     // its chain temporaries (8 fields) and its transform scratch (12 spectra)
     if (!q4 && sizeof(T) == 4) b += 32.0 * 8.0 * (double)nr_ + 12.0 * 16.0 * (double)n_ * n_ * (n_ / 2 + 8);
@@ -329,6 +347,8 @@ template <typename T> class Engine final : public IEngine {
  private:
 
   template <typename U> friend class FastRcg;
+  template <typename U> friend struct KappaInSolve;
+  int kappa_fields_ = 0;   // real vector fields the solve's kappa tests hold (0 or 4)
   template <typename U>
   friend void rcg_solve_fast(Engine<U>&, int, const mlq_rcg_cfg&, mlq_rcg_result&, const LogSink&);
 };

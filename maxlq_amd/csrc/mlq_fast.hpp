@@ -867,6 +867,8 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   auto msg = [&](int iter, int code) { log.emit(MLQ_LOG_MSG, iter, code, 0, nullptr); };
   e.rescale(u, B);                                                 // :61
   e.divAvg_free(u);                                                // :62
+  KappaInSolve<T> kap(e, cfg, log);
+  kap.initial(cfg.start_iter, u);                                  // :69-72
   double J0 = e.eval_J(u, MLQ_J_MAXDLQDT);                         // :78
   double J1 = J0;
   FastRcg<T> F(e);
@@ -887,10 +889,18 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   std::vector<double> dspec;
   while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
     iter += 1;
+    if (kap.every > 0 && iter % kap.every == 0) {                  // :147-149
+      F.download_u(e.fphys(u));
+      kap.periodic(iter, u);
+    }
     auto dir = F.direction(iter, cfg.reset_every);
     for (int m : dir.msgs) msg(iter, m);
     const double beta = dir.beta;
     F.precompute_ls();
+    if (kap.ends) {
+      auto d1 = F.direction_field();
+      kap.keep_d1(d1);
+    }
     // the spectrum of d1 (:381) is taken while its physical field exists and
     // delivered after the iteration's record
     if (e.spectra_due(cfg, iter)) {
@@ -942,6 +952,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
     }
     // J1 = R(rescale(u + tau1 d1)) = -J_LineMin(tau1)                 (:349)
     J1 = -F.J_value(tau1);
+    kap.keep_u0(F.u_field());
     F.transport_and_update(tau1);
     const double deltaJ = (J1 - J0) / std::fabs(J0);
     if (deltaJ < -MACH_EPSILON) {                                  // :351-365
@@ -976,6 +987,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   const bool ok = (iter < cfg.max_iter) && while_flag < 1;
   if (stop == 0 && !ok) stop = 1;
   msg(iter, 1);
+  if (kap.ends) kap.final(iter, ok, (stop == 2 || stop == 3) ? u : kap.u0);   // :457-469
   res.J1 = J1;
   res.iters = iter;
   res.successful = ok ? 1 : 0;
@@ -1035,6 +1047,10 @@ template <typename T> Engine<double>* Engine<T>::fp64_twin() {
 
 template <typename T>
 void Engine<T>::rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res, const LogSink& log) {
+  // the kappa tests' fields (perturbation and gradient; uvec0 and d1 for the
+  // end tests) count against the resident-state footprint
+  struct Reset { int& k; ~Reset() { k = 0; } } reset{kappa_fields_};
+  kappa_fields_ = (cfg.flags & MLQ_RCG_KAPPA) ? 4 : (cfg.kappa_every > 0 ? 2 : 0);
   if ((cfg.flags & MLQ_RCG_REFERENCE) || !fast_path_ok()) rcg_solve_ref(u, cfg, res, log);
   else rcg_solve_fast(*this, u, cfg, res, log);
 }

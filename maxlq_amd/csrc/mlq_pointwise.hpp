@@ -40,6 +40,7 @@ template <typename T> struct F_copy {   // 16-byte vectors
   }
 };
 template <typename T> struct F_scale { T* o; T s; __device__ void operator()(long long i) const { o[i] *= s; } };
+template <typename T> struct F_divide { T* o; double s; __device__ void operator()(long long i) const { o[i] = (T)((double)o[i] / s); } };
 // o = a*x + b*y
 template <typename T> struct F_axpby {
   const T* x; const T* y; T* o; T a, b;

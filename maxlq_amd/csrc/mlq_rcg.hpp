@@ -12,6 +12,81 @@
 
 namespace mlq {
 
+// Kappa tests inside the solve: kappaTest at the start (gradient in L2 at the
+// rescaled, projected start field, optimization.f90:69-72) and the end
+// (:457-469), kappaTestEveryXiteration inside the loop (:147-149).  The
+// perturbation is made once per solve: "load-random-poly-b" (m1 = -5) of the
+// caller's field, else "create-random-poly-b" (m1 = -5) with the caller's
+// seed (:1019-1031).  At the end the reference tests at uvec0, the iterate
+// before the last update, with gradJ1, the H_l gradient (HilbertGradient and
+// divAvg_free, :146-153) of the last iteration's field, and d1, its last
+// direction: after a failed bracket (:269-290) that field is the current
+// iterate, which has not been updated, while uvec0 is the previous
+// iteration's; before any update uvec0 is the start field here (the
+// reference's array is not yet assigned).
+template <typename T> struct KappaInSolve {
+  enum { INITIAL = 0, PERIODIC = 1, END_GRAD = 2, END_D1 = 3 };
+  Engine<T>& e;
+  const LogSink& log;
+  bool ends = false;
+  int every = 0;
+  int pert = -1, grad = -1, u0 = -1, d1 = -1;
+  KappaInSolve(Engine<T>& eng, const mlq_rcg_cfg& cfg, const LogSink& lg) : e(eng), log(lg) {
+    ends = (cfg.flags & MLQ_RCG_KAPPA) != 0;
+    every = cfg.kappa_every;
+    if (!ends && every < 1) return;
+    pert = e.field_alloc(3);
+    grad = e.field_alloc(3);
+    if (ends) {
+      u0 = e.field_alloc(3);
+      d1 = e.field_alloc(3);
+    }
+    if (cfg.kappa_field) e.kappa_pert(pert, MLQ_PERT_LOAD_POLY, -5.0, 0.0, 0.0, 0, cfg.kappa_field);
+    else e.kappa_pert(pert, MLQ_PERT_RANDOM_POLY, -5.0, 0.0, 0.0, cfg.kappa_seed, nullptr);
+  }
+  ~KappaInSolve() {
+    for (int h : {pert, grad, u0, d1})
+      if (h >= 0) try { e.field_free(h); } catch (...) {}
+  }
+  KappaInSolve(const KappaInSolve&) = delete;
+  KappaInSolve& operator=(const KappaInSolve&) = delete;
+  void run(int iter, int id, bool adjusted, int phi, int g, int ip) {
+    double d[83];
+    double frac = 0.0, best = 0.0;
+    int row = 0;
+    e.kappa_test(phi, g, pert, MLQ_J_MAXDLQDT, ip, d, &frac, &best, &row);
+    d[80] = frac;
+    d[81] = best;
+    d[82] = (double)row;
+    log.emit(MLQ_LOG_KAPPA, iter, id, adjusted ? 1 : 0, d);
+  }
+  // the start field u (rescaled and projected)
+  void initial(int iter, int u) {
+    if (!ends) return;
+    e.grad_L2(u, grad);
+    run(iter, INITIAL, false, u, grad, MLQ_IP_L2);
+    copy(e.fphys(u), u0);
+  }
+  void periodic(int iter, int u) {
+    if (every < 1 || iter % every != 0) return;
+    e.grad_L2(u, grad);
+    run(iter, PERIODIC, false, u, grad, MLQ_IP_L2);
+  }
+  void keep_u0(const T* u) { if (ends) copy(u, u0); }
+  void keep_d1(const T* d) { if (ends) copy(d, d1); }
+  // gfield: the field the last gradient was taken at; converged: the solve
+  // ended successfully
+  void final(int iter, bool converged, int gfield) {
+    if (!ends) return;
+    e.grad_L2(gfield, grad);
+    e.HilbertGradient(grad, 1.5 - 1.0 / e.p_.q);
+    e.divAvg_free(grad);
+    if (!converged) run(iter, END_D1, true, u0, d1, MLQ_IP_HL);
+    run(iter, END_GRAD, true, u0, grad, MLQ_IP_HL);
+  }
+  void copy(const T* src, int h) { e.map(F_copy<T>{src, e.fphys(h)}, 3 * e.nr_); }
+};
+
 template <typename T>
 void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res,
                               const LogSink& log) {
@@ -40,6 +115,8 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
 
   rescale(u, B);                                                    // :61
   divAvg_free(u);                                                   // :62
+  KappaInSolve<T> kap(*this, cfg, log);
+  kap.initial(cfg.start_iter, u);                                   // :69-72
   double J0 = eval_J(u, MLQ_J_MAXDLQDT);                            // :78
   double J1 = J0;
   int iter = cfg.start_iter;
@@ -59,6 +136,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   while (std::fabs(avg) > cfg.optim_tol && iter < cfg.max_iter && while_flag < 1) {
     iter += 1;
     grad_L2(u, g1);                                                 // :146
+    kap.periodic(iter, u);                                          // :147-149
     HilbertGradient(g1, s);                                         // :151
     divAvg_free(g1);                                                // :153
     constraint_derivative(u, nH);                                   // :162
@@ -83,6 +161,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     else beta = beta / g0n;
     if (cfg.reset_every > 0 && iter % cfg.reset_every == 0) beta = 0.0;
     axpby(gu1, 1.0, vd0, beta, d1);
+    kap.keep_d1(fphys(d1));
     double m0[3];
     calc0thFourierModes(d1, m0);                                    // :251-261
     if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
@@ -139,6 +218,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     vector_transport(u, q, B, eta, gu1, vg0);
     vector_transport(u, q, B, eta, d1, vd0);
     // update (:344-350)
+    kap.keep_u0(fphys(u));
     axpby(u, 1.0, d1, tau1, u);
     rescale(u, B);
     J1 = eval_J(u, MLQ_J_MAXDLQDT);
@@ -173,6 +253,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   const bool ok = (iter < cfg.max_iter) && while_flag < 1;          // :451
   if (stop == 0 && !ok) stop = 1;
   msg(iter, 1);
+  if (kap.ends) kap.final(iter, ok, (stop == 2 || stop == 3) ? u : kap.u0);   // :457-469
   res.J1 = J1;
   res.iters = iter;
   res.successful = ok ? 1 : 0;

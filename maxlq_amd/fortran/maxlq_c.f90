@@ -16,8 +16,13 @@ module maxlq_c
    integer(c_int), parameter :: MLQ_J_MAXDLQDT = 0, MLQ_J_LINEMIN = 1, MLQ_J_LQNORM = 2
    integer(c_int), parameter :: MLQ_LOG_ITER = 1, MLQ_LOG_LINEMIN = 2, MLQ_LOG_BRENT = 3, &
                                 MLQ_LOG_MSG = 4, MLQ_LOG_ERR = 5, MLQ_LOG_SPECTRUM = 6, &
-                                MLQ_LOG_FIELD = 7
-   integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1
+                                MLQ_LOG_FIELD = 7, MLQ_LOG_KAPPA = 8
+   integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1, MLQ_RCG_SPECTRA_RAW = 2, MLQ_RCG_KAPPA = 4
+   integer(c_int), parameter :: MLQ_PERT_SINE = 0, MLQ_PERT_DIVFREE_SINE = 1, MLQ_PERT_LOAD_RAW = 2, &
+                                MLQ_PERT_LOAD = 3, MLQ_PERT_LOAD_SMOOTH = 4, &
+                                MLQ_PERT_LOAD_SMOOTH_RAW = 5, MLQ_PERT_LOAD_K = 6, &
+                                MLQ_PERT_LOAD_POLY = 7, MLQ_PERT_LOAD_EXP = 8, &
+                                MLQ_PERT_RANDOM_POLY = 9, MLQ_PERT_RANDOM = 10
 
    type, bind(C) :: mlq_params
       real(c_double) :: q, B, visc, lambda1, viscCoefficient, pressureCoefficient
@@ -28,7 +33,9 @@ module maxlq_c
       integer(c_int) :: max_iter
       real(c_double) :: optim_tol
       integer(c_int) :: start_iter, batch_linesearch, reset_every, rolling, flags, spectra_every
-      integer(c_int) :: save_every
+      integer(c_int) :: save_every, kappa_every
+      integer(c_long_long) :: kappa_seed
+      type(c_ptr) :: kappa_field
    end type mlq_rcg_cfg
 
    type, bind(C) :: mlq_rcg_result
@@ -317,6 +324,28 @@ module maxlq_c
          integer(c_int), value :: phi, grad, pert, mysystem, inner_kind
          real(c_double), intent(out) :: table(5, 16), frac, best_kappa
          integer(c_int), intent(out) :: best_row
+      end function
+      ! diagnosticScalars (function_ops.f90:2503-2602): out(21) = K(3), E(3),
+      ! Umax(3), Wmax(3), vorCore(3), magUmax, magWmax, H, maxHel, minHel, divU_L2
+      integer(c_int) function mlq_diagnosticScalars(ctx, u, out, fields, ring_xyz, ring_cap, &
+            ring_points) bind(C, name="mlq_diagnosticScalars")
+         import :: c_ptr, c_int, c_double, c_long_long
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         real(c_double), intent(out) :: out(21)
+         type(c_ptr), value :: fields, ring_xyz
+         integer(c_long_long), value :: ring_cap
+         integer(c_long_long), intent(out) :: ring_points
+      end function
+      ! kappa_test_pert (function_ops.f90:613-977); loaded may be c_null_ptr
+      integer(c_int) function mlq_kappa_test_pert(ctx, out, kind, m1, m2, m3, seed, loaded) &
+            bind(C, name="mlq_kappa_test_pert")
+         import :: c_ptr, c_int, c_double, c_long_long
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: out, kind
+         real(c_double), value :: m1, m2, m3
+         integer(c_long_long), value :: seed
+         type(c_ptr), value :: loaded
       end function
       integer(c_int) function mlq_rcg_path(ctx, resident_state) bind(C, name="mlq_rcg_path")
          import :: c_ptr, c_int

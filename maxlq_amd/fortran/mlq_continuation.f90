@@ -5,8 +5,14 @@
 !      iteration-info-B<bbb>.dat         save_diagnostics_optim (data_ops.f90:764-799)
 !   <home>ncFiles/u_result_q<q>_n<n>_B<bbb>_iter<iiiii|end>.nc
 !                                        save_field (data_ops.f90:9-41; optimization.f90:388-390, 477)
+!   <home>constraintDirs/B<bbb>-<B>/kappaTest/kappa_B<bbb>_<name>.dat
+!                                        save_kappa_test (data_ops.f90:1448-1491), with kappa=
 ! Usage:
-!   mlq_continuation <home/> <n> <q> <nB> <max_iter> <optim_tol> <fp64|mixed> <save_every> [<file>]
+!   mlq_continuation <home/> <n> <q> <nB> <max_iter> <optim_tol> <fp64|mixed> <save_every>
+!                    [kappa=<every>] [<file>]
+! kappa=<every> turns kappaTest on (tests at the start and end of every solve,
+! optimization.f90:69-72, 457-469) with kappaTestEveryXiteration = <every>
+! (< 1 never, :147-149); the perturbation is create-random-poly-b with seed 1.
 ! Without <file> the first B solve starts from the seeded divergence-free field
 ! (seed 1, exp(-|m|^2/16)).  With <file> (a u_result name) q, n, bIterOffset
 ! and optimizationIterOffset come from its name (function_ops.f90:4635-4674)
@@ -56,6 +62,9 @@ contains
          open(10, file=fname, form="FORMATTED", position="APPEND")
          write(10, "(I5.4, 3 ES20.12)") i0, dd(1), dd(2), dd(3)
          close(10)
+      case (MLQ_LOG_KAPPA)
+         call c_f_pointer(d, dd, [83])
+         call save_kappa(iter, i0, i1 /= 0, dd)
       case (MLQ_LOG_FIELD)
          call c_f_pointer(d, dd, [3 * grid_n**3])
          fname = field_file(i0)
@@ -65,6 +74,56 @@ contains
          end if
       end select
    end subroutine on_log
+
+   ! save_kappa_test for the 16 rows of one kappa test (data_ops.f90:1448-1491;
+   ! without the adjusted columns the first row is written twice, as there)
+   subroutine save_kappa(iter, id, adjusted, d)
+      integer(c_int), intent(in) :: iter, id
+      logical, intent(in) :: adjusted
+      real(c_double), intent(in) :: d(83)
+      character(len=:), allocatable :: kdir, fname, ident
+      character(len=5) :: itxt
+      integer :: ii
+      real(c_double) :: eps, ip, dJ, kap, kadj
+      select case (id)
+      case (0); ident = "initial_grad_L2"
+      case (1)
+         write(itxt, "(i5.5)") iter
+         ident = "gradL2_iter" // itxt
+      case (2); ident = "end_gradJ"
+      case default; ident = "end_d1"
+      end select
+      kdir = constraint_dir // "kappaTest/"
+      call make_dir(kdir)
+      fname = kdir // "kappa" // "_B" // trim(bIterTxt) // "_" // ident // ".dat"
+      do ii = 1, 16
+         eps = d((ii - 1) * 5 + 1); ip = d((ii - 1) * 5 + 2); dJ = d((ii - 1) * 5 + 3)
+         kap = d((ii - 1) * 5 + 4); kadj = d((ii - 1) * 5 + 5)
+         if (adjusted) then
+            if (ii == 1) then
+               open(10, file=fname, form="FORMATTED", status="REPLACE")
+               write(10, "(9 G20.12)") "eps", "deltaJ", "deltaJ/eps", "inner_prod", "kappa_adj", &
+                  "LOG10|kap_adj-1|", "kappa", "LOG10|kap-1|", "adj_factor"
+               close(10)
+            end if
+            open(10, file=fname, form="FORMATTED", status="OLD", position="APPEND")
+            write(10, "(9 ES20.12)") eps, dJ, dJ / eps, ip, kadj, log10(abs(kadj - 1.0_c_double)), &
+               kap, log10(abs(kap - 1.0_c_double)), d(81)
+            close(10)
+         else
+            if (ii == 1) then
+               open(10, file=fname, form="FORMATTED", status="REPLACE")
+               write(10, "(6 G20.12)") "eps", "deltaJ", "deltaJ/eps", "inner_prod", "kappa", &
+                  "LOG10|kap-1|"
+               write(10, "(6 ES20.12)") eps, dJ, dJ / eps, ip, kap, log10(abs(kap - 1.0_c_double))
+               close(10)
+            end if
+            open(10, file=fname, form="FORMATTED", status="OLD", position="APPEND")
+            write(10, "(6 ES20.12)") eps, dJ, dJ / eps, ip, kap, log10(abs(kap - 1.0_c_double))
+            close(10)
+         end if
+      end do
+   end subroutine save_kappa
 end module mlq_cont_io
 
 program mlq_continuation
@@ -74,7 +133,8 @@ program mlq_continuation
    implicit none
    character(len=4096) :: arg, resume
    integer(c_int) :: n, nB, max_iter, save_every, prec, rc, h, bIterOffset, optimIterOffset, j, jend
-   integer(c_int) :: resol, iterOut
+   integer(c_int) :: resol, iterOut, kappa_every, ia
+   logical :: kappa_on
    real(c_double) :: q, optim_tol, B
    real(c_double), allocatable :: B_list(:), u(:,:,:,:)
    type(c_ptr) :: ctx
@@ -87,7 +147,7 @@ program mlq_continuation
 
    if (command_argument_count() < 8) then
       print*, "usage: mlq_continuation <home/> <n> <q> <nB> <max_iter> <optim_tol> <fp64|mixed>", &
-              " <save_every> [<u_result file>]"
+              " <save_every> [kappa=<every>] [<u_result file>]"
       stop 2
    end if
    call get_command_argument(1, arg); home_dir = trim(arg)
@@ -101,8 +161,18 @@ program mlq_continuation
    bIterOffset = 0
    optimIterOffset = 0
    resume = ""
-   if (command_argument_count() >= 9) then
-      call get_command_argument(9, resume)
+   kappa_on = .false.
+   kappa_every = 0
+   do ia = 9, command_argument_count()
+      call get_command_argument(ia, arg)
+      if (arg(1:6) == "kappa=") then
+         kappa_on = .true.
+         read(arg(7:), *) kappa_every
+      else
+         resume = arg
+      end if
+   end do
+   if (len_trim(resume) > 0) then
       rc = mlq_parse_result_filename(cstr(resume), q, resol, bIterOffset, optimIterOffset)
       if (rc /= MLQ_OK) stop 3
       n = resol
@@ -159,8 +229,9 @@ program mlq_continuation
       call make_dir(constraint_dir)
       p%B = B
       if (mlq_set_params(ctx, p) /= MLQ_OK) stop 6
-      cfg = mlq_rcg_cfg(max_iter, optim_tol, 0, merge(3, 0, prec == MLQ_MIXED), 25, 10, 0, 0, &
-                        save_every)
+      cfg = mlq_rcg_cfg(max_iter, optim_tol, 0, merge(3, 0, prec == MLQ_MIXED), 25, 10, &
+                        merge(MLQ_RCG_KAPPA, 0, kappa_on), 0, save_every, kappa_every, &
+                        1_c_long_long, c_null_ptr)
       if (j == bIterOffset + 1) cfg%start_iter = optimIterOffset    ! optimization.f90:81-86
       rc = mlq_rcg_solve(ctx, h, cfg, res, c_funloc(on_log), c_null_ptr)
       if (rc /= MLQ_OK) then

@@ -73,7 +73,7 @@ program mlq_fsmoke
    end do
    rc = mlq_field_alloc(ctx, 3, h)
    rc = mlq_upload(ctx, h, u)
-   cfg = mlq_rcg_cfg(2, 1.0e-5_c_double, 0, 0, 25, 10, 0, 0, 0)
+   cfg = mlq_rcg_cfg(2, 1.0e-5_c_double, 0, 0, 25, 10, 0, 0, 0, 0, 0_c_long_long, c_null_ptr)
    rc = mlq_rcg_solve(ctx, h, cfg, res, c_null_funptr, c_null_ptr)
    if (rc /= MLQ_OK .or. res%iters < 1) then
       print*, "rcg solve rc", rc, "iters", res%iters

@@ -42,6 +42,11 @@ class RcgTrace:
     errors: list = field(default_factory=list)           # (iter, code)
     spectra: list = field(default_factory=list)          # (iter, "uvec" | "d", (n, 2) array)
     fields: list = field(default_factory=list)           # (iter, (3, n, n, n) array): save_every
+    kappa: list = field(default_factory=list)            # (iter, name, adjusted, table, frac,
+                                                         #  best kappa, best row)
+
+
+KAPPA_NAMES = ("initial_grad_L2", "gradL2_iter", "end_gradJ", "end_d1")
 
 
 class MaxLq:
@@ -305,6 +310,35 @@ class MaxLq:
                                                _p(J), _p(t)))
         return J, t[0], t[1]
 
+    def diagnosticScalars(self, u, fields=False, ring_cap=0):
+        """diagnosticScalars (function_ops.f90:2503-2602): dict of the values
+        save_diagnosticFields_global writes; with fields, also the (4, n, n, n)
+        fields magU, magW, Helicity, VortexCore; with ring_cap, the ring points."""
+        u = _f64(u, self.vshape)
+        out = np.zeros(21)
+        fl = np.zeros((4,) + self.shape) if fields else None
+        ring = np.zeros((max(ring_cap, 1), 3))
+        cnt = C.c_longlong()
+        self._chk(self.lib.mlq_diagnosticScalars(self.ctx, _p(u), _p(out),
+                                                 _p(fl) if fields else None,
+                                                 _p(ring) if ring_cap else None, int(ring_cap),
+                                                 C.byref(cnt)))
+        d = {"K": out[0:3], "E": out[3:6], "Umax": out[6:9], "Wmax": out[9:12],
+             "vorCore": out[12:15], "magUmax": out[15], "magWmax": out[16], "H": out[17],
+             "maxHel": out[18], "minHel": out[19], "divU_L2": out[20], "ring_points": cnt.value}
+        if fields:
+            d["fields"] = fl
+        if ring_cap:
+            d["ring"] = ring[:min(ring_cap, cnt.value)]
+        return d
+
+    def kappa_test_pert(self, h, kind, m1=0.0, m2=0.0, m3=0.0, seed=0, loaded=None):
+        """kappa_test_pert (function_ops.f90:613-977) into device field h."""
+        ld = None if loaded is None else _f64(loaded, self.vshape)
+        self._chk(self.lib.mlq_kappa_test_pert(self.ctx, h, int(kind), float(m1), float(m2),
+                                               float(m3), C.c_ulonglong(seed),
+                                               None if ld is None else _p(ld)))
+
     def kappa_test(self, hphi, hgrad, hpert, mysystem="maxdLqdt", inner_product_space="L2"):
         """kappa_test (optimization.f90:957-1129) on device fields; returns
         (table[16][5], frac, best_kappa, best_row)."""
@@ -317,14 +351,21 @@ class MaxLq:
         return tab.reshape(16, 5), frac.value, best.value, row.value
 
     def maxdLqdt(self, hu, max_iter=99999, optim_tol=None, start_iter=0, batch_linesearch=0,
-                 reset_every=25, rolling=10, trace=True, spectra_every=0, flags=0, save_every=0):
+                 reset_every=25, rolling=10, trace=True, spectra_every=0, flags=0, save_every=0,
+                 kappa=False, kappa_every=0, kappa_seed=0, kappa_field=None):
         """One RCG solve for the current B on the device-resident field ``hu``
-        (optimization.f90:7-514).  Returns (RcgResult, RcgTrace)."""
+        (optimization.f90:7-514).  Returns (RcgResult, RcgTrace).  kappa:
+        kappaTest (tests at the start and end); kappa_every:
+        kappaTestEveryXiteration; the perturbation is load-random-poly-b of
+        kappa_field when given, else create-random-poly-b with kappa_seed."""
         if optim_tol is None:
             optim_tol = 1.0e-4 if self.n >= 1024 else 1.0e-5       # initialize.f90:95-100
+        kf = None if kappa_field is None else _f64(kappa_field, self.vshape)
         cfg = capi.RcgCfg(int(max_iter), float(optim_tol), int(start_iter), int(batch_linesearch),
-                          int(reset_every), int(rolling), int(flags), int(spectra_every),
-                          int(save_every))
+                          int(reset_every), int(rolling),
+                          int(flags) | (capi.MLQ_RCG_KAPPA if kappa else 0), int(spectra_every),
+                          int(save_every), int(kappa_every), C.c_ulonglong(kappa_seed),
+                          None if kf is None else _p(kf))
         res = capi.RcgResult()
         tr = RcgTrace()
 
@@ -342,6 +383,10 @@ class MaxLq:
             elif kind == capi.MLQ_LOG_FIELD:
                 a = np.ctypeslib.as_array(d, shape=(3 * self.n ** 3,)).copy()
                 tr.fields.append((i0, a.reshape(self.vshape)))
+            elif kind == capi.MLQ_LOG_KAPPA:
+                a = np.ctypeslib.as_array(d, shape=(83,)).copy()
+                tr.kappa.append((it, KAPPA_NAMES[i0], bool(i1), a[:80].reshape(16, 5), a[80],
+                                 a[81], int(a[82])))
             elif kind == capi.MLQ_LOG_SPECTRUM:
                 sp = np.ctypeslib.as_array(d, shape=(2 * self.n,)).copy()
                 tr.spectra.append((it, "uvec" if i0 == 0 else "d", sp.reshape(2, self.n).T.copy()))

@@ -117,3 +117,38 @@ def test_continuation_driver_and_resume(tmp_path):
 def test_dropin_modules_with_reference_globals():
     out = _run([_exe("mlq_dropin_check")], None)
     assert "PASS mlq_dropin_check" in out
+
+
+def test_continuation_driver_kappa_files(tmp_path):
+    """kappa=2: save_kappa_test's files (data_ops.f90:1448-1491) for one B
+    solve, against the kappa records of the same solve driven from Python."""
+    from maxlq_amd.lq import MaxLq
+    exe = _exe("mlq_continuation")
+    home = str(tmp_path / "output") + "/"
+    _run([exe, home, str(N), "4", "1", "4", "0.0", "fp64", "0", "kappa=2"], str(tmp_path))
+    g = MaxLq(N, q=Q, B=_B(1))
+    h = g.field(3)
+    g.seeded_field(h, 1, "gauss", 16.0)
+    res, tr = g.maxdLqdt(h, max_iter=4, optim_tol=0.0, kappa=True, kappa_every=2, kappa_seed=1)
+    g.close()
+    kdir = os.path.join(home, "constraintDirs", f"B001-{_B(1):.1E}", "kappaTest")
+    assert sorted(os.listdir(kdir)) == sorted(
+        ["kappa_B001_initial_grad_L2.dat", "kappa_B001_gradL2_iter00002.dat",
+         "kappa_B001_gradL2_iter00004.dat", "kappa_B001_end_d1.dat", "kappa_B001_end_gradJ.dat"])
+    for it, name, adjusted, tab, frac, best, row in tr.kappa:
+        ident = f"gradL2_iter{it:05d}" if name == "gradL2_iter" else name
+        rows = open(os.path.join(kdir, f"kappa_B001_{ident}.dat")).read().splitlines()
+        vals = np.array([[float(x) for x in r.split()] for r in rows[1:]])
+        if adjusted:
+            assert rows[0].split()[:5] == ["eps", "deltaJ", "deltaJ/eps", "inner_prod", "kappa_adj"]
+            assert vals.shape == (16, 9)
+            np.testing.assert_allclose(vals[:, 4], tab[:, 4], rtol=1e-11)
+            np.testing.assert_allclose(vals[:, 8], frac, rtol=1e-11)
+        else:
+            assert rows[0].split() == ["eps", "deltaJ", "deltaJ/eps", "inner_prod", "kappa",
+                                       "LOG10|kap-1|"]
+            vals = vals[1:]          # the first row is written twice
+            assert vals.shape == (16, 6)
+            np.testing.assert_allclose(vals[:, 4], tab[:, 3], rtol=1e-11)
+        np.testing.assert_allclose(vals[:, 0], tab[:, 0], rtol=1e-11)
+        np.testing.assert_allclose(vals[:, 3], tab[:, 1], rtol=1e-11)
