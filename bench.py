@@ -70,6 +70,32 @@ def run_rcg(n, q, B, precision, batch, warmup, steps, seed=4, a=256.0, flags=0):
     return g, h, res, t_it, t1 - t0
 
 
+def iteration_algorithmic_bytes(g, h, batch=3):
+    """Algorithmic bytes of one RCG iteration by kernel class: the library's
+    per-launch count (one read of every input element a kernel needs, one
+    write of every output element, pruned passes counted on their band),
+    summed over the launches between two iteration records of a 2-iteration
+    solve continuing from the field in h (profiling on, so not timed)."""
+    import ctypes as C
+    from maxlq_amd import capi
+    snaps = []
+
+    def cb(user, kind, it, i0, i1, d):
+        if kind == capi.MLQ_LOG_ITER:
+            snaps.append(g.prof())
+
+    g.prof_enable(True)
+    cfg = capi.RcgCfg(2, 0.0, 0, int(batch), 25, 10, 0)
+    res = capi.RcgResult()
+    g._chk(g.lib.mlq_rcg_solve(g.ctx, h, C.byref(cfg), C.byref(res), capi.LOG_CB(cb), None))
+    g.prof_enable(False)
+    if len(snaps) < 3:
+        return None, None
+    a, b = snaps[-2], snaps[-1]
+    by = {k: b[k][2] - a[k][2] for k in b if b[k][2] - a[k][2] > 0}
+    return sum(by.values()), by
+
+
 def fft_roofline(g, batch=12, reps=5):
     """Per-pass average launch durations (HIP events on the library stream)
     and algorithmic bytes (input + output buffer bytes per launch)."""
@@ -149,21 +175,46 @@ def iteration_traffic(n):
     return None
 
 
-def cpu_baseline(q, B, threads):
-    """Bounded sample of the reference-structured FP64 CPU path (the oracle):
-    one GradL2ForLq and one eval_J("LineMin") at 256^3, scaled to one 512^3
-    iteration with the reference's transform counts at q = 4 (548 C2C
-    transforms per iteration outside the line search, 94 per J evaluation)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """The host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the machine's share for this job)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n
+
+
+def cpu_baseline(n, q, B, threads, iters=2):
+    """The reference-structured FP64 RCG on the host (the oracle's maxdLqdt,
+    optimization.f90:7-514: gradient, Hilbert gradient, Leray projection,
+    normal, CG direction, mnbrak + brent with its own J evaluations, vector
+    transport, retraction), `iters` iterations on the bench's field at n^3
+    (same seed, envelope scaled to the grid); the last iteration is timed,
+    as the GPU line times iterations after a warm-up one."""
     from oracle import lq_oracle as L
     from oracle.seeded import seeded_field
     L.set_threads(threads)
-    n = 256
-    u = seeded_field(n, 4, "gauss", 256.0 / 4.0, q=q, B=B)
-    orc = L.Oracle(L.Params(n=n, q=q, B=B))
+    u = seeded_field(n, 4, "gauss", 256.0 * (n / 512.0) ** 2, q=q, B=B)
+    orc = L.Oracle(L.Params(n=n, q=q, B=B, MAX_ITER=iters, OPTIM_TOL=0.0))
+    stamps = []
     t0 = time.perf_counter()
-    orc.eval_J(u, "LineMin")
-    tJ = time.perf_counter() - t0
-    return tJ
+    res = L.maxdLqdt(orc, u, on_start=lambda: stamps.append(time.perf_counter()),
+                     on_iter=lambda rec, uu: stamps.append(time.perf_counter()))
+    total = time.perf_counter() - t0
+    if len(stamps) != iters + 1:
+        raise RuntimeError(f"oracle solve stopped early ({res.stop_reason})")
+    return stamps[-1] - stamps[-2], res.records[-1].nJ, total
 
 
 def bench_linesearch8(args):
@@ -331,6 +382,8 @@ def main():
     ap.add_argument("--B", type=float, default=2.0)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=96,
+                    help="grid of the timed host oracle iteration (cpu_baseline)")
     ap.add_argument("--fft-only", choices=["mixed", "fp64"], default=None,
                     help="only the batched transform sweep and a copy kernel (for profilers)")
     ap.add_argument("--case", choices=["iteration", "linesearch8", "continuation", "n1024",
@@ -369,6 +422,8 @@ def main():
     nJm = resm.n_J / max(resm.iters, 1)
     log(f"mixed: {resm.n_J} J requested, {resm.n_J_computed} computed, {resm.n_J_poly} by the "
         f"line-search polynomials")
+    alg_bytes, alg_by_class = iteration_algorithmic_bytes(gm, hm)
+    log(f"algorithmic bytes per iteration: {alg_bytes / 1e9:.1f} GB")
     mem_used, mem_peak, _, _ = gm.mem_info()
     fft_passes, fft_gbs, fft_strict = fft_roofline(gm)
     log(f"fft passes: " + ", ".join(f"{k}={v['GBs']:.0f}GB/s" for k, v in fft_passes.items()))
@@ -413,14 +468,20 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        thr = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline (oracle) with {thr} threads")
-        tJ = cpu_baseline(q, B, thr)
-        scale = 8.0 * (27.0 / 24.0)                     # (512/256)^3 * log ratio
-        t_iter_cpu = (548.0 / 94.0 + nJm) * tJ * scale
-        cpu = {"value": 1.0 / t_iter_cpu, "unit": "iter/s", "cores": thr, "kind": "port",
-               "sample": f"oracle eval_J(LineMin) at 256^3 took {tJ:.2f}s; scaled x{scale:.2f} "
-                         f"to 512^3 and x(548/94 + n_J={nJm:.1f}) J-equivalents per iteration"}
+        thr = host_threads()
+        nc = args.cpu_n
+        log(f"cpu baseline: oracle RCG at {nc}^3 with {thr} threads")
+        t_cpu, nJ_cpu, t_total = cpu_baseline(nc, q, B, thr)
+        # n^3 log n work scaling of the transforms and pointwise passes
+        scale = (n / nc) ** 3 * math.log(n) / math.log(nc)
+        cpu = {"value": 1.0 / t_cpu, "unit": f"iter/s at {nc}^3", "cores": thr, "kind": "port",
+               "sample": f"oracle maxdLqdt (numpy FP64, reference-structured) on the bench field "
+                         f"at {nc}^3: 2 iterations in {t_total:.1f} s, the second timed "
+                         f"({t_cpu:.2f} s, its own line search: n_J = {nJ_cpu}); transforms on "
+                         f"{thr} threads (scipy.fft workers), pointwise work on one",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "extrapolated_iter_per_s_at_bench_n": 1.0 / (t_cpu * scale),
+               "extrapolation": f"x(n/{nc})^3 log(n)/log({nc}) = x{scale:.0f} to {n}^3"}
 
     out = {
         "metric": "RCG iter/s at 512^3 q=4: mixed vs FP64 (|dR|/R<1e-7); FFT HBM GB/s",
@@ -448,6 +509,13 @@ def main():
                    "copy_12_fields_GBs": round(copy_gbs, 1),
                    "column_copy_12_spectra_GBs": colcopy,
                    "iteration_hbm_bytes": iteration_traffic(n),
+                   "iteration_algorithmic_bytes": alg_bytes,
+                   "iteration_algorithmic_bytes_by_class": alg_by_class,
+                   "iteration_hbm_over_algorithmic": (round(iteration_traffic(n) / alg_bytes, 3)
+                                                      if iteration_traffic(n) and alg_bytes
+                                                      else None),
+                   "iteration_algorithmic_GBs": (round(alg_bytes / t_mixed / 1e9, 1)
+                                                 if alg_bytes else None),
                    "iteration_hbm_GBs": (round(iteration_traffic(n) / t_mixed / 1e9, 1)
                                          if iteration_traffic(n) else None),
                    "transforms_per_iter_mixed": {"forward": gm.tr_per_iter[0],

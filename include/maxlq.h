@@ -305,7 +305,10 @@ typedef struct mlq_rcg_result {
 } mlq_rcg_result;
 
 /* log record kinds delivered to the callback (data_ops.f90 writers) */
-#define MLQ_LOG_ITER 1     /* save_diagnostics_optim: i0=iter, d[0..2]=tau,beta,J (:764-799) */
+#define MLQ_LOG_ITER 1     /* save_diagnostics_optim: i0=iter, d[0..2]=tau,beta,J (:764-799);
+                              d[3..4] = tau_brack (the t_bra(a), t_bra(b) of the iteration's
+                              terminal line, optimization.f90:420), d[5] = J requests of its
+                              mnbrak and brent */
 #define MLQ_LOG_LINEMIN 2  /* save_linemin_data: i0=brakIter, i1=0 replace/1 append,
                               d[0..5]=tA,tB,tC,FA,FB,FC (:1403-1442) */
 #define MLQ_LOG_BRENT 3    /* brent-info: i0=j, d[0..1]=X,FX (optimization.f90:836-946) */

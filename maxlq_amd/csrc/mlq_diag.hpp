@@ -177,7 +177,7 @@ void Engine<T>::diag_scalars(const T* u, double* out, double* fields, double* ri
   hipLaunchKernelGGL((k_diag<T>), dim3(DIAG_BLOCKS), dim3(DIAG_THREADS), 0, st_, u, Gb.get(), nr_,
                      part);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_REDUCE, 0);
+  prof_end(PC_REDUCE, 12.0 * real_bytes());
   std::vector<double> h(DIAG_BLOCKS * DIAG_K);
   MLQ_HIP(hipMemcpyAsync(h.data(), part, pbytes, hipMemcpyDeviceToHost, st_));
   sync();

@@ -239,6 +239,7 @@ template <typename T> void Engine<T>::prof_get(int cls, long long* count, double
 template <typename T> template <class F> void Engine<T>::map(const F& f, long long n, int cls) {
   constexpr int vec = VecW<F>::v;
   if (n % vec) throw Error(MLQ_EINVAL, "map length not a multiple of its vector width");
+  const double ab = (double)n * AlgB<F>::of(f);
   n /= vec;
   if (n <= 0) return;
   // one call per thread (the grid-stride loop only runs past 2^24
@@ -249,13 +250,14 @@ template <typename T> template <class F> void Engine<T>::map(const F& f, long lo
   prof_begin(cls);
   hipLaunchKernelGGL((k_map<F>), dim3((unsigned)blocks), dim3(MAP_THREADS), 0, st_, f, n);
   MLQ_HIP(hipGetLastError());
-  prof_end(cls, 0);
+  prof_end(cls, ab);
 }
 
 template <typename T> template <int K, class F>
 void Engine<T>::reduce(const F& f, long long n, double* out) {
   constexpr int vec = VecW<F>::v;
   if (n % vec) throw Error(MLQ_EINVAL, "reduction length not a multiple of its vector width");
+  const double ab = (double)n * AlgB<F>::of(f);
   n /= vec;
   prof_begin(PC_REDUCE);
   const int nblk = (int)red_blocks(n);
@@ -264,7 +266,7 @@ void Engine<T>::reduce(const F& f, long long n, double* out) {
   hipLaunchKernelGGL(k_reduce_final, dim3(K), dim3(RED_THREADS), 0, st_, part_, nblk, K,
                      red_dev_);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_REDUCE, 0);
+  prof_end(PC_REDUCE, ab);
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * K, hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));
   for (int k = 0; k < K; ++k) out[k] = red_host_[k];
@@ -393,7 +395,7 @@ void Engine<T>::fwd_n(int nb, const T* const* src, C* const* dst, const SpecOp* 
   else
     hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_XFWD, nb * (real_bytes() + spec_bytes()));
+  prof_end(PC_XFWD, nb * (real_bytes() + ab_xcols(keep)));
   fwd_yz_n<N>(nb, dst, ops, keep);
 }
 
@@ -411,13 +413,13 @@ void Engine<T>::fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep, boo
   prof_begin(PC_YFWD);
   launch_strided<T, N, -1, 0>(svar_[keep >= 0 ? 4 : 0], st_, sa, keep >= 0 ? keep + 1 : PX, N * nb);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_YFWD, 2 * sb);
+  prof_end(PC_YFWD, nb * (ab_xcols(keep) + ab_kcols(keep)));
   for (int b = 0; b < nb; ++b) { sa.dst[b] = dst[b]; sa.op[b] = ops[b]; }
   sa.kzero = kept_only ? 0 : 1;
   prof_begin(PC_ZFWD);
   launch_strided<T, N, -1, 2>(svar_[keep >= 0 ? 5 : 1], st_, sa, N * PP, nb);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_ZFWD, 2 * sb);
+  prof_end(PC_ZFWD, nb * (ab_kcols(keep) + (kept_only ? ab_cube(keep) : spec_bytes())));
 }
 
 template <typename T> template <int N, class P>
@@ -442,7 +444,7 @@ void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool
     else
       hipLaunchKernelGGL((k_xfwd_p<T, N, false, P>), dim3(pairs), dim3(P::NB * Tf), 0, st_, prod, xa);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_XFWD, P::NB * spec_bytes());
+    prof_end(PC_XFWD, (double)nr_ * ProdB<P>::of(prod) + P::NB * ab_xcols(keep));
     fwd_yz_n<N>(P::NB, dst, ops, keep, true);
     nfwd_ += P::NB;
     done = true;
@@ -531,7 +533,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   prof_begin(PC_ZINV);
   launch_strided<T, N, 1, 1>(svar_[keep >= 0 ? 6 : 2], st_, sa, N * PP, nz);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_ZINV, nz * sb + zin * spec_bytes());
+  prof_end(PC_ZINV, zin * ab_cube(keep) + nz * ab_kcols(keep));
   prof_begin(PC_YINV);
   for (int pass = 0; pass < 2; ++pass) {   // out of place first, then in place
     SArgs<T> ya = sa;
@@ -549,7 +551,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
                                N * nf);
     MLQ_HIP(hipGetLastError());
   }
-  prof_end(PC_YINV, 2 * ny * sb);
+  prof_end(PC_YINV, ny * (ab_kcols(keep) + ab_xcols(keep)));
   XArgs<T> xa{};
   for (int b = 0; b < nb; ++b) {
     xa.rdst[b] = dst[b];
@@ -573,7 +575,7 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   else
     hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0, st_, xa);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_XINV, rb + nb * sb);
+  prof_end(PC_XINV, rb + nb * ab_xcols(keep));
 }
 
 // Slot plan of a round-trip group whose inputs sit in scratch slots
@@ -676,7 +678,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
       hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
                          st_, xa);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_XFWD, nin * (real_bytes() + sb));
+    prof_end(PC_XFWD, nin * (real_bytes() + ab_xcols(keep)));
     SArgs<T> sa{};
     sa.spec = scratch_;
     sa.tw = tw_;
@@ -686,7 +688,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
     launch_strided<T, N, -1, 0>(svar_[keep >= 0 ? 4 : 0], st_, sa, keep >= 0 ? keep + 1 : PX,
                                 N * nin);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_YFWD, 2 * nin * sb);
+    prof_end(PC_YFWD, nin * (ab_xcols(keep) + ab_kcols(keep)));
   }
   nfwd_ += nin;
   // fused z round trips
@@ -707,7 +709,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
     hipLaunchKernelGGL((k_zrt<T, N, E, CZ>), dim3((ncols + CZ - 1) / CZ, pl.nz), dim3(CZ * (N / E)),
                        0, st_, za, ncols);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_ZRT, 2 * pl.nz * sb);
+    prof_end(PC_ZRT, 2 * pl.nz * ab_kcols(keep));
   }
   // inverse y passes (out of place first: they read slots the in-place ones rewrite)
   {
@@ -733,7 +735,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
                                  N * nf);
       MLQ_HIP(hipGetLastError());
     }
-    prof_end(PC_YINV, 2 * pl.ny * sb);
+    prof_end(PC_YINV, pl.ny * (ab_kcols(keep) + ab_xcols(keep)));
   }
   // inverse x pass (1/N^3 folded in)
   {
@@ -762,7 +764,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
       hipLaunchKernelGGL((k_xinv<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
                          st_, xa);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_XINV, nout * (real_bytes() + sb));
+    prof_end(PC_XINV, nout * (real_bytes() + ab_xcols(keep)));
   }
   ninv_ += nout;
 }

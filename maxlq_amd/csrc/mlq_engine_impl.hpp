@@ -68,6 +68,16 @@ template <typename T> class Engine final : public IEngine {
   double real_bytes() const override { return (double)nr_ * sizeof(T); }
   // algorithmic bytes of one half spectrum (the row padding is never touched)
   double spec_bytes() const override { return (double)n_ * n_ * px_ * sizeof(C); }
+  // algorithmic bytes of the transform passes under a band max|m_j| <= keep
+  // (keep < 0: none): kx columns and ky (or kz) rows a pruned pass touches
+  double ab_cols(int keep) const { return keep >= 0 ? keep + 1.0 : (double)px_; }
+  double ab_rows(int keep) const { return keep >= 0 ? 2.0 * keep + 1.0 : (double)n_; }
+  // one x-y plane row of columns, the kept (kx, ky) columns, the kept cube
+  double ab_xcols(int keep) const { return ab_cols(keep) * n_ * n_ * sizeof(C); }
+  double ab_kcols(int keep) const { return ab_cols(keep) * ab_rows(keep) * n_ * sizeof(C); }
+  double ab_cube(int keep) const {
+    return ab_cols(keep) * ab_rows(keep) * ab_rows(keep) * sizeof(C);
+  }
 
   int field_alloc(int ncomp) override;
   void field_free(int h) override;

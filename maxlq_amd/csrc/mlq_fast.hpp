@@ -882,7 +882,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
   double avg = 1.0;
   int while_flag = 0, stop = 0;
   {
-    double d[3] = {0.0, 0.0, J0};
+    double d[6] = {0.0, 0.0, J0, 0.0, 0.0, 0.0};
     log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
   }
   if (e.spectra_due(cfg, iter)) e.emit_spectrum(log, cfg, iter, 0, e.fphys(u));   // :119-121
@@ -922,11 +922,12 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
       log.emit(MLQ_LOG_BRENT, iter, j, 0, d);
     };
     msg(iter, 20);
+    const long long nJ_it = F.nJ_;
     BracketResult br = mnbrak(J, 0.0, tau0, llog);                 // :269
     if (br.flag != 0) {
       if (br.flag == 1) msg(iter, 22);
       log.emit(MLQ_LOG_ERR, iter, br.flag, 0, nullptr);
-      double d[3] = {tau1, beta, J0};
+      double d[6] = {tau1, beta, J0, 0.0, 0.0, (double)(F.nJ_ - nJ_it)};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
       while_flag = 1;
       stop = br.flag == 1 ? 2 : 3;
@@ -950,13 +951,15 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
     } else if (tau1 > std::sqrt(TAU_MAX)) {
       msg(iter, 41);
     }
+    // J requests of mnbrak and brent (J_value below is not one)
+    const long long nJ_ls = F.nJ_ - nJ_it;
     // J1 = R(rescale(u + tau1 d1)) = -J_LineMin(tau1)                 (:349)
     J1 = -F.J_value(tau1);
     kap.keep_u0(F.u_field());
     F.transport_and_update(tau1);
     const double deltaJ = (J1 - J0) / std::fabs(J0);
     if (deltaJ < -MACH_EPSILON) {                                  // :351-365
-      double d[3] = {tau1, beta, J0};
+      double d[6] = {tau1, beta, J0, br.tA, br.tC, (double)(nJ_ls)};
       log.emit(MLQ_LOG_ITER, iter, iter + 1, 0, d);
       msg(iter, 0);
       while_flag = 1;
@@ -964,7 +967,7 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
       break;
     }
     {
-      double d[3] = {tau1, beta, J1};
+      double d[6] = {tau1, beta, J1, br.tA, br.tC, (double)(nJ_ls)};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);
     }
     if (e.spectra_due(cfg, iter)) {                                // :377-381
@@ -1154,7 +1157,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
     hipLaunchKernelGGL((k_xfwd_ls<T, N, NF>), dim3((unsigned)((long long)N * N / 2 / NF)),
                        dim3(NF * Tf), 0, st_, la);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_XFWD, 3.0 * real_bytes() + nb * spec_bytes());
+  prof_end(PC_XFWD, 3.0 * real_bytes() + nb * ab_xcols(keep));
   SArgs<T> sa{};
   sa.spec = scratch_;
   sa.tw = tw_;
@@ -1163,7 +1166,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   prof_begin(PC_YFWD);
   launch_strided<T, N, -1, 0>(svar_[4], st_, sa, keep + 1, N * nb);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_YFWD, 2.0 * nb * spec_bytes());
+  prof_end(PC_YFWD, nb * (ab_xcols(keep) + ab_kcols(keep)));
   prof_begin(PC_ZFWD);
   switch (var & 3) {
     case 1: launch_zls_v<T, N, 1>(st_, la, ncols, (long long)N * PP, nb); break;
@@ -1175,7 +1178,8 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   hipLaunchKernelGGL(k_reduce_final, dim3(2 * nb), dim3(RED_THREADS), 0, st_, lspart_, nbx, 2 * nb,
                      red_dev_);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_ZFWD, 2.0 * nb * spec_bytes());
+  // the z pass reads the y outputs and the kept cubes of Q0..Q2, and stores nothing
+  prof_end(PC_ZFWD, nb * ab_kcols(keep) + 3.0 * ab_cube(keep));
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));
   for (int k = 0; k < 2 * nb; ++k) out[k] = red_host_[k];
@@ -1197,7 +1201,7 @@ void Engine<T>::ls_risky(const T* S0, const T* S1, const T* S2, std::vector<doub
   hipLaunchKernelGGL((k_ls_risky<T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st_, S0, S1, S2, nr_,
                      cnt, lsrisky_, CAP);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_REDUCE, 0);
+  prof_end(PC_REDUCE, 3.0 * real_bytes());
   unsigned c = 0;
   MLQ_HIP(hipMemcpyAsync(&c, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));

@@ -85,6 +85,7 @@ template <typename T> struct F_ls_products {
 template <typename T> struct P_ls_products {
   static constexpr int NB = 12;
   const T* u; const T* d; const T* G; const T* Gd; long long nr; T gam;
+  double inbytes() const { return 24.0 * sizeof(T); }   // u, d, grad u, grad d
   __device__ void operator()(long long i, T (&o)[12]) const {
     ls_products_at(u, d, G, Gd, nr, gam, i, o);
   }
@@ -98,6 +99,7 @@ template <typename T> struct R_ls_poly {
   static constexpr int vec = V16<T>::w;   // consecutive points per call (16-byte loads)
   const T* a; const T* b; const T* e; const T* u; const T* d;
   const T* P0; const T* P1; const T* P2; long long nr; double gam;   // d + gam u
+  double abytes() const { return 18.0 * sizeof(T); }   // per point
   __device__ void operator()(long long iv, double (&acc)[11]) const {
     constexpr int W = vec;
     double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
@@ -181,6 +183,7 @@ template <typename T> struct F_step {
 template <typename T> struct P_step {
   static constexpr int NB = 3;
   const T* u; const T* d; const T* A; long long nr; T a, b, sc; T* un;
+  double inbytes() const { return 10.0 * sizeof(T); }   // A, u, d read, u_new written
   __device__ void operator()(long long i, T (&o)[3]) const {
     const T w = gld<NT_PROD>(A + i);
 #pragma unroll
@@ -196,6 +199,7 @@ template <typename T> struct P_step {
 template <typename T> struct F_quad {
   static constexpr int vec = V16<T>::w;   // 16-byte accesses; o may alias x0
   const T* x0; const T* x1; const T* x2; T* o; T t, sc;
+  double abytes() const { return 4.0 * sizeof(T); }
   __device__ void operator()(long long iv) const {
     T a[vec], b[vec], c[vec], r[vec];
     ld16(x0, iv, a);
@@ -255,6 +259,11 @@ template <typename T> struct F_grad_terms {
 template <typename T> struct P_grad_terms {
   static constexpr int NB = 3;
   F_grad_terms<T> f;   // Y unused
+  double inbytes() const {
+    // grad u, aux2, A, grad phi, u, grad p, Lap u; the pending update of
+    // grad u reads Gd0 and writes grad u back
+    return ((f.A ? 23.0 : 22.0) + (f.Gd0 ? 18.0 : 0.0)) * sizeof(T);
+  }
   __device__ void operator()(long long i, T (&o)[3]) const { f.at(i, o); }
 };
 
@@ -411,6 +420,7 @@ __device__ __forceinline__ KeptIdx kept_decode(long long i, int n, int px, int M
 template <typename T> struct S_dir_prep_k {
   c2<T>* V; const c2<T>* W; c2<T>* nH; long long ns; int n, px, M; const double* hw;
   double nu, sW;
+  double abytes() const { return 12.0 * sizeof(c2<T>); }   // per kept mode
   __device__ void operator()(long long i) const {
     const KeptIdx k = kept_decode(i, n, px, M);
     const long long o = k.off;
@@ -452,6 +462,7 @@ template <typename T> struct S_dir_prep_k {
 template <typename T> struct R_dir_k {
   const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px, M;
   const double* hl;
+  double abytes() const { return (G0 ? 12.0 : 9.0) * sizeof(c2<T>); }
   __device__ void operator()(long long i, double (&acc)[10]) const {
     const KeptIdx k = kept_decode(i, n, px, M);
     const long long o = k.off;
@@ -490,6 +501,7 @@ template <typename T> struct R_dir_k {
 template <typename T> struct S_dir_update_k {
   const c2<T>* V; const c2<T>* N; const c2<T>* D0; c2<T>* gu1; c2<T>* d; long long ns; int n, px, M;
   double cg, alpha, beta;
+  double abytes() const { return (D0 ? 15.0 : 12.0) * sizeof(c2<T>); }
   __device__ void operator()(long long i) const {
     const long long o = kept_decode(i, n, px, M).off;
 #pragma unroll
@@ -512,6 +524,7 @@ template <typename T> struct S_dir_update_k {
 // o = sc (x0 + t x1 + t^2 x2) on the kept cube (one spectrum)
 template <typename T> struct S_quad_k {
   const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; int n, px, M; double t, sc;
+  double abytes() const { return 4.0 * sizeof(c2<T>); }
   __device__ void operator()(long long i) const {
     const long long k = kept_decode(i, n, px, M).off;
     const c2<T> a = gld<NT_PROD>(x0 + k), b = gld<NT_PROD>(x1 + k), c = gld<NT_PROD>(x2 + k);
@@ -523,6 +536,7 @@ template <typename T> struct S_quad_k {
 // i K . W (divergence of a band-limited vector spectrum) on the kept cube
 template <typename T> struct S_div_k {
   const c2<T>* W; c2<T>* o; long long ns; int n, px, M;
+  double abytes() const { return 4.0 * sizeof(c2<T>); }
   __device__ void operator()(long long i) const {
     const KeptIdx k = kept_decode(i, n, px, M);
     const long long q = k.off;
@@ -577,6 +591,7 @@ template <typename T> struct R_uh_update {
 // left with their pending factor): acc[0] = <u_hat, u_hat>_H_l inside
 template <typename T> struct R_uh_update_k {
   c2<T>* U; const c2<T>* D; long long ns; int n, px, M; double a, b; const double* hl;
+  double abytes() const { return 9.0 * sizeof(c2<T>); }
   __device__ void operator()(long long i, double (&acc)[1]) const {
     const KeptIdx k = kept_decode(i, n, px, M);
     const long long o = k.off;
@@ -760,6 +775,7 @@ template <typename T> struct F_pB {
 // the kept cube (W masked), for three right-hand sides at once.
 template <typename T> struct R_kept_dot3 {
   const c2<T>* W; const c2<T>* x; const c2<T>* y; const c2<T>* z; long long ns; int n, px, M;
+  double abytes() const { return 12.0 * sizeof(c2<T>); }
   __device__ void operator()(long long i, double (&acc)[3]) const {
     const KeptIdx k = kept_decode(i, n, px, M);
     const int kx = k.kx;
@@ -787,6 +803,7 @@ template <typename T> struct R_kept_dot3 {
 template <typename T> struct R_ls_poly4 {
   const c2<T>* S0; const c2<T>* S1; const c2<T>* S2;
   const c2<T>* Q0; const c2<T>* Q1; const c2<T>* Q2; int n, px, M;
+  double abytes() const { return 6.0 * sizeof(c2<T>); }
   __device__ void operator()(long long i, double (&acc)[10]) const {
     const int W = 2 * M + 1;
     const unsigned ii = (unsigned)i, r = ii / (unsigned)(M + 1);

@@ -801,6 +801,24 @@ template <class F> struct VecW<F, std::void_t<decltype(F::vec)>> {
   static constexpr int v = F::vec;
 };
 
+// algorithmic bytes per element of a map or reduction: one read of every
+// input element and one write of every output element the functor touches
+// (host method `double abytes() const`); 0 when the functor does not state it.
+// For a producer of the fused forward x pass, `inbytes()` is the bytes it
+// reads (and writes besides its outputs) per point.
+template <class F, class = void> struct AlgB {
+  static double of(const F&) { return 0.0; }
+};
+template <class F> struct AlgB<F, std::void_t<decltype(std::declval<const F&>().abytes())>> {
+  static double of(const F& f) { return f.abytes(); }
+};
+template <class F, class = void> struct ProdB {
+  static double of(const F&) { return 0.0; }
+};
+template <class F> struct ProdB<F, std::void_t<decltype(std::declval<const F&>().inbytes())>> {
+  static double of(const F& f) { return f.inbytes(); }
+};
+
 // elements per grid-stride trip: 2 unless the functor says otherwise
 // (`static constexpr int unroll`, for functors heavy in registers)
 template <class F, class = void> struct RedUnroll { static constexpr int v = 2; };

@@ -418,7 +418,7 @@ void Engine<T>::spectral_data_dev(const T* u, bool normalize, double* out) {
   MLQ_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_shell_final, dim3(n), dim3(256), 0, st_, part, (int)nblk, n, shells);
   MLQ_HIP(hipGetLastError());
-  prof_end(PC_REDUCE, 0);
+  prof_end(PC_REDUCE, 3.0 * spec_bytes());
   std::vector<double> h(n);
   MLQ_HIP(hipMemcpyAsync(h.data(), shells, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));

@@ -21,6 +21,7 @@ template <typename T> struct F_fill { T* o; T v; __device__ void operator()(long
 struct F_widen {
   static constexpr int vec = 4;
   const float* a; double* o;
+  double abytes() const { return 12.0; }
   __device__ void operator()(long long i) const {
     float v[4];
     ld16(a, i, v);
@@ -33,26 +34,38 @@ struct F_widen {
 template <typename T> struct F_copy {   // 16-byte vectors
   static constexpr int vec = V16<T>::w;
   const T* a; T* o;
+  double abytes() const { return 2.0 * sizeof(T); }
   __device__ void operator()(long long i) const {
     T v[vec];
     ld16(a, i, v);
     st16(o, i, v);
   }
 };
-template <typename T> struct F_scale { T* o; T s; __device__ void operator()(long long i) const { o[i] *= s; } };
-template <typename T> struct F_divide { T* o; double s; __device__ void operator()(long long i) const { o[i] = (T)((double)o[i] / s); } };
+template <typename T> struct F_scale {
+  T* o; T s;
+  double abytes() const { return 2.0 * sizeof(T); }
+  __device__ void operator()(long long i) const { o[i] *= s; }
+};
+template <typename T> struct F_divide {
+  T* o; double s;
+  double abytes() const { return 2.0 * sizeof(T); }
+  __device__ void operator()(long long i) const { o[i] = (T)((double)o[i] / s); }
+};
 // o = a*x + b*y
 template <typename T> struct F_axpby {
   const T* x; const T* y; T* o; T a, b;
+  double abytes() const { return (x == y ? 2.0 : 3.0) * sizeof(T); }
   __device__ void operator()(long long i) const { o[i] = a * x[i] + b * y[i]; }
 };
 template <typename T> struct F_mul {
   const T* a; const T* b; T* o;
+  double abytes() const { return 3.0 * sizeof(T); }
   __device__ void operator()(long long i) const { o[i] = a[i] * b[i]; }
 };
 // o = u0^2 + u1^2 + u2^2  (calc_uk, function_ops.f90:2332)
 template <typename T> struct F_dot3self {
   const T* u; long long nr; T* o;
+  double abytes() const { return 4.0 * sizeof(T); }
   __device__ void operator()(long long i) const {
     const T a = u[i], b = u[nr + i], c = u[2 * nr + i];
     o[i] = a * a + b * b + c * c;
@@ -148,6 +161,7 @@ template <typename T> struct F_udotforce {
 template <typename T> struct P_udotforce {
   static constexpr int NB = 1;
   F_udotforce<T> f;   // o unused
+  double inbytes() const { return 9.0 * sizeof(T); }
   __device__ void operator()(long long i, T (&r)[1]) const { r[0] = f.at(i); }
 };
 // o = sum_j u_j g_j (one component of (u . grad) u from g_j = d_j u_i,
@@ -205,6 +219,7 @@ template <typename T> struct F_noise {
 // ---- reduction functors (accumulate in FP64) --------------------------------------
 template <typename T> struct R_dot {
   const T* a; const T* b;
+  double abytes() const { return (a == b ? 1.0 : 2.0) * sizeof(T); }
   __device__ void operator()(long long i, double (&acc)[1]) const { acc[0] += (double)a[i] * (double)b[i]; }
 };
 template <typename T> struct R_sum {

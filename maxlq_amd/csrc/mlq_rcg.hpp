@@ -128,7 +128,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
   int while_flag = 0, stop = 0;
   long long nJ = 0, nJc = 0;
   {
-    double d[3] = {0.0, 0.0, J0};
+    double d[6] = {0.0, 0.0, J0, 0.0, 0.0, 0.0};
     log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                       // :117
   }
   if (spectra_due(cfg, iter)) emit_spectrum(log, cfg, iter, 0, fphys(u));   // :119-121
@@ -186,11 +186,12 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
       log.emit(MLQ_LOG_BRENT, iter, j, 0, d);
     };
     msg(iter, 20);
+    const long long nJ_it = nJ;
     BracketResult br = mnbrak(J, 0.0, tau0, llog);                  // :269
     if (br.flag != 0) {
       if (br.flag == 1) msg(iter, 22);
       log.emit(MLQ_LOG_ERR, iter, br.flag, 0, nullptr);
-      double d[3] = {tau1, beta, J0};
+      double d[6] = {tau1, beta, J0, 0.0, 0.0, (double)(nJ - nJ_it)};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :285
       while_flag = 1;
       stop = br.flag == 1 ? 2 : 3;
@@ -224,7 +225,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     J1 = eval_J(u, MLQ_J_MAXDLQDT);
     const double deltaJ = (J1 - J0) / std::fabs(J0);
     if (deltaJ < -MACH_EPSILON) {                                   // :351-365
-      double d[3] = {tau1, beta, J0};
+      double d[6] = {tau1, beta, J0, br.tA, br.tC, (double)(nJ - nJ_it)};
       log.emit(MLQ_LOG_ITER, iter, iter + 1, 0, d);
       msg(iter, 0);
       while_flag = 1;
@@ -232,7 +233,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
       break;
     }
     {
-      double d[3] = {tau1, beta, J1};
+      double d[6] = {tau1, beta, J1, br.tA, br.tC, (double)(nJ - nJ_it)};
       log.emit(MLQ_LOG_ITER, iter, iter, 0, d);                     // :375
     }
     if (spectra_due(cfg, iter)) {                                   // :377-381

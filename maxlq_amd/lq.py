@@ -32,10 +32,18 @@ def _f64(a, shape):
     return a
 
 
+def _c128(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
 @dataclass
 class RcgTrace:
     """Records the reference writes from inside maxdLqdt (data_ops.f90)."""
     iteration_info: list = field(default_factory=list)   # (iter, tau, beta, J)
+    brackets: list = field(default_factory=list)         # (iter, tau_brack a, b, J requests)
     linemin: list = field(default_factory=list)          # (iter, brakIter, mode, tA..FC)
     brent: list = field(default_factory=list)            # (iter, j, X, FX)
     msgs: list = field(default_factory=list)             # (iter, code)
@@ -92,13 +100,13 @@ class MaxLq:
 
     # -- fftwfunction ---------------------------------------------------------------
     def fftfwd(self, u):
-        u = np.ascontiguousarray(u, dtype=np.complex128)
+        u = _c128(u, self.shape)
         out = np.empty(self.shape, np.complex128)
         self._chk(self.lib.mlq_fftfwd(self.ctx, _p(u.view(np.float64)), _p(out.view(np.float64))))
         return out
 
     def fftbwd(self, fu):
-        fu = np.ascontiguousarray(fu, dtype=np.complex128)
+        fu = _c128(fu, self.shape)
         out = np.empty(self.shape, np.complex128)
         self._chk(self.lib.mlq_fftbwd(self.ctx, _p(fu.view(np.float64)), _p(out.view(np.float64))))
         return out
@@ -110,7 +118,7 @@ class MaxLq:
         return out
 
     def fftbwdr(self, fu):
-        fu = np.ascontiguousarray(fu, dtype=np.complex128)
+        fu = _c128(fu, self.hshape)
         out = np.empty(self.shape, np.float64)
         self._chk(self.lib.mlq_fftbwdr(self.ctx, _p(fu.view(np.float64)), _p(out)))
         return out
@@ -372,6 +380,7 @@ class MaxLq:
         def cb(user, kind, it, i0, i1, d):
             if kind == capi.MLQ_LOG_ITER:
                 tr.iteration_info.append((i0, d[0], d[1], d[2]))
+                tr.brackets.append((i0, d[3], d[4], int(d[5])))
             elif kind == capi.MLQ_LOG_LINEMIN:
                 tr.linemin.append((it, i0, i1) + tuple(d[k] for k in range(6)))
             elif kind == capi.MLQ_LOG_BRENT:

@@ -751,8 +751,9 @@ class SolveResult:
     stop_reason: str
 
 
-def maxdLqdt(orc: Oracle, u0, startIter=0, on_iter=None):
-    """One RCG solve for the current B (optimization.f90:7-514)."""
+def maxdLqdt(orc: Oracle, u0, startIter=0, on_iter=None, on_start=None):
+    """One RCG solve for the current B (optimization.f90:7-514).  on_start()
+    is called once the loop's initial values are set (timing hook)."""
     P = orc.p
     q, B = P.q, P.B
     HL = INNER_HL
@@ -775,6 +776,8 @@ def maxdLqdt(orc: Oracle, u0, startIter=0, on_iter=None):
     records = []
     stop_reason = ""
     s = 1.5 - 1.0 / q
+    if on_start:
+        on_start()
     while abs(avg) > P.OPTIM_TOL and it < P.MAX_ITER and while_flag < 1:
         it += 1
         msgs = []

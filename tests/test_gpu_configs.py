@@ -96,10 +96,13 @@ def test_rcg_solve_64_against_oracle_fixture():
     h = g.field(3)
     g.seeded_field(h, gold["seed"], gold["envelope"][0], gold["envelope"][1])
     recs = []
+    msgs = {}
 
     def cb(user, kind, it, i0, i1, d):
         if kind == capi.MLQ_LOG_ITER:
-            recs.append((i0, d[0], d[1], d[2]))
+            recs.append((i0, d[0], d[1], d[2], d[3], d[4], int(d[5])))
+        elif kind == capi.MLQ_LOG_MSG:
+            msgs.setdefault(it, []).append(i0)
 
     cfg = capi.RcgCfg(gold["max_iter"], gold["optim_tol"], 0, 0, 25, 10, 0)
     res = capi.RcgResult()
@@ -114,13 +117,23 @@ def test_rcg_solve_64_against_oracle_fixture():
     # the differences of the iterates accumulate over the 50 steps (observed
     # 2e-10).  Parity of every evaluation on identical inputs is 1e-10
     # (the oracle tests at 16^3-256^3); the trajectory bound is 1e-9.
-    worst, worst_tau = 0.0, 0.0
-    for (it, tau, beta, J), r in zip(got, ref):
+    worst, worst_tau, worst_br = 0.0, 0.0, 0.0
+    for (it, tau, beta, J, ta, tc, nj), r in zip(got, ref):
         assert it == r["iter"]
         worst_tau = max(worst_tau, abs(tau - r["tau"]) / abs(r["tau"]))
         worst = max(worst, abs(J - r["J1"]) / abs(r["J1"]))
-    print("max |dJ|/|J| over the trajectory", worst, "max |dtau|/tau", worst_tau)
+        # mnbrak's bracket (tA, tC) and the number of J requests of mnbrak
+        # and brent, and the message codes of the iteration (the final
+        # "optimization finished" code 1 is outside the oracle's records)
+        worst_br = max(worst_br, abs(ta - r["tau_brack"][0]) / max(abs(r["tau_brack"][1]), 1e-300),
+                       abs(tc - r["tau_brack"][1]) / abs(r["tau_brack"][1]))
+        assert nj == r["nJ"], (it, nj, r["nJ"])
+        m = [c for c in msgs.get(it, []) if c != 1 or it != res.iters]
+        assert m == r["msgs"], (it, m, r["msgs"])
+    print("max |dJ|/|J| over the trajectory", worst, "max |dtau|/tau", worst_tau,
+          "max bracket difference", worst_br)
     assert worst_tau < 1e-4
+    assert worst_br < 1e-4
     assert worst < 1e-9
     assert abs(res.J1 - gold["J1"]) < 1e-9 * abs(gold["J1"])
     u = g.download(h)
