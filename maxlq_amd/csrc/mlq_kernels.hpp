@@ -525,8 +525,20 @@ void k_strided(SArgs<T> a, int ncols) {
     // register footprint by E; the barrier orders these slot accesses before
     // the first FFT pass writes other threads' slots.
     const double fs = c0 * op.scale;
-    if (col_factor_ok(op)) {
+    if ((op.flags & ~SP_MASK) == 0) {
+      // the mask and a scale: a real factor, fs or 0, per element
       if (op.flags != 0 || fs != 1.0) {
+        const int mxy = max(kx, mabs(ky, N));
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const bool in = !(op.flags & SP_MASK) || max(mxy, mabs(t + m * Tf, N)) <= op.mcut;
+          v[m] = in ? (fs == 1.0 ? v[m]
+                                 : cmk<T>((T)((double)v[m].x * fs), (T)((double)v[m].y * fs)))
+                    : cmk<T>((T)0, (T)0);
+        }
+      }
+    } else if (col_factor_ok(op)) {
+      {
         const ColFactor cf = col_factor_init(op, fs, kx, ky, N);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -588,6 +600,21 @@ void k_strided(SArgs<T> a, int ncols) {
       if (zout) {
 #pragma unroll
         for (int m = 0; m < E; ++m) gst<NTS>(d + (t + m * Tf) * estride, cmk<T>((T)0, (T)0));
+      }
+    } else if ((op.flags & ~SP_MASK) == 0) {
+      // the mask and a scale: a real factor, op.scale or 0, per element (a
+      // plain transform stores its values as they are)
+      const int mxy = max(kx, mabs(ky, N));
+      const double fs = op.scale;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = t + m * Tf;
+        if (!zout && band && out_band(n, keep, N)) continue;
+        const bool in = !(op.flags & SP_MASK) || max(mxy, mabs(n, N)) <= op.mcut;
+        const c2<T> r = in ? (fs == 1.0 ? v[m]
+                                        : cmk<T>((T)((double)v[m].x * fs), (T)((double)v[m].y * fs)))
+                           : cmk<T>((T)0, (T)0);
+        gst<NTS>(d + n * estride, r);
       }
     } else if (col_factor_ok(op)) {
       // closed form along the column, in registers
