@@ -334,12 +334,18 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
 // spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
 // inverse FFT, unpack, scale.
+__device__ __forceinline__ float2 shfl_c2(float2 v, int src, int width) {
+  return cmk<float>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+__device__ __forceinline__ double2 shfl_c2(double2 v, int src, int width) {
+  return cmk<double>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
 template <typename T, int N, int NF, bool PR, bool ZB = false>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   constexpr int LP = LdsRow<T>::pitch(N);
-  constexpr int LS = LP > N + 2 ? LP : N + 2;   // row stride: FFT rows, or the staged input
-  __shared__ c2<T> sm[NF * LS];
+  static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
+  __shared__ c2<T> sm[NF * LP];
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const long long pairs_per = (long long)N * N / 2;
   const long long gp = (long long)blockIdx.x * NF + f;
@@ -350,55 +356,59 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const bool xik = a.xmap && a.xik[b];
   const c2<T>* sp = a.spec + slot * a.sstride + row0 * PX;
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
-  c2<double> v[E];
-  {
-    // each mode is loaded once: the kx <= N/2 entries of both rows go to the
-    // row's LDS slots (A at [0, N/2], B at [N/2+1, N+1]) and the Hermitian
-    // extension k > N/2 reads its mirror N-k from there.  Modes beyond the
-    // kept cube are zero: their loads are redirected to kx = 0 (no branch)
-    // and discarded.
-    c2<T>* st = sm + f * LS;
+  // Each mode is loaded once, by the thread that owns kx = t + m Tf (m < E/2)
+  // in the packed row; slot E/2 of thread 0 holds kx = N/2.  Modes beyond the
+  // kept cube are zero: their loads are redirected to kx = 0 (no branch) and
+  // discarded.
+  c2<T> A[E / 2 + 1], B[E / 2 + 1];
 #pragma unroll
-    for (int m = 0; m < E / 2; ++m) {
-      const int k = t + m * Tf;
-      const bool in = k <= kmax;
-      const int ks = in ? k : 0;
-      c2<T> As = cmk<T>((T)0, (T)0), Bs = As;
-      if (!PR || m * Tf <= kmax) {   // uniform: a slot run wholly past kmax loads nothing
-        As = gld<nt_ld<PC_XI>()>(sp + ks);
-        Bs = gld<nt_ld<PC_XI>()>(sp + PX + ks);
-        if (PR && !in) { As = cmk<T>((T)0, (T)0); Bs = As; }
-      }
-      if (xik) {   // i K_x, K_x = 2 pi k (k < N/2 here)
-        const double K = TWO_PI * k;
-        As = cmk<T>((T)(-K * (double)As.y), (T)(K * (double)As.x));
-        Bs = cmk<T>((T)(-K * (double)Bs.y), (T)(K * (double)Bs.x));
-      }
-      st[k] = As;
-      st[N / 2 + 1 + k] = Bs;
-    }
-    if (t == 0) {   // Nyquist kx = N/2: zero under i K_x (REAL(ifft) rule)
-      const bool in = N / 2 <= kmax && !xik;
-      c2<T> As = gld<nt_ld<PC_XI>()>(sp + (in ? N / 2 : 0)),
-            Bs = gld<nt_ld<PC_XI>()>(sp + PX + (in ? N / 2 : 0));
-      if (!in) { As = cmk<T>((T)0, (T)0); Bs = As; }
-      st[N / 2] = As;
-      st[N + 1] = Bs;
-    }
-    __syncthreads();
+  for (int m = 0; m <= E / 2; ++m) { A[m] = cmk<T>((T)0, (T)0); B[m] = A[m]; }
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = t + m * Tf;
-      const int kk = k <= N / 2 ? k : N - k;
-      const c2<T> As = st[kk], Bs = st[N / 2 + 1 + kk];
-      double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
-      if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
-      if (k > N / 2) { ay = -ay; by = -by; }
-      v[m] = cmk<double>(ax - by, ay + bx);
+  for (int m = 0; m < E / 2; ++m) {
+    const int k = t + m * Tf;
+    const bool in = k <= kmax;
+    const int ks = in ? k : 0;
+    if (!PR || m * Tf <= kmax) {   // uniform: a slot run wholly past kmax loads nothing
+      A[m] = gld<nt_ld<PC_XI>()>(sp + ks);
+      B[m] = gld<nt_ld<PC_XI>()>(sp + PX + ks);
+      if (PR && !in) { A[m] = cmk<T>((T)0, (T)0); B[m] = A[m]; }
     }
-    __syncthreads();   // the first FFT pass overwrites the staged rows
+    if (xik) {   // i K_x, K_x = 2 pi k (k < N/2 here)
+      const double K = TWO_PI * k;
+      A[m] = cmk<T>((T)(-K * (double)A[m].y), (T)(K * (double)A[m].x));
+      B[m] = cmk<T>((T)(-K * (double)B[m].y), (T)(K * (double)B[m].x));
+    }
   }
-  LdsRow<T> L{sm + f * LS};
+  if (t == 0) {   // Nyquist kx = N/2: zero under i K_x (REAL(ifft) rule)
+    const bool in = N / 2 <= kmax && !xik;
+    A[E / 2] = gld<nt_ld<PC_XI>()>(sp + (in ? N / 2 : 0));
+    B[E / 2] = gld<nt_ld<PC_XI>()>(sp + PX + (in ? N / 2 : 0));
+    if (!in) { A[E / 2] = cmk<T>((T)0, (T)0); B[E / 2] = A[E / 2]; }
+  }
+  // The Hermitian extension k = t + m Tf > N/2 (m >= E/2) needs mode N - k.
+  // For t >= 1 that is kx = (Tf - t) + (E - 1 - m) Tf, owned by lane Tf - t
+  // of the same transform (a lane exchange inside the wave, no LDS round
+  // trip); for t = 0 it is kx = (E - m) Tf, the thread's own slot E - m.
+  const int p = (Tf - t) & (Tf - 1);
+  c2<double> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    c2<T> As, Bs;
+    if (m < E / 2) {
+      As = A[m];
+      Bs = B[m];
+    } else {
+      const c2<T> ash = shfl_c2(A[E - 1 - m], p, Tf), bsh = shfl_c2(B[E - 1 - m], p, Tf);
+      As = t == 0 ? A[E - m] : ash;
+      Bs = t == 0 ? B[E - m] : bsh;
+    }
+    const int k = t + m * Tf;
+    double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
+    if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
+    if (k > N / 2) { ay = -ay; by = -by; }
+    v[m] = cmk<double>(ax - by, ay + bx);
+  }
+  LdsRow<T> L{sm + f * LP};
   // ZB (pruned launches with keep < 3N/8): elements 3N/8 .. 5N/8 - 1 of the
   // packed row are exact zeros
   fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, a.tw, L);
