@@ -241,6 +241,14 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
   return cmk<T>(v.x * fx - v.y * fy, v.x * fy + v.y * fx);
 }
 
+// complex value from lane src of a width-wide lane group
+__device__ __forceinline__ float2 shfl_c2(float2 v, int src, int width) {
+  return cmk<float>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+__device__ __forceinline__ double2 shfl_c2(double2 v, int src, int width) {
+  return cmk<double>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+
 // ---------------------------------------------------------------------------
 // x pass, forward: two real rows packed as one complex row (a + i b), one
 // N-point FFT, split into the two half spectra.
@@ -256,21 +264,28 @@ __device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t
   // ZB (kmax < 3N/8): entries 3N/8 .. 5N/8 - 1 of the packed row are never
   // read by the split below
   fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, tw, L);
+  static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
+  // The split needs Z[k] and Z[N - k] (rounded to the field precision, as an
+  // LDS exchange would).  For k = t + m Tf, m < E/2, element N - k sits in
+  // slot E - 1 - m of lane Tf - t of the same transform (t >= 1) or in the
+  // thread's own slot (E - m) mod E (t = 0): a lane exchange inside the wave.
+  const int p = (Tf - t) & (Tf - 1);
+  c2<T> z[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
-  __syncthreads();
+  for (int m = 0; m < E; ++m) z[m] = cmk<T>((T)v[m].x, (T)v[m].y);
 #pragma unroll
   for (int m = 0; m < E / 2; ++m) {
+    const c2<T> sh = shfl_c2(z[E - 1 - m], p, Tf);
     const int k = t + m * Tf;
     if (k > kmax) continue;
-    const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
+    const c2<double> zk = ccvt<double>(z[m]), zn = ccvt<double>(t == 0 ? z[(E - m) % E] : sh);
     gst<nt_st<PC_XF>()>(d + k, cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y))));
     gst<nt_st<PC_XF>()>(d + PX + k, cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x))));
   }
   if (t == 0 && kmax == N / 2) {
-    const c2<double> z = ccvt<double>(L.ld(N / 2));
-    gst<nt_st<PC_XF>()>(d + N / 2, cmk<T>((T)z.x, (T)0));
-    gst<nt_st<PC_XF>()>(d + PX + N / 2, cmk<T>((T)z.y, (T)0));
+    const c2<double> zz = ccvt<double>(z[E / 2]);
+    gst<nt_st<PC_XF>()>(d + N / 2, cmk<T>((T)zz.x, (T)0));
+    gst<nt_st<PC_XF>()>(d + PX + N / 2, cmk<T>((T)zz.y, (T)0));
   }
 }
 
@@ -334,12 +349,6 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
 // x pass, inverse (C2R): rebuild the packed complex row from the two half
 // spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
 // inverse FFT, unpack, scale.
-__device__ __forceinline__ float2 shfl_c2(float2 v, int src, int width) {
-  return cmk<float>(__shfl(v.x, src, width), __shfl(v.y, src, width));
-}
-__device__ __forceinline__ double2 shfl_c2(double2 v, int src, int width) {
-  return cmk<double>(__shfl(v.x, src, width), __shfl(v.y, src, width));
-}
 template <typename T, int N, int NF, bool PR, bool ZB = false>
 __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
