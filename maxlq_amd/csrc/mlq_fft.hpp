@@ -125,6 +125,8 @@ template <typename T, int R, int DIR> __device__ __forceinline__ void dftR(c2<T>
 // an FP64 butterfly result to it.
 template <typename S> struct LdsRow {
   using Store = S;
+  // the Tf threads of a row transform are consecutive (Tf divides 64): one wave
+  static constexpr bool wave_local = true;
   c2<S>* base;
   static __host__ __device__ constexpr int pitch(int N) { return N + N / 8; }
   __device__ __forceinline__ c2<S> ld(int n) const { return base[n + (n >> 3)]; }
@@ -136,6 +138,7 @@ template <typename S> struct LdsRow {
 // consecutive threads (consecutive columns) touch consecutive LDS words.
 template <typename S, int C> struct LdsCol {
   using Store = S;
+  static constexpr bool wave_local = false;   // a column's threads are C apart
   c2<S>* base;
   int c;
   __device__ __forceinline__ c2<S> ld(int n) const { return base[n * C + c]; }
@@ -153,6 +156,21 @@ template <typename D> __device__ __forceinline__ c2<D> ccvt(double2 a) {
   return cmk<D>((D)a.x, (D)a.y);
 }
 
+// Synchronisation between the LDS exchanges of one transform: a workgroup
+// barrier when the transform's threads span waves (column view), a
+// wave-level one when each transform lies within one wave (row view).  LDS
+// instructions of one wave execute in order, so the wave-level form only keeps
+// the compiler from moving LDS accesses across it.
+template <class L> __device__ __forceinline__ void lds_sync() {
+  if constexpr (L::wave_local) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // ---- one Stockham pass -----------------------------------------------------------
 // Precision: every twiddle product and butterfly is evaluated in FP64.  The
 // per-thread register array v has type RT: the storage precision in the
@@ -167,6 +185,7 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
   constexpr int Ns = ns_at(N, E, P);
   constexpr int NB = E / R;  // butterflies per thread
   constexpr bool last = (P == npasses(N, E) - 1);
+  static_assert(!L::wave_local || (Tf <= 64 && 64 % Tf == 0), "a row transform within one wave");
   if constexpr (P > 0) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -174,7 +193,7 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b + r * NB] = ccvt<RT>(lds.ld(j + r * (N / R)));
     }
-    __syncthreads();
+    lds_sync<L>();
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -211,7 +230,7 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
     }
   }
   if constexpr (!last) {
-    __syncthreads();
+    lds_sync<L>();
     fft_pass<RT, N, E, DIR, P + 1, L, ZB>(v, t, tw, lds);
   }
 }
