@@ -386,6 +386,10 @@ int mlq_time_column_copy(mlq_ctx* ctx, int axis, int tile, int nfields, int reps
  * ms per launch) and choice[pass] (8 values) may be null.  mlq_init runs it
  * for n >= 128 unless the environment sets MLQ_AUTOTUNE=0. */
 int mlq_autotune(mlq_ctx* ctx, int batch, double* times, int* choice);
+/* set the tile of each of the 8 strided passes (choice[pass] = shape +
+ * 4*order, as mlq_autotune reports it), e.g. to repeat a measurement with the
+ * tiles of an earlier run.  No counterpart in the reference. */
+int mlq_set_tiles(mlq_ctx* ctx, const int* choice);
 /* bytes of one real field and of one half spectrum in the active precision */
 int mlq_sizes(const mlq_ctx* ctx, double* real_bytes, double* spec_bytes);
 /* per-kernel-class timing accumulated while profiling is on (HIP events on the

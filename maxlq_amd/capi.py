@@ -41,7 +41,7 @@ EXPORTS = [
     "mlq_calc_local_dLqdt_inclParts", "mlq_calc_global_dLqdt_inclParts", "mlq_GradL2ForLq",
     "mlq_eval_J", "mlq_field_alloc", "mlq_field_free", "mlq_upload", "mlq_download",
     "mlq_seeded_field", "mlq_eval_J_batch", "mlq_time_linesearch", "mlq_kappa_test", "mlq_rcg_path", "mlq_rcg_solve", "mlq_time_fft", "mlq_time_copy", "mlq_time_column_copy",
-    "mlq_autotune", "mlq_sizes",
+    "mlq_autotune", "mlq_set_tiles", "mlq_sizes",
     "mlq_prof_enable", "mlq_prof_get", "mlq_prof_name", "mlq_mem_info", "mlq_mem_live",
     "mlq_transform_counts", "mlq_save_field_nc", "mlq_read_field_nc", "mlq_result_filename",
     "mlq_q_text", "mlq_parse_result_filename", "mlq_fftw_mpi_local_size_3d",
@@ -139,6 +139,7 @@ def load(path: str = LIB_PATH):
         "mlq_time_copy": (ci, [vp, ci, ci, dp]),
         "mlq_time_column_copy": (ci, [vp, ci, ci, ci, ci, dp, dp]),
         "mlq_autotune": (ci, [vp, ci, dp, C.POINTER(ci)]),
+        "mlq_set_tiles": (ci, [vp, C.POINTER(ci)]),
         "mlq_sizes": (ci, [vp, dp, dp]),
         "mlq_prof_enable": (ci, [vp, ci]),
         "mlq_prof_get": (ci, [vp, ci, C.POINTER(C.c_longlong), dp, dp]),

@@ -84,8 +84,11 @@ int mlq_init(mlq_ctx** ctx, int n1, int n2, int n3, const mlq_params* p) {
   int rc = guarded(c, [&] {
     c->eng = p->precision == MLQ_MIXED ? make_engine_fp32(n, *p) : make_engine_fp64(n, *p);
     noalloc_bind();
+    // MLQ_AUTOTUNE: 0 skips the tile autotuner, a positive value sets the
+    // number of spectra it times each candidate on (default 3)
     const char* at = std::getenv("MLQ_AUTOTUNE");
-    if (n >= 128 && !(at && at[0] == '0')) c->eng->autotune(3, nullptr, nullptr);
+    const int ab = at ? std::atoi(at) : 3;
+    if (n >= 128 && !(at && at[0] == '0')) c->eng->autotune(ab > 0 ? ab : 3, nullptr, nullptr);
   });
   if (rc != MLQ_OK) { delete c; return rc; }
   *ctx = c;
@@ -431,6 +434,14 @@ int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms) {
 }
 int mlq_autotune(mlq_ctx* ctx, int batch, double* times, int* choice) {
   return guarded(ctx, [&] { ctx->eng->autotune(batch, times, choice); });
+}
+int mlq_set_tiles(mlq_ctx* ctx, const int* choice) {
+  return guarded(ctx, [&] {
+    if (!choice) throw Error(MLQ_EINVAL, "null argument");
+    for (int p = 0; p < 8; ++p)
+      if (choice[p] < 0 || choice[p] > 7) throw Error(MLQ_EINVAL, "tile choice out of range 0..7");
+    ctx->eng->set_tiles(choice);
+  });
 }
 int mlq_time_copy(mlq_ctx* ctx, int nfields, int reps, double* t_ms) {
   return guarded(ctx, [&] { ctx->eng->time_copy(nfields, reps, t_ms); });

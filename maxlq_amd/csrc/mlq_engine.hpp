@@ -104,6 +104,7 @@ class IEngine {
   // time the tile candidates of the strided passes and keep the fastest;
   // times[pass*8+cand] in ms per launch, choice[pass]
   virtual void autotune(int batch, double* times, int* choice) = 0;
+  virtual void set_tiles(const int* choice) = 0;
   virtual void prof_enable(bool on) = 0;
   virtual void prof_get(int cls, long long* count, double* ms, double* bytes) = 0;
   virtual void mem_info(double* used, double* peak) = 0;

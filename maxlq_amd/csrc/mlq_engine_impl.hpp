@@ -145,6 +145,9 @@ template <typename T> class Engine final : public IEngine {
   void time_column_copy(int axis, int var, int nfields, int reps, double* t_ms,
                         double* bytes) override;
   void autotune(int batch, double* times, int* choice) override;
+  void set_tiles(const int* choice) override {
+    for (int p = 0; p < 8; ++p) svar_[p] = choice[p];
+  }
   void prof_enable(bool on) override;
   void prof_get(int cls, long long* count, double* ms, double* bytes) override;
   void mem_info(double* used, double* peak) override { *used = (double)used_; *peak = (double)peak_; }
@@ -297,7 +300,8 @@ template <typename T> class Engine final : public IEngine {
   int sb_ = MAXB;     // spectra in the transform scratch (largest transform batch)
   // tile variant of the strided passes {y fwd, z fwd, z inv, y inv} (autotuned)
   int svar_[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // {y fwd, z fwd, z inv, y inv} x {full, band-limited}
-  template <int N> double time_pass_n(int pass, int var, int batch, int reps, int keep = -1);
+  template <int N> double time_pass_n(int pass, int var, int batch, int reps, int keep = -1,
+                                      C* ext = nullptr);
   template <int N>
   double time_colcopy_n(int axis, int var, const C* src, C* dst, int nfields, int reps);
   long long nr_, ns_;

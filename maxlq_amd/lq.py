@@ -420,6 +420,11 @@ class MaxLq:
         self._chk(self.lib.mlq_autotune(self.ctx, batch, _p(t), ch))
         return t.reshape(8, 8), list(ch)
 
+    def set_tiles(self, choice):
+        """Set the tile of each strided pass (8 values as autotune returns them)."""
+        ch = (C.c_int * 8)(*[int(x) for x in choice])
+        self._chk(self.lib.mlq_set_tiles(self.ctx, ch))
+
     def time_copy(self, nfields=3, reps=5):
         r = C.c_double()
         self._chk(self.lib.mlq_time_copy(self.ctx, nfields, reps, C.byref(r)))
