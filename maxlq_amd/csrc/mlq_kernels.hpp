@@ -241,12 +241,32 @@ template <typename T> __device__ __forceinline__ c2<T> apply_factor(c2<T> v, dou
   return cmk<T>(v.x * fx - v.y * fy, v.x * fy + v.y * fx);
 }
 
+// a, or zero when !in (component selects: a select of whole array elements
+// can make the compiler keep the array in scratch memory)
+template <typename T> __device__ __forceinline__ c2<T> keep_or_zero(bool in, c2<T> a) {
+  return cmk<T>(in ? a.x : (T)0, in ? a.y : (T)0);
+}
+
 // complex value from lane src of a width-wide lane group
 __device__ __forceinline__ float2 shfl_c2(float2 v, int src, int width) {
   return cmk<float>(__shfl(v.x, src, width), __shfl(v.y, src, width));
 }
 __device__ __forceinline__ double2 shfl_c2(double2 v, int src, int width) {
   return cmk<double>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+
+// The field a transform of a row-view launch belongs to.  When every wave
+// holds whole groups of consecutive row pairs of one field (the transforms per
+// wave divide both the transforms per workgroup and the row pairs per field),
+// the index is wave-uniform and is moved to a scalar register, so the
+// per-field kernel arguments indexed by it load through the scalar unit
+// instead of as a vector load the wave has to wait for.
+template <int N, int NF> __device__ __forceinline__ int wave_field(int b) {
+  constexpr int per_wave = 64 / FftShape<N>::Tf;
+  if constexpr (NF % per_wave == 0 && ((long long)N * N / 2) % per_wave == 0)
+    return __builtin_amdgcn_readfirstlane(b);
+  else
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,7 +317,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const long long pairs_per = (long long)N * N / 2;
   const long long gp = (long long)blockIdx.x * NF + f;
-  const int b = (int)(gp / pairs_per);
+  const int b = wave_field<N, NF>((int)(gp / pairs_per));
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const T* s = a.rsrc[b] + row0 * N;
@@ -358,7 +378,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const long long pairs_per = (long long)N * N / 2;
   const long long gp = (long long)blockIdx.x * NF + f;
-  const int b = (int)(gp / pairs_per);
+  const int b = wave_field<N, NF>((int)(gp / pairs_per));
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const int slot = a.xmap ? a.xin[b] : b;
@@ -367,32 +387,41 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   // Each mode is loaded once, by the thread that owns kx = t + m Tf (m < E/2)
   // in the packed row; slot E/2 of thread 0 holds kx = N/2.  Modes beyond the
-  // kept cube are zero: their loads are redirected to kx = 0 (no branch) and
-  // discarded.
+  // kept cube are zero: their loads are redirected to kx = 0 and discarded.
+  // Every load of the thread is issued before any of them is used (one
+  // memory round trip per transform, no wait inside the load sequence).
   c2<T> A[E / 2 + 1], B[E / 2 + 1];
-#pragma unroll
-  for (int m = 0; m <= E / 2; ++m) { A[m] = cmk<T>((T)0, (T)0); B[m] = A[m]; }
 #pragma unroll
   for (int m = 0; m < E / 2; ++m) {
     const int k = t + m * Tf;
-    const bool in = k <= kmax;
-    const int ks = in ? k : 0;
+    const int ks = k <= kmax ? k : 0;
     if (!PR || m * Tf <= kmax) {   // uniform: a slot run wholly past kmax loads nothing
       A[m] = gld<nt_ld<PC_XI>()>(sp + ks);
       B[m] = gld<nt_ld<PC_XI>()>(sp + PX + ks);
-      if (PR && !in) { A[m] = cmk<T>((T)0, (T)0); B[m] = A[m]; }
+    } else {
+      A[m] = cmk<T>((T)0, (T)0);
+      B[m] = A[m];
+    }
+  }
+  // Nyquist kx = N/2 (zero under i K_x, the REAL(ifft) rule), used by thread
+  // 0 only; the other lanes load kx = 0 again, the line of their slot 0
+  const bool nyq = t == 0 && N / 2 <= kmax && !xik;
+  A[E / 2] = gld<nt_ld<PC_XI>()>(sp + (nyq ? N / 2 : 0));
+  B[E / 2] = gld<nt_ld<PC_XI>()>(sp + PX + (nyq ? N / 2 : 0));
+  A[E / 2] = keep_or_zero<T>(nyq, A[E / 2]);
+  B[E / 2] = keep_or_zero<T>(nyq, B[E / 2]);
+#pragma unroll
+  for (int m = 0; m < E / 2; ++m) {
+    const int k = t + m * Tf;
+    if (PR) {
+      A[m] = keep_or_zero<T>(k <= kmax, A[m]);
+      B[m] = keep_or_zero<T>(k <= kmax, B[m]);
     }
     if (xik) {   // i K_x, K_x = 2 pi k (k < N/2 here)
       const double K = TWO_PI * k;
       A[m] = cmk<T>((T)(-K * (double)A[m].y), (T)(K * (double)A[m].x));
       B[m] = cmk<T>((T)(-K * (double)B[m].y), (T)(K * (double)B[m].x));
     }
-  }
-  if (t == 0) {   // Nyquist kx = N/2: zero under i K_x (REAL(ifft) rule)
-    const bool in = N / 2 <= kmax && !xik;
-    A[E / 2] = gld<nt_ld<PC_XI>()>(sp + (in ? N / 2 : 0));
-    B[E / 2] = gld<nt_ld<PC_XI>()>(sp + PX + (in ? N / 2 : 0));
-    if (!in) { A[E / 2] = cmk<T>((T)0, (T)0); B[E / 2] = A[E / 2]; }
   }
   // The Hermitian extension k = t + m Tf > N/2 (m >= E/2) needs mode N - k.
   // For t >= 1 that is kx = (Tf - t) + (E - 1 - m) Tf, owned by lane Tf - t
