@@ -49,7 +49,7 @@ template <typename T> class FastRcg {
     poly_ok_ = q4_ && M_ < n_ / 2;
     N3_ = (double)nr_;
     N6_ = N3_ * N3_;
-    nk_ = (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1);
+    nkc_ = kept_calls<T>(M_);
     // weights of |K| = 2 pi |m| by |m|^2: Hilbert 1/(1 + (lam |K|)^(2s)) and the
     // squared H_l weight (1 + (lam |K|)^s)^2, s = 3/2 - 1/q (function_ops.f90:3611, 3449)
     const int m2max = 3 * (n_ / 2) * (n_ / 2);
@@ -216,7 +216,7 @@ template <typename T> class FastRcg {
     RT gphi = e_.rtmp(3);
     {
       ST dv = e_.stmp();
-      e_.map(S_div_k<T>{W_, dv, ns_, n_, pp_, M_}, nk_, PC_SPECTRAL);
+      e_.map(S_div_k<T>{W_, dv, ns_, n_, pp_, M_}, nkc_, PC_SPECTRAL);
       InvSrc is[3];
       SpecOp ops[3];
       T* dst[3];
@@ -255,7 +255,7 @@ template <typename T> class FastRcg {
     }
     ST nH = e_.stmp(3);
     e_.map(S_dir_prep_k<T>{Vh, W_, nH, ns_, n_, pp_, M_, hwt_, nuvc_, std::pow(L_, 1.0 - q_)},
-           nk_, PC_SPECTRAL);
+           nkc_, PC_SPECTRAL);
     // every H_l product of the step in one pass; the tangential projection
     // gproj = Vh - alpha nH (:177), the normalisation (:211-214) and the
     // Polak-Ribiere numerator (:226) follow by bilinearity
@@ -263,7 +263,7 @@ template <typename T> class FastRcg {
     // comes from the last update of u_hat)
     double r[10];
     e_.template reduce<10>(R_dir_k<T>{Vh, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_, pp_,
-                                      M_, hlt_}, nk_, r);
+                                      M_, hlt_}, nkc_, r);
     for (double& x : r) x /= N6_;
     // transported gradient vT_g0 = vg_s gradJused0 + vg_u u (see transport_and_update)
     const double vv = r[0], vn = r[1], nn = r[2], uu = uu_;
@@ -290,7 +290,7 @@ template <typename T> class FastRcg {
     gu1_ = e_.stmp(3);
     ST dBn = e_.stmp(3);
     e_.map(S_dir_update_k<T>{Vh, nH, have_cg_ ? dB_.get() : nullptr, gu1_, dBn, ns_, n_, pp_, M_,
-                             c, alpha, beta * vd_B_}, nk_, PC_SPECTRAL);
+                             c, alpha, beta * vd_B_}, nkc_, PC_SPECTRAL);
     gam_ = have_cg_ ? beta * vd_u_ : 0.0;
     dB_ = std::move(dBn);
     warm_[0] = ST();
@@ -409,7 +409,7 @@ template <typename T> class FastRcg {
         double pa[10];
         e_.template reduce<10>(R_ls_poly4<T>{S0h_, S12h_, S12h_ + ns_, Q0h_, Q12h_, Q12h_ + ns_,
                                               n_, pp_, M_},
-                               (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), pa);
+                               nkc_, pa);
         for (int k = 0; k < 5; ++k) { lpoly_[k] = pa[k]; qpoly_[k] = pa[5 + k]; }
         e_.ls_risky(s_, S1_, S2_, risky_, nrisky_);
         // k = 0 coefficients: sum over the grid of S0, S1, S2
@@ -512,7 +512,7 @@ template <typename T> class FastRcg {
     // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u
     double ir[3];
     e_.template reduce<3>(R_kept_dot3<T>{Wphi, gu1_, dB_, uh_, ns_, n_, pp_, M_},
-                          (long long)(M_ + 1) * (2 * M_ + 1) * (2 * M_ + 1), ir);
+                          nkc_, ir);
     const double ir0 = ir[0] / (N6_ * wsc), ir1 = (ir[1] + gam_ * ir[2]) / (N6_ * wsc);
     // vector transport Gamma(u, tau1 d1, xi) = sc xi - c_xi phi_hat (:1746-1747)
     // with phi_hat = u_hat_new / sc, and the retraction u_hat_new = sc phi_hat:
@@ -534,7 +534,7 @@ template <typename T> class FastRcg {
     if (dB_limited_) {
       double r;
       e_.template reduce<1>(R_uh_update_k<T>{uh_, dB_, ns_, n_, pp_, M_, sc * opg, sc * tau1, hlt_},
-                            nk_, &r);
+                            nkc_, &r);
       hk_ *= sc * opg;
       uu_ = (r + hk_ * hk_ * hsum_) / N6_;
     } else {
@@ -560,11 +560,11 @@ template <typename T> class FastRcg {
     std::swap(s_, s1);
     e_.map(F_quad<T>{a_, b_, ee_, a_, (T)tau1, (T)(sc * sc)}, 3 * nr_);   // 3 components
     // masked spectra: only the kept cube changes
-    e_.map(S_quad_k<T>{ph_, P12h_, P12h_ + ns_, ph_, n_, pp_, M_, tau1, sc * sc}, nk_, PC_SPECTRAL);
-    e_.map(S_quad_k<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, n_, pp_, M_, tau1, sc * sc}, nk_,
+    e_.map(S_quad_k<T>{ph_, P12h_, P12h_ + ns_, ph_, n_, pp_, M_, tau1, sc * sc}, nkc_, PC_SPECTRAL);
+    e_.map(S_quad_k<T>{Q0h_, Q12h_, Q12h_ + ns_, Q0h_, n_, pp_, M_, tau1, sc * sc}, nkc_,
            PC_SPECTRAL);
     if (poly_ok_)
-      e_.map(S_quad_k<T>{S0h_, S12h_, S12h_ + ns_, S0h_, n_, pp_, M_, tau1, sc * sc}, nk_,
+      e_.map(S_quad_k<T>{S0h_, S12h_, S12h_ + ns_, S0h_, n_, pp_, M_, tau1, sc * sc}, nkc_,
              PC_SPECTRAL);
     if (!q4_) e_.map(F_quad<T>{Q0_, Q1_, Q2_, Q0_, (T)tau1, (T)(sc * sc)}, nr_);
     W_ = std::move(Wphi);
@@ -796,7 +796,7 @@ template <typename T> class FastRcg {
   int n_, pp_, M_;
   long long nr_, ns_;
   bool q4_, poly_ok_;
-  long long nk_;                      // modes in the kept cube
+  long long nkc_;                     // calls of a kept-cube kernel (kw<T>() modes each)
   double* hwt_ = nullptr;             // Hilbert weight by |m|^2
   double* hlt_ = nullptr;             // squared H_l weight by |m|^2
   size_t tab_bytes_ = 0;

@@ -41,10 +41,17 @@ template <typename T> struct F_copy {   // 16-byte vectors
     st16(o, i, v);
   }
 };
-template <typename T> struct F_scale {
+template <typename T> struct F_scale {   // 16-byte vectors
+  static constexpr int vec = V16<T>::w;
   T* o; T s;
   double abytes() const { return 2.0 * sizeof(T); }
-  __device__ void operator()(long long i) const { o[i] *= s; }
+  __device__ void operator()(long long i) const {
+    T v[vec];
+    ld16(o, i, v);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) v[l] *= s;
+    st16(o, i, v);
+  }
 };
 template <typename T> struct F_divide {
   T* o; double s;
@@ -52,10 +59,18 @@ template <typename T> struct F_divide {
   __device__ void operator()(long long i) const { o[i] = (T)((double)o[i] / s); }
 };
 // o = a*x + b*y
-template <typename T> struct F_axpby {
+template <typename T> struct F_axpby {   // 16-byte vectors; o may alias x or y
+  static constexpr int vec = V16<T>::w;
   const T* x; const T* y; T* o; T a, b;
   double abytes() const { return (x == y ? 2.0 : 3.0) * sizeof(T); }
-  __device__ void operator()(long long i) const { o[i] = a * x[i] + b * y[i]; }
+  __device__ void operator()(long long i) const {
+    T u[vec], v[vec];
+    ld16(x, i, u);
+    ld16(y, i, v);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) u[l] = a * u[l] + b * v[l];
+    st16(o, i, u);
+  }
 };
 template <typename T> struct F_mul {
   const T* a; const T* b; T* o;
