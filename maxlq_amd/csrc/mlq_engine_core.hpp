@@ -327,9 +327,15 @@ template <typename T, int N, int V> struct SVar {
 #define MLQ_FWD_ZB 1    // band-limited forward passes: zero-band last radix pass
 #endif
 template <typename T, int N, int V, int DIR, int MODE>
-void launch_strided_v(hipStream_t st, const SArgs<T>& sa, int ncols, int planes) {
+void launch_strided_v(hipStream_t st, SArgs<T> sa, int ncols, int planes) {
   using S = SVar<T, N, V>;
-  const dim3 grid((ncols + S::C - 1) / S::C, planes), block(S::C * (N / S::E));
+  // pruned z passes that read and write kept columns only run the kept tiles
+  // alone (the other tiles of a full grid exit at once, 57% of them at the
+  // 2/3 mask)
+  sa.kcomp = MODE != 0 && sa.keep >= 0 && 2 * sa.keep + 1 < N && (MODE == 1 || sa.kzero == 0);
+  const unsigned gx = sa.kcomp ? (unsigned)(2 * sa.keep + 1) * (unsigned)((sa.keep + S::C) / S::C)
+                               : (unsigned)((ncols + S::C - 1) / S::C);
+  const dim3 grid(gx, planes), block(S::C * (N / S::E));
   // band-limited launches: zero-band first (inverse) or last (forward) radix pass
   if ((DIR > 0 ? MLQ_XINV_ZB : MLQ_FWD_ZB) && sa.keep >= 0 && 8 * sa.keep < 3 * N) {
     hipLaunchKernelGGL((k_strided<T, N, S::E, S::C, DIR, MODE, true>), grid, block, 0, st, sa,

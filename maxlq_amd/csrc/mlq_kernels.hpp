@@ -149,6 +149,10 @@ template <typename T> struct SArgs {
   // spectrum outside the kept cube, 0 stores the kept cube only (for outputs
   // whose every reader stays inside it)
   int kzero = 1;
+  // z passes of a pruned launch that touch kept columns only (set by the
+  // launcher): the grid covers the kept tiles alone, 2 keep + 1 kept ky rows
+  // times ceil((keep + 1) / C) column groups
+  int kcomp = 0;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -490,12 +494,17 @@ void k_strided(SArgs<T> a, int ncols) {
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   unsigned bx, by;
   xcd_tile(a.remap != 0, bx, by);
-  const int col = (int)bx * C + c;
+  const int keep = a.keep;
+  int col = (int)bx * C + c;
+  if (MODE != 0 && a.kcomp) {
+    const unsigned nt = (unsigned)(keep + C) / C, jy = bx / nt, jt = bx - jy * nt;
+    const int kyt = (int)jy <= keep ? (int)jy : (int)jy + N - 2 * keep - 1, kxt = (int)jt * C + c;
+    col = kxt < PP ? kyt * PP + kxt : ncols;   // a lane past the row pitch is invalid
+  }
   bool valid = col < ncols;
   const int plane = (int)by;
   c2<T> v[E];
   int kx = 0, ky = 0;
-  const int keep = a.keep;
   // kept: the column carries modes inside the kept cube (z passes); rowin(n):
   // element n of the column is read (inverse passes skip masked rows)
   bool kept = true;
