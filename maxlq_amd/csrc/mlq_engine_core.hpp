@@ -314,8 +314,10 @@ template <typename T, int N, int V> struct SVar {
   static constexpr int C0 = cmin > 256 / (N / E0) ? cmin : 256 / (N / E0);
   static constexpr int Er = V == 0 ? E0 : (V == 3 ? 8 : 16);
   static constexpr int Cr = V == 0 ? C0 : (V == 1 ? cmin : (V == 2 ? 2 * cmin : cmin / 2));
+  // the column tile and the twiddle table (N complex doubles) fit the LDS
   static constexpr bool ok = V == 0 || (N >= 128 && Er <= N && (N / Er) * Cr <= 1024 &&
-                                        (long long)N * Cr * 2 * (long long)sizeof(T) <= 160 * 1024);
+                                        (long long)N * Cr * 2 * (long long)sizeof(T) + 16LL * N <=
+                                            160 * 1024);
   static constexpr int E = ok ? Er : E0;
   static constexpr int C = ok ? Cr : C0;
 };

@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
       v[m] = cmk<double>((double)ls_gg(r0[i], r1[i], r2[i], tb),
                          (double)ls_gg(r0[N + i], r1[N + i], r2[N + i], tb));
     }
-    fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, a.tw, L);
+    fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{a.tw}, L);
 #pragma unroll
     for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
     __syncthreads();
@@ -1024,8 +1024,11 @@ __global__ __launch_bounds__(C*(N / E)) void k_zls(LsArgs<T> a, int ncols, long 
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   constexpr int NW = C * Tf / 64;
   __shared__ c2<T> sm[N * C];
+  __shared__ double2 smtw[N];   // twiddle table (read from the second radix pass on)
   __shared__ double red[NW > 0 ? NW : 1][2];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
+  TwStage<N, C * Tf> tws;
+  tws.load(a.tw);
   unsigned bx, by;
   xcd_tile(a.remap != 0, bx, by);
   const int col = (int)bx * C + c;
@@ -1046,7 +1049,8 @@ __global__ __launch_bounds__(C*(N / E)) void k_zls(LsArgs<T> a, int ncols, long 
     for (int m = 0; m < E; ++m) v[m] = cmk<T>((T)0, (T)0);
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<T, N, E, -1, LdsCol<T, C>, ZB>(v, t, a.tw, L);
+  tws.store(smtw);   // ordered before the first twiddle read by the first pass's barrier
+  fft_core<T, N, E, -1, LdsCol<T, C>, ZB>(v, t, TwTab{smtw}, L);
   double s0 = 0.0, s1 = 0.0;
   if (valid) {
     const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;

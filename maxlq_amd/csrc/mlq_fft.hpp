@@ -171,15 +171,75 @@ template <class L> __device__ __forceinline__ void lds_sync() {
   }
 }
 
+// ---- twiddle sources ---------------------------------------------------------------
+// Both multiply x[r], r = 1 .. R-1, of a butterfly with local index k (within
+// its Ns-point sub-transforms) by w^(r k), w = exp(-+2 pi i / (Ns R)), from
+// the FP64 table tw[m] = exp(-2 pi i m / N).
+// TwPow: one table load per butterfly (w^k); the other twiddles follow as
+// FP64 products (within a few FP64 ulps of the table values).  For the row
+// view, whose 64 lanes of a wave need 64 distinct twiddles: measured at
+// 512^3 FP32, one global table load per twiddle cost the passes as much as
+// their whole butterfly arithmetic.
+struct TwPow {
+  const double2* __restrict__ tw;
+  template <int N, int R, int Ns, int DIR>
+  __device__ __forceinline__ void apply(c2<double> (&x)[R], int k) const {
+    const double2 t1 = tw[k * (N / (Ns * R))];
+    const c2<double> w1 = cmk<double>(t1.x, DIR > 0 ? -t1.y : t1.y);
+    c2<double> w = w1;
+    x[1] = cmul(x[1], w);
+#pragma unroll
+    for (int r = 2; r < R; ++r) {
+      w = cmul(w, w1);
+      x[r] = cmul(x[r], w);
+    }
+  }
+};
+// TwTab: every twiddle read from a copy of the table in LDS (the caller fills
+// it and orders the fill before the first pass with twiddles).  For the
+// column view, whose lanes share their twiddles (C adjacent columns per row
+// index: a few distinct addresses per wave, broadcast reads).
+struct TwTab {
+  const double2* tw;
+  template <int N, int R, int Ns, int DIR>
+  __device__ __forceinline__ void apply(c2<double> (&x)[R], int k) const {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      const double2 w = tw[r * k * (N / (Ns * R))];
+      x[r] = cmul(x[r], cmk<double>(w.x, DIR > 0 ? -w.y : w.y));
+    }
+  }
+};
+// Copy of the twiddle table into LDS by the NT threads of a workgroup, in two
+// steps: load() at the kernel's start (the table loads overlap the data
+// loads), store() before the transform's first LDS barrier.
+template <int N, int NT> struct TwStage {
+  static constexpr int PT = (N + NT - 1) / NT;   // entries per thread
+  double2 r[PT];
+  __device__ __forceinline__ void load(const double2* __restrict__ tw) {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int i = threadIdx.x + u * NT;
+      r[u] = tw[i < N ? i : 0];
+    }
+  }
+  __device__ __forceinline__ void store(double2* sm) const {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int i = threadIdx.x + u * NT;
+      if (i < N) sm[i] = r[u];
+    }
+  }
+};
+
 // ---- one Stockham pass -----------------------------------------------------------
 // Precision: every twiddle product and butterfly is evaluated in FP64.  The
 // per-thread register array v has type RT: the storage precision in the
 // strided pass kernels (an FP32 field then holds E complex floats between
 // passes, the same rounding as the LDS exchange, and the kernel stays within
 // 64 VGPRs), FP64 where the caller keeps more registers to spare.
-template <typename RT, int N, int E, int DIR, int P, class L, bool ZB = false>
-__device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
-                                         const L& lds) {
+template <typename RT, int N, int E, int DIR, int P, class L, bool ZB = false, class TW>
+__device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const TW& tw, const L& lds) {
   constexpr int Tf = N / E;
   constexpr int R = radix_at(N, E, P);
   constexpr int Ns = ns_at(N, E, P);
@@ -201,23 +261,7 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
     c2<double> x[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = ccvt<double>(v[b + r * NB]);
-    if constexpr (Ns > 1) {
-      // One table load per butterfly: w = exp(-+2 pi i k / (Ns R)); the
-      // twiddles w^r, r = 2 .. R-1, follow as FP64 products (within a few
-      // FP64 ulps of the table values).  Measured at 512^3 FP32: the vector
-      // memory instructions of one table load per twiddle cost the passes
-      // as much as their whole butterfly arithmetic.
-      const int k = j & (Ns - 1);
-      const double2 t1 = tw[k * (N / (Ns * R))];
-      const c2<double> w1 = cmk<double>(t1.x, DIR > 0 ? -t1.y : t1.y);
-      c2<double> w = w1;
-      x[1] = cmul(x[1], w);
-#pragma unroll
-      for (int r = 2; r < R; ++r) {
-        w = cmul(w, w1);
-        x[r] = cmul(x[r], w);
-      }
-    }
+    if constexpr (Ns > 1) tw.template apply<N, R, Ns, DIR>(x, j & (Ns - 1));
     if constexpr (ZB && DIR > 0 && P == 0 && R == 8) {
       dft8_z34<double, DIR>(x);   // inverse: inputs 3N/8 .. 5N/8 - 1 are zero
     } else {
@@ -239,20 +283,20 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const double2* _
   }
   if constexpr (!last) {
     lds_sync<L>();
-    fft_pass<RT, N, E, DIR, P + 1, L, ZB>(v, t, tw, lds);
+    fft_pass<RT, N, E, DIR, P + 1, L, ZB, TW>(v, t, tw, lds);
   }
 }
 
 // Full length-N transform; DIR = -1 forward (exp(-i...)), +1 inverse
-// (unnormalised).  tw[k] = exp(-2 pi i k / N) in FP64.
+// (unnormalised), twiddles from tw (TwPow or TwTab over the table
+// exp(-2 pi i k / N) in FP64).
 // ZB, for a band-limited launch with keep < 3N/8: an inverse transform's
 // elements 3N/8 .. 5N/8 - 1 are zero on entry (the first radix-8 pass skips
 // them); a forward transform's outputs there are never stored (the last pass
 // leaves them zero).
-template <typename RT, int N, int E, int DIR, class L, bool ZB = false>
-__device__ __forceinline__ void fft_core(c2<RT> (&v)[E], int t, const double2* __restrict__ tw,
-                                         const L& lds) {
-  fft_pass<RT, N, E, DIR, 0, L, ZB>(v, t, tw, lds);
+template <typename RT, int N, int E, int DIR, class L, bool ZB = false, class TW>
+__device__ __forceinline__ void fft_core(c2<RT> (&v)[E], int t, const TW& tw, const L& lds) {
+  fft_pass<RT, N, E, DIR, 0, L, ZB, TW>(v, t, tw, lds);
 }
 
 }  // namespace mlq

@@ -287,7 +287,7 @@ __device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   // ZB (kmax < 3N/8): entries 3N/8 .. 5N/8 - 1 of the packed row are never
   // read by the split below
-  fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, tw, L);
+  fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
   static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
   // The split needs Z[k] and Z[N - k] (rounded to the field precision, as an
   // LDS exchange would).  For k = t + m Tf, m < E/2, element N - k sits in
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   LdsRow<T> L{sm + f * LP};
   // ZB (pruned launches with keep < 3N/8): elements 3N/8 .. 5N/8 - 1 of the
   // packed row are exact zeros
-  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, a.tw, L);
+  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, TwPow{a.tw}, L);
   T* o = a.rdst[b] + row0 * N;
   const double sc = a.scale;
 #pragma unroll
@@ -491,7 +491,10 @@ void k_strided(SArgs<T> a, int ncols) {
   constexpr int estride = MODE == 0 ? PP : N * PP;
   constexpr long long pstride = MODE == 0 ? (long long)N * PP : (long long)N * N * PP;
   __shared__ c2<T> sm[N * C];
+  __shared__ double2 smtw[N];   // twiddle table (read from the second radix pass on)
   const int c = threadIdx.x % C, t = threadIdx.x / C;
+  TwStage<N, C * Tf> tws;
+  tws.load(a.tw);
   unsigned bx, by;
   xcd_tile(a.remap != 0, bx, by);
   const int keep = a.keep;
@@ -643,7 +646,8 @@ void k_strided(SArgs<T> a, int ncols) {
     }
   }
   LdsCol<T, C> L{sm, c};
-  fft_core<T, N, E, DIR, LdsCol<T, C>, ZB>(v, t, a.tw, L);
+  tws.store(smtw);   // ordered before the first twiddle read by the first pass's barrier
+  fft_core<T, N, E, DIR, LdsCol<T, C>, ZB>(v, t, TwTab{smtw}, L);
   if (!valid) return;
   if (MODE == 1 && !kept) return;   // the pruned inverse y pass never reads it
   if (MODE == 2) {
@@ -741,15 +745,21 @@ template <typename T> struct ZArgs {
   SpecOp op[MAXB];    // factors along z only (col_factor_ok)
 };
 // columns per workgroup: 16 (128-byte FP32 segments) while the FP64 column
-// tile N x C fits the 160 KB of LDS, else 8
-template <int N> constexpr int zrt_C() { return (long long)N * 16 * 16 <= 160 * 1024 ? 16 : 8; }
+// tile N x C and the twiddle table fit the 160 KB of LDS, else 8
+template <int N> constexpr int zrt_C() { return (long long)N * 16 * 17 <= 160 * 1024 ? 16 : 8; }
 
 template <typename T, int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   constexpr int estride = N * PP;
   __shared__ double2 sm[N * C];
+  // twiddles from an LDS table while it takes one entry per thread (the
+  // registers that carry it to LDS spilled the 1024-point kernel)
+  constexpr bool TAB = C * Tf >= N;
+  __shared__ double2 smtw[TAB ? N : 1];   // read from the second radix pass on
   const int c = threadIdx.x % C, t = threadIdx.x / C;
+  TwStage<TAB ? N : 1, C * Tf> tws;
+  if constexpr (TAB) tws.load(a.tw);
   unsigned bx, by;
   xcd_tile(a.remap != 0, bx, by);
   const int col = (int)bx * C + c;
@@ -766,14 +776,22 @@ __global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
   for (int m = 0; m < E; ++m)
     v[m] = ok ? ccvt<double>(gld<NTL>(s + (t + m * Tf) * estride)) : cmk<double>(0.0, 0.0);
   LdsCol<double, C> L{reinterpret_cast<c2<double>*>(sm), c};
-  fft_core<double, N, E, -1, LdsCol<double, C>>(v, t, a.tw, L);
   const SpecOp op = a.op[plane];
   const ColFactor cf = col_factor_init(op, op.scale, kx, ky, N);
+  auto round_trip = [&](const auto& tw) {
+    fft_core<double, N, E, -1, LdsCol<double, C>>(v, t, tw, L);
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = cmul(v[m], col_factor(cf, t + m * Tf, N));
-  // the last forward pass read its LDS rows before its barrier: the first
-  // inverse pass may overwrite them
-  fft_core<double, N, E, +1, LdsCol<double, C>>(v, t, a.tw, L);
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], col_factor(cf, t + m * Tf, N));
+    // the last forward pass read its LDS rows before its barrier: the first
+    // inverse pass may overwrite them
+    fft_core<double, N, E, +1, LdsCol<double, C>>(v, t, tw, L);
+  };
+  if constexpr (TAB) {
+    tws.store(smtw);   // ordered before the first twiddle read by the first pass's barrier
+    round_trip(TwTab{smtw});
+  } else {
+    round_trip(TwPow{a.tw});
+  }
   if (!ok) return;
   c2<T>* d = a.spec + a.zout[plane] * a.sstride + col;
 #pragma unroll
