@@ -272,6 +272,18 @@ void Engine<T>::reduce(const F& f, long long n, double* out) {
   for (int k = 0; k < K; ++k) out[k] = red_host_[k];
 }
 
+template <typename T> void Engine<T>::fetch(int n, const C* const* p, std::complex<double>* out) {
+  if (n < 1 || n > FETCH_MAX) throw Error(MLQ_EINVAL, "fetch: count out of range");
+  FetchArgs<T> fa{};
+  for (int i = 0; i < n; ++i) fa.p[i] = p[i];
+  fa.n = n;
+  hipLaunchKernelGGL((k_fetch<T>), dim3(1), dim3(64), 0, st_, fa, red_dev_);
+  MLQ_HIP(hipGetLastError());
+  MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st_));
+  MLQ_HIP(hipStreamSynchronize(st_));
+  for (int i = 0; i < n; ++i) out[i] = std::complex<double>(red_host_[2 * i], red_host_[2 * i + 1]);
+}
+
 template <typename T> double Engine<T>::dot(const T* a, const T* b) {
   double r;
   reduce<1>(R_dot<T>{a, b}, nr_, &r);

@@ -4,6 +4,7 @@
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -215,6 +216,9 @@ template <typename T> class Engine final : public IEngine {
   template <class F> void map(const F& f, long long n, int cls = PC_POINTWISE);
   template <class F> void smap(const F& f) { map(f, ns_, PC_SPECTRAL); }
   template <int K, class F> void reduce(const F& f, long long n, double* out);
+  // the complex values at device addresses p[0..n) (n <= FETCH_MAX), read in
+  // one transfer to pinned host memory and one synchronisation
+  void fetch(int n, const C* const* p, std::complex<double>* out);
   void prof_begin(int cls);
   void prof_end(int cls, double bytes);
 

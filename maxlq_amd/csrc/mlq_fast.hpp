@@ -413,12 +413,10 @@ template <typename T> class FastRcg {
         for (int k = 0; k < 5; ++k) { lpoly_[k] = pa[k]; qpoly_[k] = pa[5 + k]; }
         e_.ls_risky(s_, S1_, S2_, risky_, nrisky_);
         // k = 0 coefficients: sum over the grid of S0, S1, S2
-        C z[3];
-        MLQ_HIP(hipMemcpyAsync(&z[0], S0h_.get(), sizeof(C), hipMemcpyDeviceToHost, e_.st_));
-        MLQ_HIP(hipMemcpyAsync(&z[1], S12h_.get(), sizeof(C), hipMemcpyDeviceToHost, e_.st_));
-        MLQ_HIP(hipMemcpyAsync(&z[2], S12h_ + ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
-        e_.sync();
-        for (int k = 0; k < 3; ++k) ssum_[k] = (double)z[k].x;
+        const C* p[3] = {S0h_.get(), S12h_.get(), S12h_ + ns_};
+        std::complex<double> z[3];
+        e_.fetch(3, p, z);
+        for (int k = 0; k < 3; ++k) ssum_[k] = z[k].real();
       }
     }
     cache_.clear();
@@ -656,14 +654,12 @@ template <typename T> class FastRcg {
   }
   // sum_c |Re (h + g uh)_c(k = 0)|
   double k0_abs_sum(const C* h, double g = 0.0) {
-    C v[3], w[3];
-    for (int c = 0; c < 3; ++c) {
-      MLQ_HIP(hipMemcpyAsync(&v[c], h + c * ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
-      MLQ_HIP(hipMemcpyAsync(&w[c], uh_ + c * ns_, sizeof(C), hipMemcpyDeviceToHost, e_.st_));
-    }
-    e_.sync();
+    const C* p[6];
+    for (int c = 0; c < 3; ++c) { p[c] = h + c * ns_; p[3 + c] = uh_ + c * ns_; }
+    std::complex<double> v[6];
+    e_.fetch(6, p, v);
     double r = 0.0;
-    for (int c = 0; c < 3; ++c) r += std::fabs((double)v[c].x + g * (double)w[c].x);
+    for (int c = 0; c < 3; ++c) r += std::fabs(v[c].real() + g * v[3 + c].real());
     return r;
   }
   void chain_state_from_s() {
@@ -1202,9 +1198,12 @@ void Engine<T>::ls_risky(const T* S0, const T* S1, const T* S2, std::vector<doub
                      cnt, lsrisky_, CAP);
   MLQ_HIP(hipGetLastError());
   prof_end(PC_REDUCE, 3.0 * real_bytes());
-  unsigned c = 0;
-  MLQ_HIP(hipMemcpyAsync(&c, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+  // the count through the pinned reduction buffer (a pageable destination
+  // makes the copy synchronous with a round trip of its own)
+  MLQ_HIP(hipMemcpyAsync(red_host_, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
   MLQ_HIP(hipStreamSynchronize(st_));
+  unsigned c = 0;
+  std::memcpy(&c, red_host_, sizeof(unsigned));
   count = c;
   list.clear();
   if (c > 0 && c <= CAP) {

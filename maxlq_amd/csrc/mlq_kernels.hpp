@@ -973,4 +973,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(F f, long long n, double
 
 __global__ void k_reduce_final(const double* part, int nblk, int K, double* out);
 
+// Gather of up to FETCH_MAX complex values from arbitrary device addresses
+// into one buffer (one host transfer for all of them instead of one each)
+constexpr int FETCH_MAX = 16;
+template <typename T> struct FetchArgs { const c2<T>* p[FETCH_MAX]; int n; };
+template <typename T> __global__ void k_fetch(FetchArgs<T> a, double* out) {
+  const int i = threadIdx.x;
+  if (i < a.n) {
+    const c2<T> v = *a.p[i];
+    out[2 * i] = (double)v.x;
+    out[2 * i + 1] = (double)v.y;
+  }
+}
+
 }  // namespace mlq
