@@ -9,6 +9,8 @@
 // so global loads and stores of every pass kernel use the same
 // (t + m*Tf) pattern and stay coalesced across the batch/column dimension.
 #pragma once
+#include <type_traits>
+
 #include "mlq_common.hpp"
 
 namespace mlq {
@@ -30,6 +32,11 @@ MLQ_CPLX_OPS(double2, double)
 template <typename T, int DIR> __device__ __forceinline__ c2<T> cmul_i(c2<T> a) {
   return DIR > 0 ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
 }
+
+// first radix pass of FP32 transforms in FP32 arithmetic (0: FP64 as the rest)
+#ifndef MLQ_P0_F32
+#define MLQ_P0_F32 1
+#endif
 
 // ---- compile-time radix plan ------------------------------------------------
 constexpr int radix_for(int rem, int E) {
@@ -56,6 +63,16 @@ template <int N> struct FftShape {
 };
 
 // ---- small DFTs, natural order in and out ------------------------------------
+// x * sqrt(1/2); in FP32 with the constant split into two floats (hh + hl
+// within 3e-16 of sqrt(1/2)): a single FP32 constant would scale every such
+// product by the same factor 1 - 1.7e-8
+template <typename T> __device__ __forceinline__ T mul_h(T x);
+template <> __device__ __forceinline__ double mul_h<double>(double x) {
+  return 0.70710678118654752440084436210485 * x;
+}
+template <> __device__ __forceinline__ float mul_h<float>(float x) {
+  return __builtin_fmaf(0.70710676908493042f, x, 1.2101617485882343e-08f * x);
+}
 template <typename T, int DIR> __device__ __forceinline__ void dft2(c2<T>* x) {
   c2<T> a = x[0], b = x[1];
   x[0] = cadd(a, b);
@@ -74,11 +91,10 @@ template <typename T, int DIR> __device__ __forceinline__ void dft8(c2<T>* x) {
   c2<T> o[4] = {x[1], x[3], x[5], x[7]};
   dft4<T, DIR>(e);
   dft4<T, DIR>(o);
-  const T h = (T)0.70710678118654752440084436210485;
   // o[s] *= w8^s, w8 = exp(DIR i pi/4)
-  c2<T> o1 = cmk<T>(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+  c2<T> o1 = cmk<T>(mul_h<T>(o[1].x - DIR * o[1].y), mul_h<T>(o[1].y + DIR * o[1].x));
   c2<T> o2 = cmul_i<T, DIR>(o[2]);
-  c2<T> o3 = cmk<T>(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+  c2<T> o3 = cmk<T>(mul_h<T>(-o[3].x - DIR * o[3].y), mul_h<T>(-o[3].y + DIR * o[3].x));
   x[0] = cadd(e[0], o[0]);
   x[4] = csub(e[0], o[0]);
   x[1] = cadd(e[1], o1);
@@ -99,10 +115,9 @@ template <typename T, int DIR> __device__ __forceinline__ void dft8_z34(c2<T>* x
   const c2<T> os02 = cadd(x[1], x[5]), od02 = csub(x[1], x[5]);
   const c2<T> od13 = cmul_i<T, DIR>(cmk<T>(-x[7].x, -x[7].y));
   c2<T> o[4] = {cadd(os02, x[7]), cadd(od02, od13), csub(os02, x[7]), csub(od02, od13)};
-  const T h = (T)0.70710678118654752440084436210485;
-  c2<T> o1 = cmk<T>(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+  c2<T> o1 = cmk<T>(mul_h<T>(o[1].x - DIR * o[1].y), mul_h<T>(o[1].y + DIR * o[1].x));
   c2<T> o2 = cmul_i<T, DIR>(o[2]);
-  c2<T> o3 = cmk<T>(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+  c2<T> o3 = cmk<T>(mul_h<T>(-o[3].x - DIR * o[3].y), mul_h<T>(-o[3].y + DIR * o[3].x));
   x[0] = cadd(e[0], o[0]);
   x[4] = csub(e[0], o[0]);
   x[1] = cadd(e[1], o1);
@@ -133,6 +148,7 @@ template <typename S> struct LdsRow {
   __device__ __forceinline__ void st(int n, c2<double> v) const {
     base[n + (n >> 3)] = cmk<S>((S)v.x, (S)v.y);
   }
+  __device__ __forceinline__ void st_s(int n, c2<S> v) const { base[n + (n >> 3)] = v; }
 };
 // Column view: C transforms interleaved (element n of column c at n*C + c), so
 // consecutive threads (consecutive columns) touch consecutive LDS words.
@@ -145,6 +161,7 @@ template <typename S, int C> struct LdsCol {
   __device__ __forceinline__ void st(int n, c2<double> v) const {
     base[n * C + c] = cmk<S>((S)v.x, (S)v.y);
   }
+  __device__ __forceinline__ void st_s(int n, c2<S> v) const { base[n * C + c] = v; }
 };
 
 // precision conversion of a complex value (c2<S> is not deducible: one
@@ -254,6 +271,26 @@ __device__ __forceinline__ void fft_pass(c2<RT> (&v)[E], int t, const TW& tw, co
       for (int r = 0; r < R; ++r) v[b + r * NB] = ccvt<RT>(lds.ld(j + r * (N / R)));
     }
     lds_sync<L>();
+  }
+  // The first pass of an FP32 transform (no twiddles, its outputs go to FP32
+  // LDS rows) runs in FP32: its adds round as the LDS store would, and the
+  // sqrt(1/2) products are compensated.  Every later pass is FP64.
+  if constexpr (P == 0 && !last && std::is_same<RT, float>::value &&
+                std::is_same<typename L::Store, float>::value && MLQ_P0_F32) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * Tf;
+      c2<float> x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+      if constexpr (ZB && DIR > 0 && R == 8) dft8_z34<float, DIR>(x);
+      else dftR<float, R, DIR>(x);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.st_s(j * R + r, x[r]);   // Ns = 1: idxD = j R
+    }
+    lds_sync<L>();
+    fft_pass<RT, N, E, DIR, P + 1, L, ZB, TW>(v, t, tw, lds);
+    return;
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
