@@ -42,6 +42,7 @@ template <typename T> Engine<T>::Engine(int n, const mlq_params& p) : p_(p) {
   part_ = (double*)dmalloc(sizeof(double) * std::max(RED_BLOCKS, RED_MAXBLK) * RED_MAXK);
   red_dev_ = (double*)dmalloc(sizeof(double) * 64);
   MLQ_HIP(hipHostMalloc((void**)&red_host_, sizeof(double) * 64, hipHostMallocDefault));
+  MLQ_HIP(hipEventCreateWithFlags(&done_ev_, hipEventDisableTiming));
 }
 
 template <typename T> Engine<T>::~Engine() {
@@ -61,6 +62,7 @@ template <typename T> Engine<T>::~Engine() {
   if (lsrisky_) hipFree(lsrisky_);
   hipFree(red_dev_);
   hipHostFree(red_host_);
+  hipEventDestroy(done_ev_);
   hipStreamDestroy(st_);
 }
 
@@ -268,7 +270,7 @@ void Engine<T>::reduce(const F& f, long long n, double* out) {
   MLQ_HIP(hipGetLastError());
   prof_end(PC_REDUCE, ab);
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * K, hipMemcpyDeviceToHost, st_));
-  MLQ_HIP(hipStreamSynchronize(st_));
+  wait_spin();
   for (int k = 0; k < K; ++k) out[k] = red_host_[k];
 }
 
@@ -280,8 +282,21 @@ template <typename T> void Engine<T>::fetch(int n, const C* const* p, std::compl
   hipLaunchKernelGGL((k_fetch<T>), dim3(1), dim3(64), 0, st_, fa, red_dev_);
   MLQ_HIP(hipGetLastError());
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st_));
-  MLQ_HIP(hipStreamSynchronize(st_));
+  wait_spin();
   for (int i = 0; i < n; ++i) out[i] = std::complex<double>(red_host_[2 * i], red_host_[2 * i + 1]);
+}
+
+// Wait for the stream by polling an event: a result the host needs before
+// its next launch (a reduction, a few scalars) comes back without the
+// wake-up latency of a blocking synchronisation (measured ~25-40 us per
+// read-back between two kernels of the iteration).
+template <typename T> void Engine<T>::wait_spin() {
+  MLQ_HIP(hipEventRecord(done_ev_, st_));
+  for (;;) {
+    const hipError_t e = hipEventQuery(done_ev_);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) MLQ_HIP(e);
+  }
 }
 
 template <typename T> double Engine<T>::dot(const T* a, const T* b) {

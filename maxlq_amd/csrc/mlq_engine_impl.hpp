@@ -66,6 +66,8 @@ template <typename T> class Engine final : public IEngine {
   const mlq_params& params() const override { return p_; }
   void set_params(const mlq_params& p) override { p_ = p; resolve_params(); }
   void sync() override { MLQ_HIP(hipStreamSynchronize(st_)); }
+  // wait for the stream, polling (see the definition)
+  void wait_spin();
   double real_bytes() const override { return (double)nr_ * sizeof(T); }
   // algorithmic bytes of one half spectrum (the row padding is never touched)
   double spec_bytes() const override { return (double)n_ * n_ * px_ * sizeof(C); }
@@ -320,6 +322,7 @@ template <typename T> class Engine final : public IEngine {
   double* lsrisky_ = nullptr;  // clipping-candidate list of the line search
   size_t lsrisky_bytes_ = 0;
   double* red_dev_ = nullptr;
+  hipEvent_t done_ev_ = nullptr;     // recorded and polled by wait_spin
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;
   std::unique_ptr<IEngine> twin_;   // fp64_twin()

@@ -1177,7 +1177,7 @@ void Engine<T>::ls_q4_sums_n(const T* S0, const T* S1, const T* S2, const C* Q0,
   // the z pass reads the y outputs and the kept cubes of Q0..Q2, and stores nothing
   prof_end(PC_ZFWD, nb * ab_kcols(keep) + 3.0 * ab_cube(keep));
   MLQ_HIP(hipMemcpyAsync(red_host_, red_dev_, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st_));
-  MLQ_HIP(hipStreamSynchronize(st_));
+  wait_spin();
   for (int k = 0; k < 2 * nb; ++k) out[k] = red_host_[k];
 }
 
@@ -1201,7 +1201,7 @@ void Engine<T>::ls_risky(const T* S0, const T* S1, const T* S2, std::vector<doub
   // the count through the pinned reduction buffer (a pageable destination
   // makes the copy synchronous with a round trip of its own)
   MLQ_HIP(hipMemcpyAsync(red_host_, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
-  MLQ_HIP(hipStreamSynchronize(st_));
+  wait_spin();
   unsigned c = 0;
   std::memcpy(&c, red_host_, sizeof(unsigned));
   count = c;
