@@ -472,23 +472,27 @@ template <typename T> class FastRcg {
     // phi = u + tau1 d1 = (1 + tau1 gam) u + tau1 dB and the |phi|^k chain
     const double opg = 1.0 + tau1 * gam_;
     RT s1 = e_.rtmp();
-    e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
     typename E_::Chain ch;
     // q = 4 without clipping at tau1: |phi|^2 = D(D(g g) 1), g = sqrt(max(s1, 0)),
     // is s1 itself (s1 is band-limited and positive; each dealias round trip
-    // returns its input up to rounding)
+    // returns its input up to rounding), and ||phi||_4^4 the sum of s1^2 formed
+    // with it
     const bool direct = poly_ok_ && poly_exact(tau1);
     const T* A1;
     const T* L1h;
+    double Lq_sum = 0.0;
     if (direct) {
+      e_.template reduce<1>(R_quad_sq<T>{s_, S1_, S2_, s1, (T)tau1}, nr_, &Lq_sum);
       A1 = s1;
       L1h = s1;
     } else {
+      e_.map(F_quad<T>{s_, S1_, S2_, s1, (T)tau1, (T)1}, nr_);
       e_.chain_init(ch, nullptr, s1);
       A1 = e_.chain_pow(ch, q - 2.0);
       L1h = e_.chain_pow(ch, q / 2.0);
+      Lq_sum = e_.dot(L1h, L1h);
     }
-    const double L1 = std::pow(e_.dot(L1h, L1h) / N3_, 1.0 / q);
+    const double L1 = std::pow(Lq_sum / N3_, 1.0 / q);
     const double sc = B_ / L1;
     const double Lmq = std::pow(L1, -q);
     // spectrum of D(|phi|^(q-2) phi), already scaled to the retracted state
@@ -498,10 +502,16 @@ template <typename T> class FastRcg {
     // retracted field sc phi
     ST Wphi = e_.stmp(3);
     RT unew = e_.rtmp(3);
+    // s1 of the new state, sc^2 s1, stored by the step pass when s1 is A1
+    bool s1_scaled = false;
     {
       C* wd[3] = {Wphi, Wphi + ns_, Wphi + 2 * ns_};
       const SpecOp ops[3] = {sp_mask(M_, wsc), sp_mask(M_, wsc), sp_mask(M_, wsc)};
-      if (!e_.fwd_prod(P_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, unew}, wd, ops)) {
+      P_step<T> ps{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, unew};
+      if (direct) { ps.As = s1; ps.asc = (T)(sc * sc); }
+      if (e_.fwd_prod(ps, wd, ops)) {
+        s1_scaled = direct;
+      } else {
         RT Au = e_.rtmp(3);
         e_.map(F_step<T>{u_, d_, A1, nr_, (T)opg, (T)tau1, (T)sc, Au, unew}, nr_);
         fwd3(Au, Wphi, sp_mask(M_, wsc));
@@ -554,7 +564,7 @@ template <typename T> class FastRcg {
     gA_ = sc * opg;
     gB_ = sc * tau1;
     g_pending_ = true;
-    e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
+    if (!s1_scaled) e_.map(F_scale<T>{s1, (T)(sc * sc)}, nr_);
     std::swap(s_, s1);
     e_.map(F_quad<T>{a_, b_, ee_, a_, (T)tau1, (T)(sc * sc)}, 3 * nr_);   // 3 components
     // masked spectra: only the kept cube changes

@@ -179,11 +179,13 @@ template <typename T> struct F_step {
     }
   }
 };
-// producer form: A phi_c feeds the fused forward x pass, sc phi_c is stored
+// producer form: A phi_c feeds the fused forward x pass, sc phi_c is stored;
+// with As, asc * A is stored too (As may be A: the same point, read first)
 template <typename T> struct P_step {
   static constexpr int NB = 3;
   const T* u; const T* d; const T* A; long long nr; T a, b, sc; T* un;
-  double inbytes() const { return 10.0 * sizeof(T); }   // A, u, d read, u_new written
+  T* As = nullptr; T asc = 1;
+  double inbytes() const { return (As ? 11.0 : 10.0) * sizeof(T); }   // A, u, d read, u_new (and As) written
   __device__ void operator()(long long i, T (&o)[3]) const {
     const T w = gld<NT_PROD>(A + i);
 #pragma unroll
@@ -192,6 +194,26 @@ template <typename T> struct P_step {
       o[c] = w * phi;
       gst<NT_PROD>(un + c * nr + i, (T)(sc * phi));
     }
+    if (As) gst<NT_PROD>(As + i, asc * w);
+  }
+};
+
+// o = x0 + t x1 + t^2 x2 stored, and the sum of o^2 (the q = 4 step: |phi|^2
+// and ||phi||_4^4 in one pass)
+template <typename T> struct R_quad_sq {
+  static constexpr int vec = V16<T>::w;   // 16-byte accesses
+  const T* x0; const T* x1; const T* x2; T* o; T t;
+  double abytes() const { return 4.0 * sizeof(T); }
+  __device__ void operator()(long long iv, double (&acc)[1]) const {
+    T a[vec], b[vec], c[vec], r[vec];
+    ld16(x0, iv, a);
+    ld16(x1, iv, b);
+    ld16(x2, iv, c);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) r[l] = a[l] + t * (b[l] + t * c[l]);
+    st16(o, iv, r);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) acc[0] += (double)r[l] * (double)r[l];
   }
 };
 
