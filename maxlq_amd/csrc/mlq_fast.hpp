@@ -167,34 +167,28 @@ template <typename T> class FastRcg {
   DirOut direction(int iter, int reset_every) {
     DirOut out;
     const double s = 1.5 - 1.0 / q_;
-    // grad p and Laplacian of u
-    RT gp = e_.rtmp(3), lap = e_.rtmp(3);
+    // the force nu Lap u - pc grad p: one spectral pass combines the Laplacian
+    // of u_hat (full spectrum, its pending factor outside the cube) and the
+    // gradient of the masked pressure spectrum, one full inverse transform
+    RT force = e_.rtmp(3);
     {
-      // the pressure spectrum is masked (band-limited pass); u is not
-      InvSrc is[3], il[3];
-      SpecOp ops[3], opl[3];
+      ST fh = e_.stmp(3);
+      e_.map(S_force<T>{uh_, ph_, fh, ns_, n_, pp_, M_, nuvc_, pc_, hk_}, ns_, PC_SPECTRAL);
+      InvSrc is[3];
+      SpecOp ops[3];
       T* dst[3];
-      T* dsl[3];
       for (int j = 0; j < 3; ++j) {
-        is[j] = InvSrc{{ph_.get(), nullptr, nullptr}, {1.0, 0, 0}, 1};
-        ops[j] = sp_ik(j);
-        dst[j] = gp + j * nr_;
-        il[j] = InvSrc{{uh_ + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
-        SpecOp l;   // -|K|^2, and the pending factor of u_hat outside the cube
-        l.flags = SP_NEGKSQ | SP_HSCALE;
-        l.mcut = M_;
-        l.hs = hk_;
-        opl[j] = l;
-        dsl[j] = lap + j * nr_;
+        is[j] = InvSrc{{fh + j * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+        ops[j] = sp_none();
+        dst[j] = force + j * nr_;
       }
-      e_.inv(3, is, ops, dst, M_);
-      e_.inv(3, il, opl, dsl);
+      e_.inv(3, is, ops, dst);
     }
     // aux2 = D(|u|^(q-4) D(u.(nu Lap u - grad p)))           (:1131-1138)
     RT aux2 = e_.rtmp();
     {
       // D(u.force): the product goes straight into its masked spectrum
-      const F_udotforce<T> uf{u_, lap, gp, nr_, (T)nuvc_, (T)pc_, aux2};
+      const F_udotforce<T> uf{u_, force, nullptr, nr_, (T)nuvc_, (T)pc_, aux2};
       ST h = e_.stmp();
       C* hd = h;
       const SpecOp op = sp_mask(M_);
@@ -232,7 +226,7 @@ template <typename T> class FastRcg {
     // and of the normal ||u||_q^(1-q) W (:162-169)
     ST Vh = e_.stmp(3);
     {
-      F_grad_terms<T> gt{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+      F_grad_terms<T> gt{aux2, u_, Acur(), nullptr, G_, gphi, force, nr_, (T)(q_ - 2.0), (T)pc_,
                          (T)nuvc_, nullptr};
       if (g_pending_) {   // the last step's update of G rides along
         gt.Gd0 = Gd_prev_;
@@ -245,7 +239,7 @@ template <typename T> class FastRcg {
       if (!e_.fwd_prod(P_grad_terms<T>{gt}, vd, ops)) {
         flush_G();
         RT Y = e_.rtmp(3);
-        F_grad_terms<T> g2{aux2, u_, Acur(), gp, G_, gphi, lap, nr_, (T)(q_ - 2.0), (T)pc_,
+        F_grad_terms<T> g2{aux2, u_, Acur(), nullptr, G_, gphi, force, nr_, (T)(q_ - 2.0), (T)pc_,
                            (T)nuvc_, Y};
         e_.map(g2, nr_);
         fwd3(Y, Vh, sp_mask(M_));

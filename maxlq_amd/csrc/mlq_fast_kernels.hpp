@@ -242,7 +242,8 @@ template <typename T> struct S_quad {
 
 // Y_i = q2 * aux2 * u_i - pc * A * grad_i p + 2 pc * sum_j d_i u_j d_j phi + vnu * A * Lap u_i
 // (the terms of GradL2ForLq that are dealiased before the final sum,
-// function_ops.f90:1140-1143, 1165-1169, 1178-1190)
+// function_ops.f90:1140-1143, 1165-1169, 1178-1190); with gp null, lap holds
+// the force vnu Lap u - pc grad p
 template <typename T> struct F_grad_terms {
   const T* aux2; const T* u; const T* A; const T* gp; const T* G; const T* gphi; const T* lap;
   long long nr; T q2, pc, vnu; T* Y;
@@ -266,8 +267,13 @@ template <typename T> struct F_grad_terms {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const T t5 = g[0 * 3 + c] * f0 + g[1 * 3 + c] * f1 + g[2 * 3 + c] * f2;
-      y[c] = q2 * a2 * gld<NT_PROD>(u + c * nr + i) - pc * Ai * gld<NT_PROD>(gp + c * nr + i) +
-             (T)2 * pc * t5 + vnu * Ai * gld<NT_PROD>(lap + c * nr + i);
+      if (gp) {
+        y[c] = q2 * a2 * gld<NT_PROD>(u + c * nr + i) - pc * Ai * gld<NT_PROD>(gp + c * nr + i) +
+               (T)2 * pc * t5 + vnu * Ai * gld<NT_PROD>(lap + c * nr + i);
+      } else {
+        y[c] = q2 * a2 * gld<NT_PROD>(u + c * nr + i) + (T)2 * pc * t5 +
+               Ai * gld<NT_PROD>(lap + c * nr + i);
+      }
     }
   }
   __device__ void operator()(long long i) const {
@@ -282,9 +288,9 @@ template <typename T> struct P_grad_terms {
   static constexpr int NB = 3;
   F_grad_terms<T> f;   // Y unused
   double inbytes() const {
-    // grad u, aux2, A, grad phi, u, grad p, Lap u; the pending update of
-    // grad u reads Gd0 and writes grad u back
-    return ((f.A ? 23.0 : 22.0) + (f.Gd0 ? 18.0 : 0.0)) * sizeof(T);
+    // grad u, aux2, A, grad phi, u, grad p, Lap u (or the force); the pending
+    // update of grad u reads Gd0 and writes grad u back
+    return ((f.A ? 23.0 : 22.0) - (f.gp ? 0.0 : 3.0) + (f.Gd0 ? 18.0 : 0.0)) * sizeof(T);
   }
   __device__ void operator()(long long i, T (&o)[3]) const { f.at(i, o); }
 };
@@ -644,6 +650,35 @@ template <typename T> struct S_div_k {
       r[l] = cmk<T>((T)(-im), (T)re);
     }
     kst<T>(o + k.off, k.kx0, M, r);
+  }
+};
+
+// Spectrum of the force nu Lap u - pc grad p over the whole half spectrum:
+// F_c = -vnu |K|^2 h U_c - pc [kept] i K_c P, h the pending factor of U
+// outside the kept cube (1 inside); P (the masked pressure spectrum) is read
+// inside the cube only.  One inverse transform of F replaces those of Lap u
+// and grad p, and the consumers read 3 fields instead of 6.
+template <typename T> struct S_force {
+  const c2<T>* U; const c2<T>* P; c2<T>* F; long long ns; int n, px, M; double vnu, pc, hs;
+  // per element of the padded index space: U read and F written at every
+  // mode (P, read inside the cube only, is not counted)
+  double abytes() const { return 6.0 * sizeof(c2<T>) * (n / 2 + 1) / px; }
+  __device__ void operator()(long long i) const {
+    KIdx k;
+    if (!kdecode(i, n, px, k)) return;   // row padding
+    const bool cube = max(k.mx, max(abs(k.my), abs(k.mz))) <= M;
+    const double K[3] = {TWO_PI * k.mx, TWO_PI * k.my, TWO_PI * k.mz};
+    const double ksq = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
+    const double fu = -vnu * ksq * (cube ? 1.0 : hs);
+    const c2<T> p = cube ? P[i] : cmk<T>((T)0, (T)0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const c2<T> u = U[c * ns + i];
+      // - pc i K_c (p.x + i p.y) = pc K_c p.y - i pc K_c p.x (no Nyquist mode in the cube)
+      const double pk = pc * K[c];
+      F[c * ns + i] = cmk<T>((T)(fu * (double)u.x + pk * (double)p.y),
+                             (T)(fu * (double)u.y - pk * (double)p.x));
+    }
   }
 };
 

@@ -160,14 +160,17 @@ template <typename T> struct F_unit {
   }
 };
 // aux2 = sum_i v_i (vnu * Lap u_i - pc * grad_i p)   (function_ops.f90:1119, 1132)
+// u . (vnu Lap u - pc grad p); with P null, L holds the force itself
 template <typename T> struct F_udotforce {
   const T* v; const T* L; const T* P; long long nr; T vnu, pc; T* o;
   __device__ T at(long long i) const {
     T s = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      s += gld<NT_PROD>(v + c * nr + i) *
-           (vnu * gld<NT_PROD>(L + c * nr + i) - pc * gld<NT_PROD>(P + c * nr + i));
+    for (int c = 0; c < 3; ++c) {
+      const T f = P ? vnu * gld<NT_PROD>(L + c * nr + i) - pc * gld<NT_PROD>(P + c * nr + i)
+                    : gld<NT_PROD>(L + c * nr + i);
+      s += gld<NT_PROD>(v + c * nr + i) * f;
+    }
     return s;
   }
   __device__ void operator()(long long i) const { o[i] = at(i); }
@@ -176,7 +179,7 @@ template <typename T> struct F_udotforce {
 template <typename T> struct P_udotforce {
   static constexpr int NB = 1;
   F_udotforce<T> f;   // o unused
-  double inbytes() const { return 9.0 * sizeof(T); }
+  double inbytes() const { return (f.P ? 9.0 : 6.0) * sizeof(T); }
   __device__ void operator()(long long i, T (&r)[1]) const { r[0] = f.at(i); }
 };
 // o = sum_j u_j g_j (one component of (u . grad) u from g_j = d_j u_i,
