@@ -62,15 +62,18 @@ template <typename T> class FastRcg {
       hl[m2] = w * w;
     }
     tab_bytes_ = sizeof(double) * (size_t)(m2max + 1);
-    hwt_ = (double*)e_.dmalloc(tab_bytes_);
     hlt_ = (double*)e_.dmalloc(tab_bytes_);
-    MLQ_HIP(hipMemcpy(hwt_, hw.data(), tab_bytes_, hipMemcpyHostToDevice));
     MLQ_HIP(hipMemcpy(hlt_, hl.data(), tab_bytes_, hipMemcpyHostToDevice));
+    // both weights of a mode in one 16-byte entry (one gather per mode)
+    std::vector<double2> hwl(m2max + 1);
+    for (int m2 = 0; m2 <= m2max; ++m2) hwl[m2] = make_double2(hw[m2], hl[m2]);
+    hwlt_ = (double2*)e_.dmalloc(2 * tab_bytes_);
+    MLQ_HIP(hipMemcpy(hwlt_, hwl.data(), 2 * tab_bytes_, hipMemcpyHostToDevice));
   }
   ~FastRcg() {
     try {
-      e_.dfree(hwt_, tab_bytes_);
       e_.dfree(hlt_, tab_bytes_);
+      e_.dfree(hwlt_, 2 * tab_bytes_);
     } catch (...) {
     }
   }
@@ -248,16 +251,16 @@ template <typename T> class FastRcg {
       Gd_prev_ = RT();
     }
     ST nH = e_.stmp(3);
-    e_.map(S_dir_prep_k<T>{Vh, W_, nH, ns_, n_, pp_, M_, hwt_, nuvc_, std::pow(L_, 1.0 - q_)},
-           nkc_, PC_SPECTRAL);
-    // every H_l product of the step in one pass; the tangential projection
-    // gproj = Vh - alpha nH (:177), the normalisation (:211-214) and the
-    // Polak-Ribiere numerator (:226) follow by bilinearity
+    // the Hilbert weight and Leray projection of V and of the normal, and in
+    // the same pass every H_l product of the step; the
+    // tangential projection gproj = Vh - alpha nH (:177), the normalisation
+    // (:211-214) and the Polak-Ribiere numerator (:226) follow by bilinearity
     // (V, nH and gradJused0 are band-limited: the kept cube carries them; <u,u>
     // comes from the last update of u_hat)
     double r[10];
-    e_.template reduce<10>(R_dir_k<T>{Vh, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_, pp_,
-                                      M_, hlt_}, nkc_, r);
+    e_.template reduce<10>(R_dir_prep_k<T>{Vh, W_, nH, uh_, have_cg_ ? gu0_.get() : nullptr, ns_, n_,
+                                           pp_, M_, hwlt_, nuvc_, std::pow(L_, 1.0 - q_)},
+                           nkc_, r);
     for (double& x : r) x /= N6_;
     // transported gradient vT_g0 = vg_s gradJused0 + vg_u u (see transport_and_update)
     const double vv = r[0], vn = r[1], nn = r[2], uu = uu_;
@@ -511,10 +514,21 @@ template <typename T> class FastRcg {
         fwd3(Au, Wphi, sp_mask(M_, wsc));
       }
     }
-    // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u
-    double ir[3];
-    e_.template reduce<3>(R_kept_dot3<T>{Wphi, gu1_, dB_, uh_, ns_, n_, pp_, M_},
-                          nkc_, ir);
+    // <D(|phi|^(q-2) phi), xi>_L2 for xi = g_used1 and d1 = dB + gam u; with dB
+    // band-limited the same pass applies the update of u_hat inside the cube
+    // (see below)
+    if (std::fabs(opg) < 1.0e-8)
+      throw Error(MLQ_EUNSUP, "step along a direction dominated by -u/tau (1 + tau gam = 0)");
+    double ir[3], uh_in = 0.0;
+    if (dB_limited_) {
+      double r4[4];
+      e_.template reduce<4>(R_step_k<T>{Wphi, gu1_, uh_, dB_, ns_, n_, pp_, M_, sc * opg, sc * tau1,
+                                        hlt_}, nkc_, r4);
+      for (int k = 0; k < 3; ++k) ir[k] = r4[k];
+      uh_in = r4[3];
+    } else {
+      e_.template reduce<3>(R_kept_dot3<T>{Wphi, gu1_, dB_, uh_, ns_, n_, pp_, M_}, nkc_, ir);
+    }
     const double ir0 = ir[0] / (N6_ * wsc), ir1 = (ir[1] + gam_ * ir[2]) / (N6_ * wsc);
     // vector transport Gamma(u, tau1 d1, xi) = sc xi - c_xi phi_hat (:1746-1747)
     // with phi_hat = u_hat_new / sc, and the retraction u_hat_new = sc phi_hat:
@@ -522,8 +536,6 @@ template <typename T> class FastRcg {
     //   vT_d0 = sc dB / (1 + tau1 gam) + (gam / (1 + tau1 gam) - c1/sc) u_new
     // (u_old = (u_new/sc - tau1 dB) / (1 + tau1 gam)); only scalars are kept
     const double c0 = sc * Lmq * ir0, c1 = sc * Lmq * ir1;
-    if (std::fabs(opg) < 1.0e-8)
-      throw Error(MLQ_EUNSUP, "step along a direction dominated by -u/tau (1 + tau gam = 0)");
     vg_s_ = sc;
     vg_u_ = -c0 / sc;
     vd_B_ = sc / opg;
@@ -533,12 +545,9 @@ template <typename T> class FastRcg {
     // the pending factor hk_ (their stored values times hk_), and their
     // H_l norm is hk_^2 hsum_.  Otherwise the whole spectrum is updated and the
     // pending factor applied.
-    if (dB_limited_) {
-      double r;
-      e_.template reduce<1>(R_uh_update_k<T>{uh_, dB_, ns_, n_, pp_, M_, sc * opg, sc * tau1, hlt_},
-                            nkc_, &r);
+    if (dB_limited_) {   // u_hat updated inside the cube by the pass above
       hk_ *= sc * opg;
-      uu_ = (r + hk_ * hk_ * hsum_) / N6_;
+      uu_ = (uh_in + hk_ * hk_ * hsum_) / N6_;
     } else {
       double r[2];
       e_.template reduce<2>(R_uh_update<T>{uh_, dB_, ns_, n_, pp_, M_, false, sc * opg,
@@ -797,8 +806,8 @@ template <typename T> class FastRcg {
   long long nr_, ns_;
   bool q4_, poly_ok_;
   long long nkc_;                     // calls of a kept-cube kernel (kw<T>() modes each)
-  double* hwt_ = nullptr;             // Hilbert weight by |m|^2
   double* hlt_ = nullptr;             // squared H_l weight by |m|^2
+  double2* hwlt_ = nullptr;           // (Hilbert, squared H_l) weights by |m|^2
   size_t tab_bytes_ = 0;
   double uu_ = 0.0;                   // <u_hat, u_hat>_H_l of the state
   // u_hat outside the kept cube is stored up to the factor hk_; hsum_ is the

@@ -484,19 +484,29 @@ __device__ __forceinline__ void kst(c2<T>* p, int kx0, int M, const c2<T> (&v)[k
   }
 }
 
-// S_dir_prep on the kept cube; hw[m2] = 1 / (1 + (lam |K|)^e2s)
-template <typename T> struct S_dir_prep_k {
-  c2<T>* V; const c2<T>* W; c2<T>* nH; long long ns; int n, px, M; const double* hw;
+// Direction set-up on the kept cube (V, W, G0 band-limited; U's kept part
+// suffices for <V,U>, <N,U>), with hwl[m2] = (hw, hl), hw = 1 / (1 + (lam |K|)^e2s)
+// the Hilbert weight and hl = (1 + (lam |K|)^s)^2 the squared H_l weight:
+//   V := Leray(hw (V - nu |K|^2 W)),  nH := Leray(hw sW W)
+// stored in place, and from the stored (rounded) values in the same pass
+// acc = {<V,V>, <V,N>, <N,N>, <V,G0>, <N,G0>, <V,U>, <N,U>, |K.V|^2,
+// Re(K.V conj K.N), |K.N|^2} (H_l products for the first seven)
+template <typename T> struct R_dir_prep_k {
+  c2<T>* V; const c2<T>* W; c2<T>* nH; const c2<T>* U; const c2<T>* G0; long long ns; int n, px, M;
+  const double2* hwl;
   double nu, sW;
-  double abytes() const { return kept_call_bytes<T>(M, 12.0 * sizeof(c2<T>)); }
-  __device__ void operator()(long long i) const {
+  static constexpr int unroll = 1;   // one call per trip: 15-18 loads of 16 bytes in flight
+  double abytes() const { return kept_call_bytes<T>(M, (G0 ? 18.0 : 15.0) * sizeof(c2<T>)); }
+  __device__ void operator()(long long i, double (&acc)[10]) const {
     constexpr int w = kw<T>();
     const KeptRun k = kept_run<w>(i, n, px, M);
-    c2<T> v[3][w], a[3][w];
+    c2<T> v[3][w], a[3][w], u[3][w], g[3][w];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       kld<T>(V + c * ns + k.off, k.kx0, M, v[c]);
       kld<T>(W + c * ns + k.off, k.kx0, M, a[c]);
+      kld<T>(U + c * ns + k.off, k.kx0, M, u[c]);
+      if (G0) kld<T>(G0 + c * ns + k.off, k.kx0, M, g[c]);
     }
     c2<T> ov[3][w], on[3][w];
 #pragma unroll
@@ -504,8 +514,9 @@ template <typename T> struct S_dir_prep_k {
       const int kx = k.kx0 + l, m2 = kx * kx + k.r2;
       const double e[3] = {TWO_PI * kx, TWO_PI * k.my, TWO_PI * k.mz};
       const double ksq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+      const double2 hw = hwl[m2];
       // the k = 0 mode: both outputs zero (h = 0, and no division by ksq = 0)
-      const double h = m2 == 0 ? 0.0 : hw[m2], f = -nu * ksq, kd = m2 == 0 ? 1.0 : ksq;
+      const double h = m2 == 0 ? 0.0 : hw.x, f = -nu * ksq, kd = m2 == 0 ? 1.0 : ksq;
       double vr[3], vi[3], wr[3], wi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -523,62 +534,39 @@ template <typename T> struct S_dir_prep_k {
         ov[c][l] = cmk<T>((T)(vr[c] - e[c] * dvr), (T)(vi[c] - e[c] * dvi));
         on[c][l] = cmk<T>((T)(wr[c] - e[c] * dwr), (T)(wi[c] - e[c] * dwi));
       }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      kst<T>(V + c * ns + k.off, k.kx0, M, ov[c]);
-      kst<T>(nH + c * ns + k.off, k.kx0, M, on[c]);
-    }
-  }
-};
-
-// R_dir on the kept cube (V, N, G0 band-limited; U's kept part suffices for
-// <V,U>, <N,U>); hl[m2] = (1 + (lam |K|)^s)^2.  acc = {<V,V>, <V,N>, <N,N>,
-// <V,G0>, <N,G0>, <V,U>, <N,U>, |K.V|^2, Re(K.V conj K.N), |K.N|^2}
-template <typename T> struct R_dir_k {
-  const c2<T>* V; const c2<T>* N; const c2<T>* U; const c2<T>* G0; long long ns; int n, px, M;
-  const double* hl;
-  double abytes() const { return kept_call_bytes<T>(M, (G0 ? 12.0 : 9.0) * sizeof(c2<T>)); }
-  __device__ void operator()(long long i, double (&acc)[10]) const {
-    constexpr int w = kw<T>();
-    const KeptRun k = kept_run<w>(i, n, px, M);
-    c2<T> v[3][w], a[3][w], u[3][w], g[3][w];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      kld<T>(V + c * ns + k.off, k.kx0, M, v[c]);
-      kld<T>(N + c * ns + k.off, k.kx0, M, a[c]);
-      kld<T>(U + c * ns + k.off, k.kx0, M, u[c]);
-      if (G0) kld<T>(G0 + c * ns + k.off, k.kx0, M, g[c]);
-    }
-#pragma unroll
-    for (int l = 0; l < w; ++l) {
-      const int kx = k.kx0 + l;
+      // the sums of the stored values (a mode past the cube is zero)
+      const bool in = kx <= M;
       const double m = kx == 0 ? 1.0 : 2.0;
-      const double mw = m * hl[kx * kx + k.r2];
-      const double e[3] = {TWO_PI * kx, TWO_PI * k.my, TWO_PI * k.mz};
-      double vv = 0, vn = 0, nn = 0, vg = 0, ng = 0, vu = 0, nu = 0;
-      double dvr = 0, dvi = 0, dnr = 0, dni = 0;
+      const double mw = in ? m * hw.y : 0.0;
+      double vv = 0, vn = 0, nn = 0, vg = 0, ng = 0, vu = 0, nu2 = 0;
+      double dr = 0, di = 0, dnr = 0, dni = 0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double vx = v[c][l].x, vy = v[c][l].y, ax = a[c][l].x, ay = a[c][l].y;
+        const double vx = ov[c][l].x, vy = ov[c][l].y, ax = on[c][l].x, ay = on[c][l].y;
         const double ux = u[c][l].x, uy = u[c][l].y;
         vv += vx * vx + vy * vy;
         vn += vx * ax + vy * ay;
         nn += ax * ax + ay * ay;
         vu += vx * ux + vy * uy;
-        nu += ax * ux + ay * uy;
+        nu2 += ax * ux + ay * uy;
         if (G0) {
           vg += vx * (double)g[c][l].x + vy * (double)g[c][l].y;
           ng += ax * (double)g[c][l].x + ay * (double)g[c][l].y;
         }
-        dvr += e[c] * vx; dvi += e[c] * vy;
+        dr += e[c] * vx; di += e[c] * vy;
         dnr += e[c] * ax; dni += e[c] * ay;
       }
+      const double mk = in ? m : 0.0;
       acc[0] += mw * vv; acc[1] += mw * vn; acc[2] += mw * nn;
-      acc[3] += mw * vg; acc[4] += mw * ng; acc[5] += mw * vu; acc[6] += mw * nu;
-      acc[7] += m * (dvr * dvr + dvi * dvi);
-      acc[8] += m * (dvr * dnr + dvi * dni);
-      acc[9] += m * (dnr * dnr + dni * dni);
+      acc[3] += mw * vg; acc[4] += mw * ng; acc[5] += mw * vu; acc[6] += mw * nu2;
+      acc[7] += mk * (dr * dr + di * di);
+      acc[8] += mk * (dr * dnr + di * dni);
+      acc[9] += mk * (dnr * dnr + dni * dni);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      kst<T>(V + c * ns + k.off, k.kx0, M, ov[c]);
+      kst<T>(nH + c * ns + k.off, k.kx0, M, on[c]);
     }
   }
 };
@@ -721,41 +709,6 @@ template <typename T> struct R_uh_update {
     acc[cube ? 0 : 1] += w * s;
   }
 };
-// the same update inside the cube only (D band-limited; the modes outside are
-// left with their pending factor): acc[0] = <u_hat, u_hat>_H_l inside
-template <typename T> struct R_uh_update_k {
-  c2<T>* U; const c2<T>* D; long long ns; int n, px, M; double a, b; const double* hl;
-  double abytes() const { return kept_call_bytes<T>(M, 9.0 * sizeof(c2<T>)); }
-  __device__ void operator()(long long i, double (&acc)[1]) const {
-    constexpr int w = kw<T>();
-    const KeptRun k = kept_run<w>(i, n, px, M);
-    double s[w];
-#pragma unroll
-    for (int l = 0; l < w; ++l) s[l] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long o = c * ns + k.off;
-      c2<T> u[w], d[w], r[w];
-      kld<T>(U + o, k.kx0, M, u);
-      kld<T>(D + o, k.kx0, M, d);
-#pragma unroll
-      for (int l = 0; l < w; ++l) {
-        const double ur = a * (double)u[l].x + b * (double)d[l].x;
-        const double ui = a * (double)u[l].y + b * (double)d[l].y;
-        r[l] = cmk<T>((T)ur, (T)ui);
-        s[l] += (double)r[l].x * (double)r[l].x + (double)r[l].y * (double)r[l].y;
-      }
-      kst<T>(U + o, k.kx0, M, r);
-    }
-#pragma unroll
-    for (int l = 0; l < w; ++l) {
-      const int kx = k.kx0 + l;
-      const double m = (kx == 0 || kx == (n >> 1)) ? 1.0 : 2.0;
-      acc[0] += m * hl[kx * kx + k.r2] * s[l];
-    }
-  }
-};
-
 // gu1 = cg (V - alpha N); d = gu1 + beta D0 (D0 null: d = gu1)  (optimization.f90:177, 211-214, 240)
 template <typename T> struct S_dir_update {
   const c2<T>* V; const c2<T>* N; const c2<T>* D0; c2<T>* gu1; c2<T>* d; long long ns;
@@ -941,6 +894,53 @@ template <typename T> struct R_kept_dot3 {
         acc[1] += m * ((double)wv[l].x * b[l].x + (double)wv[l].y * b[l].y);
         acc[2] += m * ((double)wv[l].x * zz[l].x + (double)wv[l].y * zz[l].y);
       }
+    }
+  }
+};
+
+// R_kept_dot3 for (X, D, old U) and the update of U inside the cube in one
+// pass (W masked, D band-limited; the modes outside the cube are left with
+// their pending factor): acc[0..2] = sum m Re(conj(W_c) xi_c) for xi = X, D
+// and the old U; U := a U + b D with acc[3] = <U_new, U_new>_H_l inside the
+// cube, hl[m2] = (1 + (lam |K|)^s)^2
+template <typename T> struct R_step_k {
+  const c2<T>* W; const c2<T>* X; c2<T>* U; const c2<T>* D; long long ns; int n, px, M;
+  double a, b; const double* hl;
+  static constexpr int unroll = 1;   // one call per trip: 12 loads of 16 bytes in flight
+  double abytes() const { return kept_call_bytes<T>(M, 15.0 * sizeof(c2<T>)); }
+  __device__ void operator()(long long i, double (&acc)[4]) const {
+    constexpr int w = kw<T>();
+    const KeptRun k = kept_run<w>(i, n, px, M);
+    double s[w];
+#pragma unroll
+    for (int l = 0; l < w; ++l) s[l] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long o = c * ns + k.off;
+      c2<T> wv[w], xv[w], uv[w], dv[w], r[w];
+      kld<T>(W + o, k.kx0, M, wv);
+      kld<T>(X + o, k.kx0, M, xv);
+      kld<T>(U + o, k.kx0, M, uv);
+      kld<T>(D + o, k.kx0, M, dv);
+#pragma unroll
+      for (int l = 0; l < w; ++l) {
+        const int kx = k.kx0 + l;
+        const double m = (kx == 0 || kx == n / 2) ? 1.0 : 2.0;
+        acc[0] += m * ((double)wv[l].x * xv[l].x + (double)wv[l].y * xv[l].y);
+        acc[1] += m * ((double)wv[l].x * dv[l].x + (double)wv[l].y * dv[l].y);
+        acc[2] += m * ((double)wv[l].x * uv[l].x + (double)wv[l].y * uv[l].y);
+        const double ur = a * (double)uv[l].x + b * (double)dv[l].x;
+        const double ui = a * (double)uv[l].y + b * (double)dv[l].y;
+        r[l] = cmk<T>((T)ur, (T)ui);
+        s[l] += (double)r[l].x * (double)r[l].x + (double)r[l].y * (double)r[l].y;
+      }
+      kst<T>(U + o, k.kx0, M, r);
+    }
+#pragma unroll
+    for (int l = 0; l < w; ++l) {
+      const int kx = k.kx0 + l;
+      const double m = (kx == 0 || kx == (n >> 1)) ? 1.0 : 2.0;
+      acc[3] += m * hl[kx * kx + k.r2] * s[l];
     }
   }
 };
