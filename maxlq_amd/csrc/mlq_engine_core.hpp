@@ -396,6 +396,18 @@ template <int N> constexpr int xpass_NF() {
   return nf < N * N / 2 ? nf : N * N / 2;
 }
 
+// the band of an inverse batch: the effective input op[b] * src[b] is
+// band-limited when the caller says so (keep >= 0) or when every op masks
+template <typename T> int Engine<T>::inv_keep(int nb, const SpecOp* ops, int keep) const {
+  int km = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= n_ / 2) { km = -1; break; }
+    km = std::max(km, ops[b].mcut);
+  }
+  if (km >= 0 && (keep < 0 || km < keep)) keep = km;
+  return keep >= n_ / 2 ? -1 : keep;
+}
+
 // pruned when every output of the batch is masked to a cube smaller than the grid
 template <typename T> int Engine<T>::fwd_keep(int nb, const SpecOp* ops) const {
   int keep = -1;
@@ -486,20 +498,47 @@ void Engine<T>::fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool
   }
 }
 
-template <typename T> template <int N>
-void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {
-  constexpr int NF = xpass_NF<N>();
-  constexpr int Tf = FftShape<N>::Tf;
-  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
-  // the effective input op[b] * src[b] is band-limited when the caller says
-  // so or when every op masks
-  int km = -1;
-  for (int b = 0; b < nb; ++b) {
-    if (!(ops[b].flags & SP_MASK) || ops[b].mcut >= N / 2) { km = -1; break; }
-    km = std::max(km, ops[b].mcut);
+// inverse transforms of P::NI fields (stored to dst[b] where set) and forward
+// transforms of the P::NB fields that P computes per point from them and its
+// own loads, with one x pass for both (k_xinv_fwd_p); false (nothing
+// launched) when the batch does not run that way: the caller then runs inv
+// and fwd_prod.  Both sides band-limited with the zero band (keep < 3N/8).
+template <typename T> template <int N, class P>
+void Engine<T>::inv_fwd_prod_n(const InvSrc* src, const SpecOp* iops, T* const* idst, int ikeep,
+                               const P& prod, C* const* dst, const SpecOp* ops, bool& done) {
+  if constexpr (xinv_fwd_p_fits<P, T, N>()) {
+    constexpr int Tf = FftShape<N>::Tf;
+    const int okeep = fwd_keep(P::NB, ops);
+    auto zb = [](int k) { return k >= 0 && 8 * k < 3 * N; };
+    if (!MLQ_XINV_ZB || !MLQ_FWD_ZB || !zb(okeep) || !zb(inv_keep(P::NI, iops, ikeep))) return;
+    const XArgs<T> ai = inv_zy_n<N>(P::NI, src, iops, idst, ikeep);
+    XArgs<T> ao{};
+    ao.spec = scratch_;
+    ao.tw = tw_;
+    ao.sstride = ns_;
+    ao.nb = P::NB;
+    ao.scale = (T)1;
+    ao.keep = okeep;
+    int nst = 0;
+    for (int b = 0; b < P::NI; ++b) nst += idst[b] != nullptr;
+    prof_begin(PC_XFWD);
+    hipLaunchKernelGGL((k_xinv_fwd_p<T, N, P>), dim3((unsigned)((long long)N * N / 2)),
+                       dim3(P::NB * Tf), 0, st_, prod, ai, ao);
+    MLQ_HIP(hipGetLastError());
+    prof_end(PC_XFWD, P::NI * ab_xcols(ikeep) + nst * real_bytes() +
+                          (double)nr_ * ProdB<P>::of(prod) + P::NB * ab_xcols(okeep));
+    fwd_yz_n<N>(P::NB, dst, ops, okeep, true);
+    nfwd_ += P::NB;
+    done = true;
   }
-  if (km >= 0 && (keep < 0 || km < keep)) keep = km;
-  if (keep >= N / 2) keep = -1;
+}
+
+// z and y passes of an inverse transform; returns the arguments of its x
+// pass (the spectra it reads from the scratch slots, keep, scale)
+template <typename T> template <int N>
+XArgs<T> Engine<T>::inv_zy_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {
+  constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
+  keep = inv_keep(nb, ops, keep);
   ninv_ += nb;
   // Factored plan.  The i K_x and i K_y parts of a spectral factor commute
   // with the z pass (i K_x also with the y pass), so they are applied in the
@@ -564,7 +603,6 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
     sa.op[j] = zop[b];
     zin += src[b].ns;
   }
-  const double rb = (double)nb * real_bytes(), sb = (double)spec_bytes();
   prof_begin(PC_ZINV);
   launch_strided<T, N, 1, 1>(svar_[keep >= 0 ? 6 : 2], st_, sa, N * PP, nz);
   MLQ_HIP(hipGetLastError());
@@ -600,6 +638,16 @@ void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* ds
   xa.nb = nb;
   xa.keep = keep;
   xa.scale = (T)(1.0 / ((double)N * N * N));
+  return xa;
+}
+
+template <typename T> template <int N>
+void Engine<T>::inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep_in) {
+  constexpr int NF = xpass_NF<N>();
+  constexpr int Tf = FftShape<N>::Tf;
+  const XArgs<T> xa = inv_zy_n<N>(nb, src, ops, dst, keep_in);
+  const int keep = xa.keep;
+  const double rb = (double)nb * real_bytes();
   const long long pairs = (long long)nb * N * N / 2;
   prof_begin(PC_XINV);
   if (MLQ_XINV_ZB && keep >= 0 && 8 * keep < 3 * N)   // the first radix pass skips the zero band
@@ -836,6 +884,14 @@ void Engine<T>::fwd(int nb, const T* const* src, C* const* dst, const SpecOp* op
     const int k = std::min(sb_, nb - b0);
     MLQ_DISPATCH_N(fwd_n, k, src + b0, dst + b0, ops + b0);
   }
+}
+template <typename T> template <class P>
+bool Engine<T>::inv_fwd_prod(const InvSrc* src, const SpecOp* iops, T* const* idst, int ikeep,
+                             const P& prod, C* const* dst, const SpecOp* ops) {
+  if (!fused_ || P::NI > sb_ || P::NB > sb_) return false;
+  bool done = false;
+  MLQ_DISPATCH_N(inv_fwd_prod_n, src, iops, idst, ikeep, prod, dst, ops, done);
+  return done;
 }
 template <typename T>
 void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep) {

@@ -235,12 +235,25 @@ template <typename T> class Engine final : public IEngine {
   template <int N> void fwd_yz_n(int nb, C* const* dst, const SpecOp* ops, int keep,
                                  bool kept_only = false);
   int fwd_keep(int nb, const SpecOp* ops) const;
+  int inv_keep(int nb, const SpecOp* ops, int keep) const;
   // forward transforms of the P::NB fields that the producer P computes per
   // point (no intermediate fields); false when it cannot run (the caller
   // then falls back to a pointwise kernel and fwd)
   template <class P> bool fwd_prod(const P& prod, C* const* dst, const SpecOp* ops);
   template <int N, class P> void fwd_prod_n(const P& prod, C* const* dst, const SpecOp* ops, bool& done);
   template <int N> void inv_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep);
+  template <int N>
+  XArgs<T> inv_zy_n(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst, int keep);
+  // inverse transforms of the P::NI inputs (written to idst[b] where set) and
+  // forward transforms of the P::NB fields P computes from them per point,
+  // with one x pass for both; false (nothing launched) when the batch cannot
+  // run that way (the caller then runs inv and fwd_prod)
+  template <class P>
+  bool inv_fwd_prod(const InvSrc* src, const SpecOp* iops, T* const* idst, int ikeep, const P& prod,
+                    C* const* dst, const SpecOp* ops);
+  template <int N, class P>
+  void inv_fwd_prod_n(const InvSrc* src, const SpecOp* iops, T* const* idst, int ikeep,
+                      const P& prod, C* const* dst, const SpecOp* ops, bool& done);
 
   // convenience
   void fwd1(const T* src, C* dst, SpecOp op = sp_none()) { fwd(1, &src, &dst, &op); }

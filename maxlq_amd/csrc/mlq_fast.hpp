@@ -310,23 +310,23 @@ template <typename T> class FastRcg {
     flush_G();
     d_ = e_.rtmp(3);
     Gd_ = e_.rtmp(9);
-    {
-      InvSrc is[12];
-      SpecOp ops[12];
-      T* dst[12];
-      for (int i = 0; i < 3; ++i) {
-        is[i] = InvSrc{{dB_ + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
-        ops[i] = sp_none();
-        dst[i] = d_ + i * nr_;
-        for (int j = 0; j < 3; ++j) {
-          is[3 + i * 3 + j] = is[i];
-          ops[3 + i * 3 + j] = sp_ik(j);
-          dst[3 + i * 3 + j] = Gd_ + (i * 3 + j) * nr_;
-        }
+    // the band-limited part of d and its gradient
+    InvSrc dis[12];
+    SpecOp dops[12];
+    T* ddst[12];
+    for (int i = 0; i < 3; ++i) {
+      dis[i] = InvSrc{{dB_ + i * ns_, nullptr, nullptr}, {1.0, 0, 0}, 1};
+      dops[i] = sp_none();
+      ddst[i] = d_ + i * nr_;
+      for (int j = 0; j < 3; ++j) {
+        dis[3 + i * 3 + j] = dis[i];
+        dops[3 + i * 3 + j] = sp_ik(j);
+        ddst[3 + i * 3 + j] = Gd_ + (i * 3 + j) * nr_;
       }
-      e_.inv(12, is, ops, dst, dB_limited_ ? M_ : -1);   // band-limited part of d
     }
+    const int dkeep = dB_limited_ ? M_ : -1;
     S1_ = e_.rtmp(); S2_ = e_.rtmp(); b_ = e_.rtmp(3); ee_ = e_.rtmp(3);
+    if (!poly_ok_) e_.inv(12, dis, dops, ddst, dkeep);
     if (poly_ok_) {
       // the 12 coefficient fields go straight into their masked spectra (one
       // fused forward pass, no intermediate fields): S1, S2 (kept for the
@@ -340,14 +340,21 @@ template <typename T> class FastRcg {
                     beh + 4 * ns_, beh + 5 * ns_, P12h_, P12h_ + ns_, Q12h_, Q12h_ + ns_};
       SpecOp ops[12];
       for (int k = 0; k < 12; ++k) ops[k] = (k == 8 || k == 9) ? poisson_op() : sp_mask(M_);
-      if (!e_.fwd_prod(P_ls_products<T>{u_, d_, G_, Gd_, nr_, (T)gam_}, dst, ops)) {
-        RT X = e_.rtmp(12);
-        e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, X, X + nr_, X + 2 * nr_, X + 5 * nr_,
-                                X + 8 * nr_, X + 9 * nr_, X + 10 * nr_, X + 11 * nr_, (T)gam_},
-               nr_);
-        const T* src[12];
-        for (int k = 0; k < 12; ++k) src[k] = X + k * nr_;
-        e_.fwd(12, src, dst, ops);
+      // d and grad d come out of the same x pass that forms the products from
+      // them (one read of u and grad u, none of d and grad d) when it can;
+      // else their inverse transforms, then the products
+      const PI_ls_products<T> pi{u_, G_, nr_, (T)gam_};
+      if (!e_.inv_fwd_prod(dis, dops, ddst, dkeep, pi, dst, ops)) {
+        e_.inv(12, dis, dops, ddst, dkeep);
+        if (!e_.fwd_prod(P_ls_products<T>{u_, d_, G_, Gd_, nr_, (T)gam_}, dst, ops)) {
+          RT X = e_.rtmp(12);
+          e_.map(F_ls_products<T>{u_, d_, G_, Gd_, nr_, X, X + nr_, X + 2 * nr_, X + 5 * nr_,
+                                  X + 8 * nr_, X + 9 * nr_, X + 10 * nr_, X + 11 * nr_, (T)gam_},
+                 nr_);
+          const T* src[12];
+          for (int k = 0; k < 12; ++k) src[k] = X + k * nr_;
+          e_.fwd(12, src, dst, ops);
+        }
       }
       // dealiased S1, S2, b, e and the pressure coefficients P0, P1, P2 in
       // one batch of band-limited inverses

@@ -17,20 +17,10 @@ constexpr int RED_MAXK_BIG = 48;
 //   Q(tau)   = |grad phi|^2                   -> Q1, Q2
 // The direction is d = dB + gam u (dB band-limited, gam a scalar) and its
 // gradient Gd = grad dB + gam G: d and Gd here hold the dB parts.
+// uu = u, dd = d, g = grad u, h = grad d at one point
 template <typename T>
-__device__ __forceinline__ void ls_products_at(const T* u, const T* d, const T* G, const T* Gd,
-                                               long long nr, T gam, long long i, T (&o)[12]) {
-  T uu[3], dd[3], g[9], h[9];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    uu[c] = gld<NT_PROD>(u + c * nr + i);
-    dd[c] = gld<NT_PROD>(d + c * nr + i) + gam * uu[c];
-  }
-#pragma unroll
-  for (int a = 0; a < 9; ++a) {
-    g[a] = gld<NT_PROD>(G + a * nr + i);
-    h[a] = gld<NT_PROD>(Gd + a * nr + i) + gam * g[a];
-  }
+__device__ __forceinline__ void ls_products_core(const T (&uu)[3], const T (&dd)[3], const T (&g)[9],
+                                                 const T (&h)[9], T (&o)[12]) {
   o[0] = (T)2 * (uu[0] * dd[0] + uu[1] * dd[1] + uu[2] * dd[2]);
   o[1] = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
 #pragma unroll
@@ -58,6 +48,22 @@ __device__ __forceinline__ void ls_products_at(const T* u, const T* d, const T* 
   o[9] = x2;
   o[10] = (T)2 * q1;
   o[11] = q2;
+}
+template <typename T>
+__device__ __forceinline__ void ls_products_at(const T* u, const T* d, const T* G, const T* Gd,
+                                               long long nr, T gam, long long i, T (&o)[12]) {
+  T uu[3], dd[3], g[9], h[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uu[c] = gld<NT_PROD>(u + c * nr + i);
+    dd[c] = gld<NT_PROD>(d + c * nr + i) + gam * uu[c];
+  }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    g[a] = gld<NT_PROD>(G + a * nr + i);
+    h[a] = gld<NT_PROD>(Gd + a * nr + i) + gam * g[a];
+  }
+  ls_products_core(uu, dd, g, h, o);
 }
 // the 12 coefficient fields of the line search, in the order
 // S1, S2, b (3), e (3), X1, X2, Q1, Q2 (pointwise version: one field each)
@@ -88,6 +94,29 @@ template <typename T> struct P_ls_products {
   double inbytes() const { return 24.0 * sizeof(T); }   // u, d, grad u, grad d
   __device__ void operator()(long long i, T (&o)[12]) const {
     ls_products_at(u, d, G, Gd, nr, gam, i, o);
+  }
+};
+
+// the same 12 values with the dB parts of d and grad d staged by the inverse
+// x pass of the same workgroup (k_xinv_fwd_p): staged input g at s[g * ss],
+// d_0..2 then grad d in the layout of Gd
+template <typename T> struct PI_ls_products {
+  static constexpr int NB = 12, NI = 12;
+  const T* u; const T* G; long long nr; T gam;
+  double inbytes() const { return 12.0 * sizeof(T); }   // u, grad u
+  __device__ void operator()(long long i, const T* s, int ss, T (&o)[12]) const {
+    T uu[3], dd[3], g[9], h[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uu[c] = gld<NT_PROD>(u + c * nr + i);
+      dd[c] = s[c * ss] + gam * uu[c];
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      g[a] = gld<NT_PROD>(G + a * nr + i);
+      h[a] = s[(3 + a) * ss] + gam * g[a];
+    }
+    ls_products_core(uu, dd, g, h, o);
   }
 };
 

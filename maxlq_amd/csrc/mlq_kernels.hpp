@@ -370,25 +370,17 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
   xfwd_pair<T, N, ZB>(v, t, a.tw, L, a.spec + f * a.sstride + row0 * PX, kmax);
 }
 
-// x pass, inverse (C2R): rebuild the packed complex row from the two half
-// spectra (Hermitian extension, real parts taken at kx = 0 and kx = N/2),
-// inverse FFT, unpack, scale.
-template <typename T, int N, int NF, bool PR, bool ZB = false>
-__global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
+// x pass, inverse (C2R) of one row pair, rows row0 and row0 + 1 of the
+// spectrum whose row row0 starts at sp: rebuild the packed complex row from
+// the two half spectra (Hermitian extension, real parts taken at kx = 0 and
+// kx = N/2) and inverse FFT; v[m] = (row0, row0 + 1) at x = t + m Tf,
+// unscaled.  xik: multiply by i K_x first; kmax: modes kx > kmax are zero.
+template <typename T, int N, bool PR, bool ZB>
+__device__ __forceinline__ void xinv_pair(const c2<T>* __restrict__ sp, int kmax, bool xik, int t,
+                                          const double2* tw, const LdsRow<T>& L,
+                                          c2<double> (&v)[FftShape<N>::E]) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
-  constexpr int LP = LdsRow<T>::pitch(N);
   static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
-  __shared__ c2<T> sm[NF * LP];
-  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
-  const long long pairs_per = (long long)N * N / 2;
-  const long long gp = (long long)blockIdx.x * NF + f;
-  const int b = wave_field<N, NF>((int)(gp / pairs_per));
-  const long long lp = gp - (long long)b * pairs_per;
-  const long long row0 = 2 * lp;
-  const int slot = a.xmap ? a.xin[b] : b;
-  const bool xik = a.xmap && a.xik[b];
-  const c2<T>* sp = a.spec + slot * a.sstride + row0 * PX;
-  const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   // Each mode is loaded once, by the thread that owns kx = t + m Tf (m < E/2)
   // in the packed row; slot E/2 of thread 0 holds kx = N/2.  Modes beyond the
   // kept cube are zero: their loads are redirected to kx = 0 and discarded.
@@ -432,7 +424,6 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   // of the same transform (a lane exchange inside the wave, no LDS round
   // trip); for t = 0 it is kx = (E - m) Tf, the thread's own slot E - m.
   const int p = (Tf - t) & (Tf - 1);
-  c2<double> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     c2<T> As, Bs;
@@ -450,10 +441,29 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
     if (k > N / 2) { ay = -ay; by = -by; }
     v[m] = cmk<double>(ax - by, ay + bx);
   }
-  LdsRow<T> L{sm + f * LP};
   // ZB (pruned launches with keep < 3N/8): elements 3N/8 .. 5N/8 - 1 of the
   // packed row are exact zeros
-  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, TwPow{a.tw}, L);
+  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
+}
+
+// x pass, inverse: one row pair per transform (xinv_pair), scaled and stored.
+template <typename T, int N, int NF, bool PR, bool ZB = false>
+__global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
+  constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
+  constexpr int LP = LdsRow<T>::pitch(N);
+  __shared__ c2<T> sm[NF * LP];
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const long long pairs_per = (long long)N * N / 2;
+  const long long gp = (long long)blockIdx.x * NF + f;
+  const int b = wave_field<N, NF>((int)(gp / pairs_per));
+  const long long lp = gp - (long long)b * pairs_per;
+  const long long row0 = 2 * lp;
+  const int slot = a.xmap ? a.xin[b] : b;
+  const bool xik = a.xmap && a.xik[b];
+  const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
+  c2<double> v[E];
+  xinv_pair<T, N, PR, ZB>(a.spec + slot * a.sstride + row0 * PX, kmax, xik, t, a.tw,
+                          LdsRow<T>{sm + f * LP}, v);
   T* o = a.rdst[b] + row0 * N;
   const double sc = a.scale;
 #pragma unroll
@@ -461,6 +471,69 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
     gst<nt_st<PC_XI>()>(o + t + m * Tf, (T)(v[m].x * sc));
     gst<nt_st<PC_XI>()>(o + N + t + m * Tf, (T)(v[m].y * sc));
   }
+}
+
+// Inverse x pass, pointwise producer and forward x pass of one row pair in
+// one workgroup (the x direction is local to a row).  Wave f (f < P::NI)
+// inverse-transforms input f (ai, as k_xinv: the spectra the inverse y pass
+// left in the scratch slots), stores it to ai.rdst[f] when that is set and
+// stages it in its own LDS row; P computes the NB output values of each of
+// the pair's 2N points from the staged inputs and its own loads, staged in
+// place; wave f then transforms output f forward into scratch slot f (as
+// k_xfwd_p).  Every read of the pair's input rows precedes the barrier after
+// the inverse phase, every write of its output rows follows it, and no other
+// workgroup touches these rows, so inputs and outputs may share scratch
+// slots.  Replaces an inverse x pass that writes the NI fields and a producer
+// pass that reads them back.
+template <class P, typename T, int N>
+constexpr bool xinv_fwd_p_fits() {
+  return P::NI <= P::NB && xfwd_p_fits<P, T, N>();
+}
+template <typename T, int N, class P>
+__global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xinv_fwd_p(P prod, XArgs<T> ai,
+                                                                      XArgs<T> ao) {
+  constexpr int NB = P::NB, NI = P::NI, E = FftShape<N>::E, Tf = FftShape<N>::Tf;
+  constexpr int PX = spec_pitch<T>(N), LP = LdsRow<T>::pitch(N);
+  static_assert(NI <= NB && 2 * N <= 2 * LP, "staged rows fit the LDS rows");
+  __shared__ c2<T> sm[NB * LP];
+  T* stg = reinterpret_cast<T*>(sm);   // field f: 2N reals at stg + 2 LP f
+  const long long row0 = 2 * (long long)blockIdx.x;
+  const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
+  const LdsRow<T> L{sm + f * LP};
+  if (f < NI) {   // uniform within each transform's lanes
+    c2<double> v[E];
+    xinv_pair<T, N, true, true>(ai.spec + ai.xin[f] * ai.sstride + row0 * PX, ai.keep,
+                                ai.xik[f] != 0, t, ai.tw, L, v);
+    // the transform's last LDS reads precede its final wave barrier: its own
+    // row is free for the staged values
+    T* o = ai.rdst[f];
+    T* s = stg + 2 * LP * f;
+    const double sc = ai.scale;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const T x0 = (T)(v[m].x * sc), x1 = (T)(v[m].y * sc);
+      s[t + m * Tf] = x0;
+      s[N + t + m * Tf] = x1;
+      if (o) {
+        gst<nt_st<PC_XI>()>(o + row0 * N + t + m * Tf, x0);
+        gst<nt_st<PC_XI>()>(o + row0 * N + N + t + m * Tf, x1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < 2 * N; p += NB * Tf) {
+    T o[NB];
+    prod(row0 * N + p, stg + p, 2 * LP, o);   // staged input g at stg[p + 2 LP g]
+#pragma unroll
+    for (int g = 0; g < NB; ++g) stg[2 * LP * g + p] = o[g];
+  }
+  __syncthreads();
+  const T* s = stg + 2 * LP * f;
+  c2<double> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
+  __syncthreads();   // the transform rewrites the staged rows
+  xfwd_pair<T, N, true>(v, t, ao.tw, L, ao.spec + f * ao.sstride + row0 * PX, ao.keep);
 }
 
 // ---------------------------------------------------------------------------
