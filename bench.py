@@ -352,15 +352,16 @@ def bench_n1024(args):
     from maxlq_amd.lq import MaxLq
     n, q, B = 1024, 9.0, 2.2
     probe = MaxLq(n, q=q, B=B, precision="mixed")
-    resident = probe.uses_resident_state()
+    path = {1: "resident-state", 2: "reference-structured, line search from coefficient fields",
+            0: "reference-structured"}[probe.rcg_path()]
     probe.close()
-    log(f"1024^3 q=9: {'resident-state' if resident else 'reference-structured'} iteration")
+    log(f"1024^3 q=9: {path} iteration")
     g, h, res, t_it, _ = run_rcg(n, q, B, "mixed", 3, 0, 1, seed=5, a=1024.0)
     used, peak, free, total = g.mem_info()
     out = {"metric": "RCG iteration 1024^3 q=9 mixed: s/iter and memory peak",
            "value": t_it, "unit": "s/iter", "higher_is_better": False,
            "config": {"workload": "1024^3 q=9 B=2.2 seed 5 exp(-|m|^2/1024)", "J1": res.J1,
-                      "iteration": "resident-state" if resident else "reference-structured",
+                      "iteration": path,
                       "n_J": res.n_J, "mem_peak_GB": peak / 1e9, "mem_peak_GiB": peak / 2**30,
                       "mem_in_use_GB": used / 1e9, "device_total_GiB": total / 2**30,
                       "device_free_GB": free / 1e9}}

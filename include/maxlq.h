@@ -242,11 +242,15 @@ int mlq_kappa_test_pert(mlq_ctx* ctx, mlq_field out, int kind, double m1, double
 int mlq_kappa_test(mlq_ctx* ctx, mlq_field phi, mlq_field grad, mlq_field pert, int mysystem,
                    int inner_kind, double* table, double* frac, double* best_kappa, int* best_row);
 
-/* Which iteration mlq_rcg_solve and mlq_eval_J_batch will run with the
- * current parameters: 1 the resident-state iteration (q >= 4 without the
- * e_u variant, dealiased, and its buffers fit in the device memory left),
- * 0 the reference-structured one (every operator recomputed from the field,
- * a smaller footprint).  No counterpart in the reference. */
+/* Which iteration mlq_rcg_solve will run with the current parameters: 1 the
+ * resident-state iteration (q >= 4 without the e_u variant, dealiased, and
+ * its buffers fit in the device memory left), 2 the reference-structured
+ * gradient and direction with the line search from coefficient fields
+ * (MLQ_RCG_LEAN: where the resident state applies but does not fit), 0 the
+ * reference-structured one (every operator recomputed from the field, the
+ * smallest footprint).  mlq_eval_J_batch takes the resident state for 1,
+ * else the reference-structured evaluation.  No counterpart in the
+ * reference. */
 int mlq_rcg_path(const mlq_ctx* ctx, int* resident_state);
 
 /* ---- the whole RCG solve for one B: maxdLqdt, optimization.f90:7-514 ---- */
@@ -285,6 +289,15 @@ typedef struct mlq_rcg_cfg {
  * end, of the H_l gradient (and, if the solve did not converge, of the
  * direction d1) at uvec0 (:457-469); results as MLQ_LOG_KAPPA records */
 #define MLQ_RCG_KAPPA 4
+/* cfg.flags: the reference-structured gradient and direction with the line
+ * search from coefficient fields (the 18 tau-coefficient fields of
+ * D(phi.phi), w_j, the pressure and |grad phi|^2 for phi = u + tau d1,
+ * formed once per iteration; each J(tau) then evaluates the |phi|^k chain and
+ * the R sums from them, the retraction by homogeneity).  The solve takes this
+ * line search by itself where the resident state applies but does not fit
+ * (mlq_rcg_path 2); the flag asks for it at any size (q >= 4 without the e_u
+ * variant, dealiased).  No counterpart in the reference. */
+#define MLQ_RCG_LEAN 8
 
 typedef struct mlq_rcg_result {
   double J1;             /* final R(u) */

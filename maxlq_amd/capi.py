@@ -26,6 +26,7 @@ MLQ_PERT_SINE, MLQ_PERT_DIVFREE_SINE, MLQ_PERT_LOAD_RAW, MLQ_PERT_LOAD = 0, 1, 2
 MLQ_PERT_LOAD_SMOOTH, MLQ_PERT_LOAD_SMOOTH_RAW, MLQ_PERT_LOAD_K = 4, 5, 6
 MLQ_PERT_LOAD_POLY, MLQ_PERT_LOAD_EXP, MLQ_PERT_RANDOM_POLY, MLQ_PERT_RANDOM = 7, 8, 9, 10
 MLQ_RCG_REFERENCE = 1   # mlq_rcg_cfg.flags: reference-structured iteration
+MLQ_RCG_LEAN = 8        # mlq_rcg_cfg.flags: line search from coefficient fields
 
 # symbols the header declares (checked by the CPU test suite)
 EXPORTS = [

@@ -398,7 +398,7 @@ int mlq_eval_J_batch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus,
 int mlq_rcg_path(const mlq_ctx* ctx, int* resident_state) {
   return guarded(const_cast<mlq_ctx*>(ctx), [&] {
     if (!resident_state) throw Error(MLQ_EINVAL, "null output");
-    *resident_state = ctx->eng->uses_fast_path() ? 1 : 0;
+    *resident_state = ctx->eng->rcg_path();
   });
 }
 int mlq_time_linesearch(mlq_ctx* ctx, mlq_field u, mlq_field d, const double* taus, int ntau,

@@ -92,6 +92,9 @@ class IEngine {
 
   // whether rcg_solve / eval_J_batch take the resident-state iteration
   virtual bool uses_fast_path() const = 0;
+  // the iteration rcg_solve takes: 1 resident state, 2 reference-structured
+  // with the line search from coefficient fields, 0 reference-structured
+  virtual int rcg_path() const = 0;
   // the RCG solve (optimization.f90:7-514)
   virtual void rcg_solve(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res,
                          const LogSink& log) = 0;

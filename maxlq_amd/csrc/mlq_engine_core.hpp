@@ -786,7 +786,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
       za.zout[j] = pl.zout[j];
       za.op[j] = pl.zop[j];
     }
-    constexpr int E = FftShape<N>::E, CZ = zrt_C<N>();
+    constexpr int E = FftShape<N>::E, CZ = zrt_C<T, N>();
     const int ncols = N * PP;
     prof_begin(PC_ZRT);
     hipLaunchKernelGGL((k_zrt<T, N, E, CZ>), dim3((ncols + CZ - 1) / CZ, pl.nz), dim3(CZ * (N / E)),

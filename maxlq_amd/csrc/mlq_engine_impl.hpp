@@ -191,7 +191,9 @@ template <typename T> class Engine final : public IEngine {
   // (the |phi|^k chain of calc_uk on D(phi.phi)): out = {sum |phi|^(q-2)
   // |grad phi|^2, sum D(|phi|^(q-4)) w.w, sum D(p |phi|^(q-4)) phi.w,
   // sum (|phi|^(q/2))^2}, unnormalised (function_ops.f90:1304-1358, 1675-1692)
-  void ls_generic(const LsGen<T>& f, double t, double out[4]);
+  // the chain of |phi|^k is that of sqrt(s2) phi (D(phi.phi) scaled by s2);
+  // the other factors of the sums are those of phi
+  void ls_generic(const LsGen<T>& f, double t, double out[4], double s2 = 1.0);
   // an FP64 engine on the same grid and parameters (created on first use):
   // the mixed path evaluates its line-search J(tau) chain there, so that J is
   // a smooth function of tau (brent resolves tau to TOL = 1e-6, J to 1e-12)
@@ -376,11 +378,23 @@ template <typename T> class Engine final : public IEngine {
     if (!q4 && sizeof(T) == 4) b += 32.0 * 8.0 * (double)nr_ + 12.0 * 16.0 * (double)n_ * n_ * (n_ / 2 + 8);
     return b;
   }
+  // the reference-structured iteration takes its line search from
+  // coefficient fields (LeanLs) when the resident state applies but does not
+  // fit, and those fields fit in the device memory left beside `held` more
+  // real fields
+  bool lean_ls_applies() const { return fast_path_applies() && !fast_path_fits(); }
+  bool lean_ls_fits(int held) const;
  public:
   bool uses_fast_path() const override { return fast_path_ok(); }
+  // (before a solve: beside its 19 component buffers)
+  int rcg_path() const override {
+    if (fast_path_ok()) return 1;
+    return lean_ls_applies() && lean_ls_fits(19) ? 2 : 0;
+  }
  private:
 
   template <typename U> friend class FastRcg;
+  template <typename U> friend struct LeanLs;
   template <typename U> friend struct KappaInSolve;
   int kappa_fields_ = 0;   // real vector fields the solve's kappa tests hold (0 or 4)
   template <typename U>

@@ -1024,10 +1024,11 @@ void rcg_solve_fast(Engine<T>& e, int u, const mlq_rcg_cfg& cfg, mlq_rcg_result&
 
 namespace mlq {
 
-template <typename T> void Engine<T>::ls_generic(const LsGen<T>& f, double t, double out[4]) {
+template <typename T>
+void Engine<T>::ls_generic(const LsGen<T>& f, double t, double out[4], double s2) {
   const double q = p_.q;
   RT s1 = rtmp();
-  map(F_quad<T>{f.S0, f.S1, f.S2, s1, (T)t, (T)1}, nr_);
+  map(F_quad<T>{f.S0, f.S1, f.S2, s1, (T)t, (T)s2}, nr_);
   Chain ch;
   chain_init(ch, nullptr, s1);
   const T* A = chain_pow(ch, q - 2.0);

@@ -120,6 +120,79 @@ template <typename T> struct PI_ls_products {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Line-search fields with a small footprint (the velocity gradients one
+// column at a time): the same fields as ls_products_at and the state's
+// s, w, |grad u|^2 and pressure source, for phi = u + tau d with d a full
+// physical field.
+// S1 = 2 u.d, S2 = d.d (the tau and tau^2 coefficients of phi.phi)
+template <typename T> struct F_uddots {
+  const T* u; const T* d; long long nr; T* S1; T* S2;
+  __device__ void operator()(long long i) const {
+    T ud = 0, dd = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T uc = u[c * nr + i], dc = d[c * nr + i];
+      ud += uc * dc;
+      dd += dc * dc;
+    }
+    S1[i] = (T)2 * ud;
+    S2[i] = dd;
+  }
+};
+// Column j of grad u and grad d, Gj = {d_j u_0..2, d_j d_0..2}: the j
+// components of w(tau) = a + tau b + tau^2 e (w_j = sum_i phi_i d_j phi_i),
+// and the column's part of |grad phi|^2 = Q0 + tau Q1 + tau^2 Q2 and of the
+// pressure source X(tau) = sum_ac d_a phi_c d_c phi_a (its a = c = j term),
+// written to Q and X (3 coefficient fields each) when first, else added
+template <typename T> struct F_ls_column {
+  const T* u; const T* d; const T* Gj; long long nr; int j; int first;
+  T* a; T* b; T* e; T* Q; T* X;
+  __device__ void operator()(long long i) const {
+    T g[3], h[3], uu[3], dd[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uu[c] = u[c * nr + i];
+      dd[c] = d[c * nr + i];
+      g[c] = Gj[c * nr + i];
+      h[c] = Gj[(3 + c) * nr + i];
+    }
+    T aj = 0, bj = 0, ej = 0, q0 = 0, q1 = 0, q2 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      aj += uu[c] * g[c];
+      bj += uu[c] * h[c] + dd[c] * g[c];
+      ej += dd[c] * h[c];
+      q0 += g[c] * g[c];
+      q1 += g[c] * h[c];
+      q2 += h[c] * h[c];
+    }
+    a[i] = aj;
+    b[i] = bj;
+    e[i] = ej;
+    const T gj = j == 0 ? g[0] : (j == 1 ? g[1] : g[2]);
+    const T hj = j == 0 ? h[0] : (j == 1 ? h[1] : h[2]);
+    const T qv[3] = {q0, (T)2 * q1, q2}, xv[3] = {gj * gj, (T)2 * gj * hj, hj * hj};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Q[k * nr + i] = first ? qv[k] : Q[k * nr + i] + qv[k];
+      X[k * nr + i] = first ? xv[k] : X[k * nr + i] + xv[k];
+    }
+  }
+};
+// the a != c terms of the pressure source for one pair a < c, from
+// P4 = {d_a u_c, d_c u_a, d_a d_c, d_c d_a} (the (a, c) and (c, a) terms are
+// equal), added to X
+template <typename T> struct F_ls_pair {
+  const T* P4; long long nr; T* X;
+  __device__ void operator()(long long i) const {
+    const T g1 = P4[i], g2 = P4[nr + i], h1 = P4[2 * nr + i], h2 = P4[3 * nr + i];
+    X[i] += (T)2 * (g1 * g2);
+    X[nr + i] += (T)2 * (g1 * h2 + h1 * g2);
+    X[2 * nr + i] += (T)2 * (h1 * h2);
+  }
+};
+
 // q = 4 polynomial coefficients (11 global sums):
 //   sum_j w_j(tau)^2 with w = a + tau b + tau^2 e          -> c0..c4
 //   p(tau) * sum_j phi_j w_j, p = P0 + tau P1 + tau^2 P2   -> r0..r5

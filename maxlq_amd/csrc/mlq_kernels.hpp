@@ -817,15 +817,24 @@ template <typename T> struct ZArgs {
   int zout[MAXB];
   SpecOp op[MAXB];    // factors along z only (col_factor_ok)
 };
-// columns per workgroup: 16 (128-byte FP32 segments) while the FP64 column
-// tile N x C and the twiddle table fit the 160 KB of LDS, else 8
-template <int N> constexpr int zrt_C() { return (long long)N * 16 * 17 <= 160 * 1024 ? 16 : 8; }
+// LDS element of the round trip's column tile: FP64 (one rounding, at the
+// store), except for FP32 fields at N >= 1024, where the FP64 tile would
+// leave one 512-thread workgroup per CU (2 waves per SIMD): FP32 rows between
+// the radix passes there, as in the separate z passes
+template <typename T, int N>
+using ZrtLds = typename std::conditional<(sizeof(T) == 4 && N >= 1024), float, double>::type;
+// columns per workgroup: 16 (128-byte FP32 segments) while the column tile
+// N x C and the twiddle table fit the 160 KB of LDS, else 8
+template <typename T, int N> constexpr int zrt_C() {
+  return (long long)N * (16 * (long long)sizeof(c2<ZrtLds<T, N>>) + 16) <= 160 * 1024 ? 16 : 8;
+}
 
 template <typename T, int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
   constexpr int Tf = N / E, PP = spec_pitch<T>(N);
   constexpr int estride = N * PP;
-  __shared__ double2 sm[N * C];
+  using LS = ZrtLds<T, N>;
+  __shared__ c2<LS> sm[N * C];
   // twiddles from an LDS table while it takes one entry per thread (the
   // registers that carry it to LDS spilled the 1024-point kernel)
   constexpr bool TAB = C * Tf >= N;
@@ -848,16 +857,16 @@ __global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
 #pragma unroll
   for (int m = 0; m < E; ++m)
     v[m] = ok ? ccvt<double>(gld<NTL>(s + (t + m * Tf) * estride)) : cmk<double>(0.0, 0.0);
-  LdsCol<double, C> L{reinterpret_cast<c2<double>*>(sm), c};
+  LdsCol<LS, C> L{sm, c};
   const SpecOp op = a.op[plane];
   const ColFactor cf = col_factor_init(op, op.scale, kx, ky, N);
   auto round_trip = [&](const auto& tw) {
-    fft_core<double, N, E, -1, LdsCol<double, C>>(v, t, tw, L);
+    fft_core<double, N, E, -1, LdsCol<LS, C>>(v, t, tw, L);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], col_factor(cf, t + m * Tf, N));
     // the last forward pass read its LDS rows before its barrier: the first
     // inverse pass may overwrite them
-    fft_core<double, N, E, +1, LdsCol<double, C>>(v, t, tw, L);
+    fft_core<double, N, E, +1, LdsCol<LS, C>>(v, t, tw, L);
   };
   if constexpr (TAB) {
     tws.store(smtw);   // ordered before the first twiddle read by the first pass's barrier

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <vector>
 
+#include "mlq_lean_ls.hpp"
 #include "mlq_linesearch.hpp"
 #include "mlq_ops_core.hpp"
 
@@ -167,11 +168,20 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
     calc0thFourierModes(u, m0);
     if (std::fabs(m0[0]) + std::fabs(m0[1]) + std::fabs(m0[2]) > 1.0000000474974513e-03 * B) msg(iter, 44);
-    // line search on J(tau) = eval_J(u + tau d1, "LineMin"), every value
-    // computed from the field as eval_J does (optimization.f90:531-539)
+    // line search on J(tau) = eval_J(u + tau d1, "LineMin"): every value
+    // computed from the field as eval_J does (optimization.f90:531-539), or,
+    // where the resident state applies but does not fit (and with
+    // MLQ_RCG_LEAN), from the coefficient fields of u and d1 formed once here
+    std::unique_ptr<LeanLs<T>> lean;
+    if (!(cfg.flags & MLQ_RCG_REFERENCE) &&
+        ((cfg.flags & MLQ_RCG_LEAN) ? fast_path_applies() : lean_ls_applies()) && lean_ls_fits(0)) {
+      lean.reset(new LeanLs<T>(*this));
+      lean->build(fphys(u), fphys(d1));
+    }
     JFun J = [&](double tau) {
       double v;
-      eval_J_batch_ref(u, d1, &tau, 1, &v);
+      if (lean) v = lean->J(tau);
+      else eval_J_batch_ref(u, d1, &tau, 1, &v);
       ++nJ; ++nJc;
       if (std::isnan(v)) throw Error(MLQ_ENAN, "nan in J");
       return v;
@@ -200,6 +210,7 @@ void Engine<T>::rcg_solve_ref(int u, const mlq_rcg_cfg& cfg, mlq_rcg_result& res
     msg(iter, 21);
     msg(iter, 30);
     BrentResult bres = brent(J, br.tA, br.tC, blog);                // :297
+    lean.reset();
     tau1 = bres.X;
     if (tau1 > br.tC) {
       if (tau1 > 2.0 * br.tC) msg(iter, 32);

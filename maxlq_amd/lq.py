@@ -301,11 +301,16 @@ class MaxLq:
         return out
 
     # -- optimization: maxdLqdt ------------------------------------------------------------
-    def uses_resident_state(self):
-        """True when mlq_rcg_solve runs the resident-state iteration."""
+    def rcg_path(self):
+        """The iteration mlq_rcg_solve runs: 1 resident state, 2 reference-structured
+        with the line search from coefficient fields, 0 reference-structured."""
         r = C.c_int()
         self._chk(self.lib.mlq_rcg_path(self.ctx, C.byref(r)))
-        return bool(r.value)
+        return r.value
+
+    def uses_resident_state(self):
+        """True when mlq_rcg_solve runs the resident-state iteration."""
+        return self.rcg_path() == 1
 
     def time_linesearch(self, hu, hd, taus, batch):
         """J_LineMin at `taus` after one coefficient precompute, in launches of

@@ -20,6 +20,11 @@ def test_config5_1024_q9_one_iteration(monkeypatch):
     g = MaxLq(n, q=q, B=B, precision="mixed")
     h = g.field(3)
     g.seeded_field(h, 5, "gauss", 1024.0)
+    # the resident state does not fit: the reference-structured gradient with
+    # the line search from coefficient fields
+    path = g.rcg_path()
+    print("1024^3 q=9 mixed: iteration path", path)
+    assert path == 2
     res, tr = g.maxdLqdt(h, max_iter=1, optim_tol=0.0, batch_linesearch=3)
     used, peak, free, total = g.mem_info()
     _, live_peak = g.mem_live()

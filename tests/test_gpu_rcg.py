@@ -9,6 +9,7 @@ from tests.fields import rel
 pytestmark = pytest.mark.gpu
 
 REF = 1  # MLQ_RCG_REFERENCE
+LEAN = 8  # MLQ_RCG_LEAN: line search from coefficient fields
 
 
 def _solve_gpu(u0, n, q, B, iters, flags=0, batch=0, precision="fp64"):
@@ -40,7 +41,8 @@ def _oracle_solve(u0, n, q, B, iters):
     return L.maxdLqdt(orc, u0)
 
 
-@pytest.mark.parametrize("q,flags", [(4.0, 0), (4.0, REF), (6.0, 0), (9.0, 0)])
+@pytest.mark.parametrize("q,flags", [(4.0, 0), (4.0, REF), (6.0, 0), (9.0, 0), (4.0, LEAN),
+                                     (6.0, LEAN), (9.0, LEAN)])
 def test_rcg_trajectory_fp64(q, flags):
     n, B, iters = 16, 1.0, 4
     u0 = seeded_field(n, 1, "gauss", 16.0, q=q, B=B)
@@ -74,6 +76,33 @@ def test_rcg_fast_matches_reference_path_32():
     assert rel(u2, u1) < 1e-9
     assert r2.n_J == r1.n_J
     assert r2.n_J_computed <= r2.n_J + 2 * r2.iters
+
+
+@pytest.mark.parametrize("q,precision", [(6.0, "fp64"), (9.0, "fp64"), (9.0, "mixed")])
+def test_rcg_lean_linesearch_matches_reference_path_32(q, precision):
+    """The line search from coefficient fields (the path of grids where the
+    resident state does not fit) against the reference-structured solve, which
+    evaluates every J(tau) from the field: same iterations, mnbrak / brent
+    step sizes and values, final field (FP64 to rounding; mixed within the
+    rounding of FP32 fields)."""
+    n, B, iters = 32, 1.0, 3
+    u0 = seeded_field(n, 4, "gauss", 16.0, q=q, B=B)
+    r1, t1, u1 = _solve_gpu(u0, n, q, B, iters, flags=REF, precision=precision)
+    r2, t2, u2 = _solve_gpu(u0, n, q, B, iters, flags=LEAN, precision=precision)
+    assert r1.iters == r2.iters
+    if precision == "fp64":
+        assert r1.n_J == r2.n_J and len(t1["br"]) == len(t2["br"])
+        for a, b in zip(t1["br"], t2["br"]):
+            assert a[:2] == b[:2]
+            assert abs(a[2] - b[2]) <= 1e-8 * max(abs(a[2]), 1e-300)
+            assert abs(a[3] - b[3]) <= 1e-10 * abs(a[3])
+    # mixed: both evaluate J from FP32 fields in different orders (the mixed
+    # path's J agrees with FP64 to 1e-4 on this case, test_mixed_vs_fp64_rcg_32)
+    tj, tu = (1e-10, 1e-9) if precision == "fp64" else (1e-4, 1e-3)
+    for a, b in zip(t1["iter"], t2["iter"]):
+        print(precision, "J per iteration, reference-structured vs lean:", a[3], b[3])
+        assert abs(a[3] - b[3]) <= tj * abs(a[3])
+    assert rel(u2, u1) < tu
 
 
 def test_rcg_fast_unfused_matches_fused_32(monkeypatch):
