@@ -99,7 +99,7 @@ def iteration_algorithmic_bytes(g, h, batch=3):
 def fft_roofline(g, batch=12, reps=5):
     """Per-pass average launch durations (HIP events on the library stream)
     and algorithmic bytes (input + output buffer bytes per launch)."""
-    t = g.time_fft(batch, reps)
+    t, batch = g.time_fft(batch, reps, with_batch=True)   # the batch the scratch holds
     rb, sb = g.sizes()
     passes = ["x_fwd", "y_fwd", "z_fwd", "z_inv", "y_inv", "x_inv"]
     byts = [batch * (rb + sb), 2 * batch * sb, 2 * batch * sb,

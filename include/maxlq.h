@@ -376,7 +376,9 @@ int mlq_parse_result_filename(const char* name, double* q, int* resol, int* b_it
 /* ---- measurement helpers ---- */
 /* time `reps` launches of the 3-D real transform pair of a batch of `batch`
  * fields on the context stream with HIP events; per-pass average kernel
- * durations (ms) are returned in t_ms[0..5] = {x fwd, y fwd, z fwd, z inv, y inv, x inv} */
+ * durations (ms) are returned in t_ms[0..5] = {x fwd, y fwd, z fwd, z inv, y inv, x inv}
+ * and the batch timed in t_ms[6] (`batch`, at most the transform scratch:
+ * 12 spectra, fewer where they would take a large share of the device) */
 int mlq_time_fft(mlq_ctx* ctx, int batch, int reps, double* t_ms);
 /* time `reps` launches of a plain device copy of `nfields` real fields (the
  * calibration kernel for HBM byte counters); average ms per launch in *t_ms */

@@ -977,6 +977,7 @@ template <typename T> void Engine<T>::time_fft(int batch, int reps, double* t_ms
   }
   prof_flush();
   for (int c = 0; c < 6; ++c) t_ms[c] = prof_count_[c] ? prof_ms_[c] / prof_count_[c] : 0.0;
+  t_ms[6] = batch;
   prof_enable(was);
 }
 

@@ -17,7 +17,8 @@ module maxlq_c
    integer(c_int), parameter :: MLQ_LOG_ITER = 1, MLQ_LOG_LINEMIN = 2, MLQ_LOG_BRENT = 3, &
                                 MLQ_LOG_MSG = 4, MLQ_LOG_ERR = 5, MLQ_LOG_SPECTRUM = 6, &
                                 MLQ_LOG_FIELD = 7, MLQ_LOG_KAPPA = 8
-   integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1, MLQ_RCG_SPECTRA_RAW = 2, MLQ_RCG_KAPPA = 4
+   integer(c_int), parameter :: MLQ_RCG_REFERENCE = 1, MLQ_RCG_SPECTRA_RAW = 2, MLQ_RCG_KAPPA = 4, &
+                                 MLQ_RCG_LEAN = 8
    integer(c_int), parameter :: MLQ_PERT_SINE = 0, MLQ_PERT_DIVFREE_SINE = 1, MLQ_PERT_LOAD_RAW = 2, &
                                 MLQ_PERT_LOAD = 3, MLQ_PERT_LOAD_SMOOTH = 4, &
                                 MLQ_PERT_LOAD_SMOOTH_RAW = 5, MLQ_PERT_LOAD_K = 6, &
@@ -356,7 +357,7 @@ module maxlq_c
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx
          integer(c_int), value :: batch, reps
-         real(c_double), intent(out) :: t_ms(6)
+         real(c_double), intent(out) :: t_ms(7)
       end function
       integer(c_int) function mlq_calculate_spectral_data(ctx, u, normalize, spectral_data) &
             bind(C, name="mlq_calculate_spectral_data")

@@ -410,10 +410,13 @@ class MaxLq:
         return res, tr
 
     # -- measurement ------------------------------------------------------------------------
-    def time_fft(self, batch=3, reps=5):
-        out = np.zeros(6)
+    def time_fft(self, batch=3, reps=5, with_batch=False):
+        """Per-pass average launch durations (ms) of the transform pair of a
+        batch of fields; with_batch: also the batch timed (at most the
+        transform scratch)."""
+        out = np.zeros(7)
         self._chk(self.lib.mlq_time_fft(self.ctx, batch, reps, _p(out)))
-        return out
+        return (out[:6], int(out[6])) if with_batch else out[:6]
 
     def autotune(self, batch=3):
         """Time the strided-pass tile candidates; returns (times[8][8] ms,
