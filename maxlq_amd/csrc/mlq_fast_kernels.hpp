@@ -1153,25 +1153,25 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd_ls(LsArgs<T> a) {
   LdsRow<T> L{sm + f * LP};
   for (int b = 0; b < a.nb; ++b) {
     const T tb = (T)a.tau[b];
-    c2<double> v[E];
+    c2<XReg<T>> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int i = t + m * Tf;
-      v[m] = cmk<double>((double)ls_gg(r0[i], r1[i], r2[i], tb),
-                         (double)ls_gg(r0[N + i], r1[N + i], r2[N + i], tb));
+      v[m] = cmk<XReg<T>>((XReg<T>)ls_gg(r0[i], r1[i], r2[i], tb),
+                          (XReg<T>)ls_gg(r0[N + i], r1[N + i], r2[N + i], tb));
     }
-    fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{a.tw}, L);
+    fft_core<XReg<T>, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{a.tw}, L);
 #pragma unroll
-    for (int m = 0; m < E; ++m) L.st(t + m * Tf, v[m]);
+    for (int m = 0; m < E; ++m) L.st(t + m * Tf, ccvt<double>(v[m]));
     __syncthreads();
     c2<T>* d = a.spec + b * a.sstride + row0 * PX;
 #pragma unroll
     for (int m = 0; m < E / 2; ++m) {
       const int k = t + m * Tf;
       if (k > a.keep) continue;
-      const c2<double> zk = ccvt<double>(L.ld(k)), zn = ccvt<double>(L.ld((N - k) & (N - 1)));
-      d[k] = cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y)));
-      d[PX + k] = cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x)));
+      const c2<T> zk = L.ld(k), zn = L.ld((N - k) & (N - 1));
+      d[k] = cmk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+      d[PX + k] = cmk<T>((T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x));
     }
     __syncthreads();   // the LDS rows are reused by the next step's transform
   }

@@ -273,6 +273,15 @@ template <int N, int NF> __device__ __forceinline__ int wave_field(int b) {
     return b;
 }
 
+// Register type of the x-pass transforms: the field precision, so that an
+// FP32 transform's first radix pass (no twiddles) runs in FP32 as in the
+// column passes; fft_pass keeps every later pass FP64 in registers.
+#ifndef MLQ_XP0_F32
+#define MLQ_XP0_F32 1
+#endif
+template <typename T>
+using XReg = typename std::conditional<MLQ_XP0_F32 != 0, T, double>::type;
+
 // ---------------------------------------------------------------------------
 // x pass, forward: two real rows packed as one complex row (a + i b), one
 // N-point FFT, split into the two half spectra.
@@ -281,13 +290,13 @@ template <int N, int NF> __device__ __forceinline__ int wave_field(int b) {
 // the split into the half spectra at d (row0) and d + PX (row0 + 1); only
 // kx <= kmax is stored.
 template <typename T, int N, bool ZB = false>
-__device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t,
+__device__ __forceinline__ void xfwd_pair(c2<XReg<T>> (&v)[FftShape<N>::E], int t,
                                           const double2* __restrict__ tw, const LdsRow<T>& L,
                                           c2<T>* d, int kmax) {
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   // ZB (kmax < 3N/8): entries 3N/8 .. 5N/8 - 1 of the packed row are never
   // read by the split below
-  fft_core<double, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
+  fft_core<XReg<T>, N, E, -1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
   static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
   // The split needs Z[k] and Z[N - k] (rounded to the field precision, as an
   // LDS exchange would).  For k = t + m Tf, m < E/2, element N - k sits in
@@ -302,14 +311,15 @@ __device__ __forceinline__ void xfwd_pair(c2<double> (&v)[FftShape<N>::E], int t
     const c2<T> sh = shfl_c2(z[E - 1 - m], p, Tf);
     const int k = t + m * Tf;
     if (k > kmax) continue;
-    const c2<double> zk = ccvt<double>(z[m]), zn = ccvt<double>(t == 0 ? z[(E - m) % E] : sh);
-    gst<nt_st<PC_XF>()>(d + k, cmk<T>((T)(0.5 * (zk.x + zn.x)), (T)(0.5 * (zk.y - zn.y))));
-    gst<nt_st<PC_XF>()>(d + PX + k, cmk<T>((T)(0.5 * (zk.y + zn.y)), (T)(-0.5 * (zk.x - zn.x))));
+    // in the field precision: the sum rounds once and the halving is exact,
+    // as the rounding of the sum formed in FP64
+    const c2<T> zk = z[m], zn = t == 0 ? z[(E - m) % E] : sh;
+    gst<nt_st<PC_XF>()>(d + k, cmk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y)));
+    gst<nt_st<PC_XF>()>(d + PX + k, cmk<T>((T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x)));
   }
   if (t == 0 && kmax == N / 2) {
-    const c2<double> zz = ccvt<double>(z[E / 2]);
-    gst<nt_st<PC_XF>()>(d + N / 2, cmk<T>((T)zz.x, (T)0));
-    gst<nt_st<PC_XF>()>(d + PX + N / 2, cmk<T>((T)zz.y, (T)0));
+    gst<nt_st<PC_XF>()>(d + N / 2, cmk<T>(z[E / 2].x, (T)0));
+    gst<nt_st<PC_XF>()>(d + PX + N / 2, cmk<T>(z[E / 2].y, (T)0));
   }
 }
 
@@ -325,10 +335,11 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const T* s = a.rsrc[b] + row0 * N;
-  c2<double> v[E];
+  c2<XReg<T>> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m)
-    v[m] = cmk<double>((double)gld<nt_ld<PC_XF>()>(s + t + m * Tf), (double)gld<nt_ld<PC_XF>()>(s + N + t + m * Tf));
+    v[m] = cmk<XReg<T>>((XReg<T>)gld<nt_ld<PC_XF>()>(s + t + m * Tf),
+                        (XReg<T>)gld<nt_ld<PC_XF>()>(s + N + t + m * Tf));
   LdsRow<T> L{sm + f * LP};
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   xfwd_pair<T, N, ZB>(v, t, a.tw, L, a.spec + b * a.sstride + row0 * PX, kmax);
@@ -361,9 +372,10 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
   __syncthreads();
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const T* s = stg + 2 * LP * f;
-  c2<double> v[E];
+  c2<XReg<T>> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
+  for (int m = 0; m < E; ++m)
+    v[m] = cmk<XReg<T>>((XReg<T>)s[t + m * Tf], (XReg<T>)s[N + t + m * Tf]);
   __syncthreads();   // the transform rewrites the staged rows
   LdsRow<T> L{sm + f * LP};
   const int kmax = PR ? a.keep : N / 2;
@@ -378,7 +390,8 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xfwd_p(P prod, XArgs
 template <typename T, int N, bool PR, bool ZB>
 __device__ __forceinline__ void xinv_pair(const c2<T>* __restrict__ sp, int kmax, bool xik, int t,
                                           const double2* tw, const LdsRow<T>& L,
-                                          c2<double> (&v)[FftShape<N>::E]) {
+                                          c2<XReg<T>> (&v)[FftShape<N>::E]) {
+  using R = XReg<T>;
   constexpr int E = FftShape<N>::E, Tf = FftShape<N>::Tf, PX = spec_pitch<T>(N);
   static_assert(Tf <= 64 && (64 % Tf) == 0, "one transform lies within one wave");
   // Each mode is loaded once, by the thread that owns kx = t + m Tf (m < E/2)
@@ -436,14 +449,14 @@ __device__ __forceinline__ void xinv_pair(const c2<T>* __restrict__ sp, int kmax
       Bs = t == 0 ? B[E - m] : bsh;
     }
     const int k = t + m * Tf;
-    double ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
-    if (k == 0 || k == N / 2) { ay = 0.0; by = 0.0; }
+    R ax = As.x, ay = As.y, bx = Bs.x, by = Bs.y;
+    if (k == 0 || k == N / 2) { ay = (R)0; by = (R)0; }
     if (k > N / 2) { ay = -ay; by = -by; }
-    v[m] = cmk<double>(ax - by, ay + bx);
+    v[m] = cmk<R>(ax - by, ay + bx);
   }
   // ZB (pruned launches with keep < 3N/8): elements 3N/8 .. 5N/8 - 1 of the
   // packed row are exact zeros
-  fft_core<double, N, E, +1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
+  fft_core<R, N, E, +1, LdsRow<T>, ZB>(v, t, TwPow{tw}, L);
 }
 
 // x pass, inverse: one row pair per transform (xinv_pair), scaled and stored.
@@ -461,11 +474,11 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xinv(XArgs<T> a) {
   const int slot = a.xmap ? a.xin[b] : b;
   const bool xik = a.xmap && a.xik[b];
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
-  c2<double> v[E];
+  c2<XReg<T>> v[E];
   xinv_pair<T, N, PR, ZB>(a.spec + slot * a.sstride + row0 * PX, kmax, xik, t, a.tw,
                           LdsRow<T>{sm + f * LP}, v);
   T* o = a.rdst[b] + row0 * N;
-  const double sc = a.scale;
+  const XReg<T> sc = (XReg<T>)a.scale;   // 1/N^3, a power of two: exact in either precision
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     gst<nt_st<PC_XI>()>(o + t + m * Tf, (T)(v[m].x * sc));
@@ -501,14 +514,14 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xinv_fwd_p(P prod, X
   const int f = threadIdx.x / Tf, t = threadIdx.x % Tf;
   const LdsRow<T> L{sm + f * LP};
   if (f < NI) {   // uniform within each transform's lanes
-    c2<double> v[E];
+    c2<XReg<T>> v[E];
     xinv_pair<T, N, true, true>(ai.spec + ai.xin[f] * ai.sstride + row0 * PX, ai.keep,
                                 ai.xik[f] != 0, t, ai.tw, L, v);
     // the transform's last LDS reads precede its final wave barrier: its own
     // row is free for the staged values
     T* o = ai.rdst[f];
     T* s = stg + 2 * LP * f;
-    const double sc = ai.scale;
+    const XReg<T> sc = (XReg<T>)ai.scale;   // 1/N^3, a power of two
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const T x0 = (T)(v[m].x * sc), x1 = (T)(v[m].y * sc);
@@ -529,9 +542,10 @@ __global__ __launch_bounds__(P::NB* FftShape<N>::Tf) void k_xinv_fwd_p(P prod, X
   }
   __syncthreads();
   const T* s = stg + 2 * LP * f;
-  c2<double> v[E];
+  c2<XReg<T>> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = cmk<double>((double)s[t + m * Tf], (double)s[N + t + m * Tf]);
+  for (int m = 0; m < E; ++m)
+    v[m] = cmk<XReg<T>>((XReg<T>)s[t + m * Tf], (XReg<T>)s[N + t + m * Tf]);
   __syncthreads();   // the transform rewrites the staged rows
   xfwd_pair<T, N, true>(v, t, ao.tw, L, ao.spec + f * ao.sstride + row0 * PX, ao.keep);
 }
