@@ -296,7 +296,8 @@ typedef struct mlq_rcg_cfg {
  * the R sums from them, the retraction by homogeneity).  The solve takes this
  * line search by itself where the resident state applies but does not fit
  * (mlq_rcg_path 2); the flag asks for it at any size (q >= 4 without the e_u
- * variant, dealiased).  No counterpart in the reference. */
+ * variant, dealiased; MLQ_RCG_REFERENCE takes precedence).  No counterpart
+ * in the reference. */
 #define MLQ_RCG_LEAN 8
 
 typedef struct mlq_rcg_result {
