@@ -788,9 +788,14 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
     }
     constexpr int E = FftShape<N>::E, CZ = zrt_C<T, N>();
     const int ncols = N * PP;
+    // band-limited: only the tiles of kept columns (2 keep + 1 kept ky rows of
+    // ceil((keep + 1) / CZ) tiles) are launched
+    za.kcomp = keep >= 0 && 2 * keep + 1 < N;
+    const unsigned gx = za.kcomp ? (unsigned)(2 * keep + 1) * (unsigned)((keep + CZ) / CZ)
+                                 : (unsigned)((ncols + CZ - 1) / CZ);
     prof_begin(PC_ZRT);
-    hipLaunchKernelGGL((k_zrt<T, N, E, CZ>), dim3((ncols + CZ - 1) / CZ, pl.nz), dim3(CZ * (N / E)),
-                       0, st_, za, ncols);
+    hipLaunchKernelGGL((k_zrt<T, N, E, CZ>), dim3(gx, pl.nz), dim3(CZ * (N / E)), 0, st_, za,
+                       ncols);
     MLQ_HIP(hipGetLastError());
     prof_end(PC_ZRT, 2 * pl.nz * ab_kcols(keep));
   }

@@ -827,6 +827,7 @@ template <typename T> struct ZArgs {
   long long sstride;
   int keep = -1;
   int remap = 0;
+  int kcomp = 0;      // grid over the kept tiles only (band-limited launches)
   int zin[MAXB];
   int zout[MAXB];
   SpecOp op[MAXB];    // factors along z only (col_factor_ok)
@@ -858,11 +859,16 @@ __global__ __launch_bounds__(C*(N / E)) void k_zrt(ZArgs<T> a, int ncols) {
   if constexpr (TAB) tws.load(a.tw);
   unsigned bx, by;
   xcd_tile(a.remap != 0, bx, by);
-  const int col = (int)bx * C + c;
+  const int keep = a.keep;
+  int col = (int)bx * C + c;
+  if (a.kcomp) {   // tile bx of the kept rows: (keep + C) / C tiles per kept ky
+    const unsigned nt = (unsigned)(keep + C) / C, jy = bx / nt, jt = bx - jy * nt;
+    const int kyt = (int)jy <= keep ? (int)jy : (int)jy + N - 2 * keep - 1, kxt = (int)jt * C + c;
+    col = kxt < PP ? kyt * PP + kxt : ncols;   // a lane past the row pitch is invalid
+  }
   const int plane = (int)by;
   const int ky = col / PP, kx = col - ky * PP;
   const bool valid = col < ncols && kx <= N / 2;
-  const int keep = a.keep;
   const bool ok = valid && (keep < 0 || (kx <= keep && mabs(ky, N) <= keep));
   if (!__syncthreads_or(ok)) return;   // block-uniform: no column of the tile is read later
   constexpr bool NTL = nt_ld<PC_ZI>(), NTS = nt_st<PC_ZI>();
