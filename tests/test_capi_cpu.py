@@ -19,6 +19,32 @@ def test_header_and_binding_agree():
     assert set(_header_symbols()) == set(capi.EXPORTS)
 
 
+def _header_constants():
+    txt = open(os.path.join(ROOT, "include", "maxlq.h")).read()
+    return {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define\s+(MLQ_[A-Z0-9_]+)\s+(-?\d+)\b", txt)}
+
+
+def test_constants_agree_across_bindings():
+    """Every integer constant of the header that the Python or the Fortran
+    binding restates has the header's value there (flags, kinds, codes)."""
+    from maxlq_amd import capi
+    hdr = _header_constants()
+    assert "MLQ_RCG_LEAN" in hdr and "MLQ_RCG_REFERENCE" in hdr
+    py = {k: getattr(capi, k) for k in hdr if hasattr(capi, k)}
+    assert py, "no constant of the header in capi.py"
+    for k, v in py.items():
+        assert v == hdr[k], (k, v, hdr[k])
+    f90 = open(os.path.join(ROOT, "maxlq_amd", "fortran", "maxlq_c.f90")).read()
+    ft = {m.group(1): int(m.group(2)) for m in re.finditer(r"\b(MLQ_[A-Z0-9_]+)\s*=\s*(-?\d+)", f90)}
+    assert ft, "no constant in the Fortran interface"
+    for k, v in ft.items():
+        if k in hdr:
+            assert v == hdr[k], (k, v, hdr[k])
+    for k in ("MLQ_RCG_REFERENCE", "MLQ_RCG_KAPPA", "MLQ_RCG_LEAN"):
+        assert py.get(k) == hdr[k] and ft.get(k) == hdr[k], k
+
+
 def test_library_exports_every_symbol():
     from maxlq_amd import capi
     if not os.path.exists(capi.LIB_PATH):
