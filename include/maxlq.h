@@ -311,7 +311,7 @@ typedef struct mlq_rcg_result {
   int n_J_computed;      /* J evaluations computed (>= n_J with speculation) */
   double seconds;
   int n_J_poly;          /* of n_J_computed, those given by the exact line-search
-                            polynomials (q = 4, no clipping of D(phi.phi)) */
+                            polynomials (q = 4 and q = 6, no clipping of D(phi.phi)) */
   int pad_;
   double enstrophy;      /* calc_global_summed_enstrophy(Uvec) of the result and */
   double dEnstrophydt;   /* calc_global_dEnstrophydt(Uvec), for

@@ -194,6 +194,9 @@ template <typename T> class Engine final : public IEngine {
   // the chain of |phi|^k is that of sqrt(s2) phi (D(phi.phi) scaled by s2);
   // the other factors of the sums are those of phi
   void ls_generic(const LsGen<T>& f, double t, double out[4], double s2 = 1.0);
+  // q = 6: the polynomial coefficients of the R sums, and the norm sum at t
+  void ls6_coefs(const LsGen<T>& f, double r1[7], double r2[7], double r3[8]);
+  double ls6_norm(const LsGen<T>& f, double t);
   // an FP64 engine on the same grid and parameters (created on first use):
   // the mixed path evaluates its line-search J(tau) chain there, so that J is
   // a smooth function of tau (brent resolves tau to TOL = 1e-6, J to 1e-12)
@@ -371,11 +374,15 @@ template <typename T> class Engine final : public IEngine {
   double fast_path_bytes() const {
     const bool q4 = std::fabs(p_.q - 4.0) < 1e-14;
     // + 9 fields: the velocity-gradient operand of the deferred state update
-    double b = (q4 ? 57.0 : 65.0) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
+    // (q = 6: 2 more at the polynomial precompute, its 10 product fields
+    // against the 8 chain temporaries of a J evaluation)
+    const bool q6 = std::fabs(p_.q - 6.0) < 1e-14;
+    double b = (q4 ? 57.0 : (q6 ? 67.0 : 65.0)) * real_bytes() + 28.0 * (double)sizeof(C) * (double)ns_;
     b += 3.0 * kappa_fields_ * real_bytes();
     // FP32 fields, q != 4: the FP64 twin's copy of the 24 line-search fields,
     // its chain temporaries (8 fields) and its transform scratch (12 spectra)
-    if (!q4 && sizeof(T) == 4) b += 32.0 * 8.0 * (double)nr_ + 12.0 * 16.0 * (double)n_ * n_ * (n_ / 2 + 8);
+    if (!q4 && sizeof(T) == 4)
+      b += (q6 ? 34.0 : 32.0) * 8.0 * (double)nr_ + 12.0 * 16.0 * (double)n_ * n_ * (n_ / 2 + 8);
     return b;
   }
   // the reference-structured iteration takes its line search from

@@ -47,6 +47,7 @@ template <typename T> class FastRcg {
     M_ = e.mcut_of(2.0);
     q4_ = std::fabs(q_ - 4.0) < 1e-14;
     poly_ok_ = q4_ && M_ < n_ / 2;
+    q6_ = std::fabs(q_ - 6.0) < 1e-14 && M_ < n_ / 2;
     N3_ = (double)nr_;
     N6_ = N3_ * N3_;
     nkc_ = kept_calls<T>(M_);
@@ -425,6 +426,20 @@ template <typename T> class FastRcg {
     }
     cache_.clear();
     g64_fresh_ = false;
+    if (q6_) {
+      // q = 6: the R sums as polynomials in tau (FP64, on the twin for FP32
+      // fields) and the points that can clip
+      if constexpr (std::is_same<T, double>::value) {
+        e_.ls6_coefs(LsGen<T>{s_, S1_, S2_, P_, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, gam_}, r6_[0],
+                     r6_[1], r6_[2]);
+      } else {
+        Engine<double>* tw = e_.fp64_twin();
+        load_twin(tw);
+        tw->ls6_coefs(gen64_, r6_[0], r6_[1], r6_[2]);
+      }
+      e_.ls_risky(s_, S1_, S2_, risky_, nrisky_);
+      e_.template reduce<3>(R_sum3<T>{s_, S1_, S2_}, nr_, ssum_);
+    }
   }
 
   // ---- J_LineMin(tau) = -R(B phi/||phi||_q) for a batch of step sizes ---------------
@@ -774,6 +789,31 @@ template <typename T> class FastRcg {
   // depends on tau; brent then sees a smooth J and stops after as many
   // requests as in FP64.
   void eval_generic(double t, double* J, double* Lout) {
+    // q = 6 where no point clips (as at q = 4: the clipped mass below 1e-14 of
+    // sum s(tau) for FP64 fields, 1e-9 for FP32 fields, whose own rounding is
+    // larger): the R sums from their polynomials, the norm from two dealias
+    // round trips (FP64, on the twin for FP32 fields)
+    if (q6_ && poly_enabled_ && poly_exact(t)) {
+      double lq;
+      if constexpr (std::is_same<T, double>::value) {
+        lq = e_.ls6_norm(LsGen<T>{s_, S1_, S2_, P_, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, gam_}, t);
+      } else {
+        Engine<double>* tw = e_.fp64_twin();
+        if (!g64_fresh_) load_twin(tw);
+        lq = tw->ls6_norm(gen64_, t);
+      }
+      auto pv = [t](const double* c, int n) {
+        double v = 0.0;
+        for (int k = n - 1; k >= 0; --k) v = v * t + c[k];
+        return v;
+      };
+      const double R1 = -nuvc_ * pv(r6_[0], 7) / N3_;
+      const double R2 = -nuvc_ * (q_ - 2.0) * pv(r6_[1], 7) / N3_;
+      const double R3 = pc_ * (q_ - 2.0) * pv(r6_[2], 8) / N3_;
+      ++nJpoly_;
+      finish_J(t, R1 + R2, R3, lq / N3_, J, Lout);
+      return;
+    }
     double acc[4];
     if constexpr (std::is_same<T, double>::value) {
       e_.ls_generic(LsGen<T>{s_, S1_, S2_, P_, Q0_, Q1_, Q2_, a_, b_, ee_, u_, d_, gam_}, t, acc);
@@ -812,6 +852,8 @@ template <typename T> class FastRcg {
   int n_, pp_, M_;
   long long nr_, ns_;
   bool q4_, poly_ok_;
+  bool q6_ = false;                   // q = 6: the line-search sums by polynomials
+  double r6_[3][8] = {};              // their coefficients: R1, R2 (degree 6), R3 (degree 7)
   long long nkc_;                     // calls of a kept-cube kernel (kw<T>() modes each)
   double* hlt_ = nullptr;             // squared H_l weight by |m|^2
   double2* hwlt_ = nullptr;           // (Hilbert, squared H_l) weights by |m|^2
@@ -855,6 +897,9 @@ template <typename T> class FastRcg {
   // the polynomials are used at tau when the mass clipped by max(s(tau), 0)
   // is negligible against sum s(tau): 1e-14 (FP64) or 1e-9 (FP32) relative
   bool poly_exact(double t) const {
+    return poly_exact(t, sizeof(T) == sizeof(double) ? 1.0e-14 : 1.0e-9);
+  }
+  bool poly_exact(double t, double tol) const {
     if (nrisky_ > 0 && risky_.empty()) return false;   // too many points to account for
     double m = 0.0;
     for (size_t i = 0; i < risky_.size(); i += 3) {
@@ -862,7 +907,6 @@ template <typename T> class FastRcg {
       if (v < 0.0) m -= v;
     }
     const double tot = ssum_[0] + t * (ssum_[1] + t * ssum_[2]);
-    const double tol = sizeof(T) == sizeof(double) ? 1.0e-14 : 1.0e-9;
     return tot > 0.0 && m <= tol * tot;
   }
   std::map<double, double> cache_;
@@ -1052,6 +1096,55 @@ void Engine<T>::ls_generic(const LsGen<T>& f, double t, double out[4], double s2
   }
   reduce<4>(R_ls_generic<T>{A, Bwp, Pd, Lh, f.Q0, f.Q1, f.Q2, f.a, f.b, f.e, f.u, f.d, nr_, t,
                             f.gam}, nr_, out);
+}
+
+// q = 6: the coefficients of the R1, R2 and R3 sums of J as polynomials in
+// tau (unnormalised, r1 and r2 of degree 6, r3 of degree 7), valid where
+// max(s(tau), 0) does not clip (see F_ls6_prod): |phi|^4 = D(s^2) from the
+// dealiased C_k, the pressure term D(p s) from the dealiased E_k, |phi|^2 = s
+template <typename T>
+void Engine<T>::ls6_coefs(const LsGen<T>& f, double r1[7], double r2[7], double r3[8]) {
+  RT C = rtmp(5), E = rtmp(5);
+  map(F_ls6_prod<T>{f.S0, f.S1, f.S2, f.P, f.P + nr_, f.P + 2 * nr_, nr_, C, E}, nr_);
+  {
+    T* fl[10];
+    for (int k = 0; k < 5; ++k) {
+      fl[k] = C + k * nr_;
+      fl[5 + k] = E + k * nr_;
+    }
+    dealias_dev(fl, 10, 2.0);
+  }
+  double aq[15], aw[15], b[20];
+  reduce<15>(R_ls6q<T>{C, f.Q0, f.Q1, f.Q2, nr_}, nr_, aq);
+  reduce<15>(R_ls6w<T>{f.S0, f.S1, f.S2, f.a, f.b, f.e, nr_}, nr_, aw);
+  reduce<20>(R_ls6b<T>{E, f.u, f.d, f.a, f.b, f.e, nr_, f.gam}, nr_, b);
+  for (int m = 0; m < 7; ++m) r1[m] = r2[m] = 0.0;
+  for (int m = 0; m < 8; ++m) r3[m] = 0.0;
+  for (int k = 0; k < 5; ++k)
+    for (int j = 0; j < 3; ++j) r1[k + j] += aq[3 * k + j];
+  for (int i = 0; i < 3; ++i)
+    for (int n = 0; n < 5; ++n) r2[i + n] += aw[5 * i + n];
+  for (int k = 0; k < 5; ++k)
+    for (int l = 0; l < 4; ++l) r3[k + l] += b[4 * k + l];
+}
+
+// q = 6: sum over the grid of |phi|^6 = (|phi|^3)^2 as the chain forms it
+// where s(tau) does not clip: |phi|^3 = D(D(g g) D_1(g)), g = sqrt(s), with
+// D(g g) = s (s is band-limited) and D_1 the order-1 dealias
+template <typename T> double Engine<T>::ls6_norm(const LsGen<T>& f, double t) {
+  RT s1 = rtmp(), g = rtmp();
+  map(F_quad<T>{f.S0, f.S1, f.S2, s1, (T)t, (T)1}, nr_);
+  map(F_sqrtclip<T>{s1, g}, nr_);
+  {
+    T* p = g;
+    dealias_dev(&p, 1, 1.0);
+  }
+  map(F_mul<T>{s1, g, g}, nr_);
+  {
+    T* p = g;
+    dealias_dev(&p, 1, 2.0);
+  }
+  return dot(g, g);
 }
 
 template <typename T> Engine<double>* Engine<T>::fp64_twin() {

@@ -250,6 +250,105 @@ template <typename T> struct R_ls_poly {
   }
 };
 
+// ---------------------------------------------------------------------------
+// q = 6 line search by polynomials.  Where max(s(tau), 0) does not clip
+// (s = S0 + tau S1 + tau^2 S2), the chain of J gives |phi|^4 = D(s^2),
+// |phi|^2 = s and the pressure term D(p(tau) s(tau)): R1, R2, R3 are
+// polynomials in tau whose coefficients are sums over the grid of products of
+// coefficient fields.  C_k = sum_{i+j=k} S_i S_j and E_k = sum_{i+j=k} P_i S_j
+// (k = 0..4), dealiased by the caller.
+template <typename T> struct F_ls6_prod {
+  const T* S0; const T* S1; const T* S2; const T* P0; const T* P1; const T* P2; long long nr;
+  T* C; T* E;   // 5 fields each (stride nr)
+  __device__ void operator()(long long i) const {
+    const T s0 = S0[i], s1 = S1[i], s2 = S2[i], p0 = P0[i], p1 = P1[i], p2 = P2[i];
+    C[i] = s0 * s0;
+    C[nr + i] = (T)2 * s0 * s1;
+    C[2 * nr + i] = s1 * s1 + (T)2 * s0 * s2;
+    C[3 * nr + i] = (T)2 * s1 * s2;
+    C[4 * nr + i] = s2 * s2;
+    E[i] = p0 * s0;
+    E[nr + i] = p0 * s1 + p1 * s0;
+    E[2 * nr + i] = p0 * s2 + p1 * s1 + p2 * s0;
+    E[3 * nr + i] = p1 * s2 + p2 * s1;
+    E[4 * nr + i] = p2 * s2;
+  }
+};
+// acc[3k + j] = sum D(C_k) Q_j (the R1 sums)
+template <typename T> struct R_ls6q {
+  static constexpr int unroll = 1;
+  const T* DC; const T* Q0; const T* Q1; const T* Q2; long long nr;
+  double abytes() const { return 8.0 * sizeof(T); }
+  __device__ void operator()(long long i, double (&acc)[15]) const {
+    const double q[3] = {(double)Q0[i], (double)Q1[i], (double)Q2[i]};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double c = DC[k * nr + i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc[3 * k + j] += c * q[j];
+    }
+  }
+};
+// acc[5i + n] = sum S_i c_n with sum_j w_j(tau)^2 = sum_n c_n tau^n (the R2 sums)
+template <typename T> struct R_ls6w {
+  static constexpr int unroll = 1;
+  const T* S0; const T* S1; const T* S2; const T* a; const T* b; const T* e; long long nr;
+  double abytes() const { return 12.0 * sizeof(T); }
+  __device__ void operator()(long long i, double (&acc)[15]) const {
+    const double sv[3] = {(double)S0[i], (double)S1[i], (double)S2[i]};
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double aj = a[j * nr + i], bj = b[j * nr + i], ej = e[j * nr + i];
+      c0 += aj * aj;
+      c1 += 2.0 * aj * bj;
+      c2 += bj * bj + 2.0 * aj * ej;
+      c3 += 2.0 * bj * ej;
+      c4 += ej * ej;
+    }
+    const double cn[5] = {c0, c1, c2, c3, c4};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) acc[5 * m + n] += sv[m] * cn[n];
+  }
+};
+// acc[4k + l] = sum D(E_k) v_l with phi.w(tau) = sum_l v_l tau^l, phi = u +
+// tau (d + gam u) (the R3 sums)
+template <typename T> struct R_ls6b {
+  static constexpr int unroll = 1;
+  const T* DE; const T* u; const T* d; const T* a; const T* b; const T* e; long long nr;
+  double gam;
+  double abytes() const { return 20.0 * sizeof(T); }
+  __device__ void operator()(long long i, double (&acc)[20]) const {
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double aj = a[j * nr + i], bj = b[j * nr + i], ej = e[j * nr + i];
+      const double uj = u[j * nr + i], dj = (double)d[j * nr + i] + gam * uj;
+      v0 += uj * aj;
+      v1 += uj * bj + dj * aj;
+      v2 += uj * ej + dj * bj;
+      v3 += dj * ej;
+    }
+    const double vl[4] = {v0, v1, v2, v3};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double c = DE[k * nr + i];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) acc[4 * k + l] += c * vl[l];
+    }
+  }
+};
+template <typename T> struct R_sum3 {
+  const T* a; const T* b; const T* c;
+  __device__ void operator()(long long i, double (&acc)[3]) const {
+    acc[0] += (double)a[i];
+    acc[1] += (double)b[i];
+    acc[2] += (double)c[i];
+  }
+};
+
 // g_b = sqrt(max(S0 + tau_b S1 + tau_b^2 S2, 0)) and gg_b = g_b * g_b for a
 // batch of step sizes (calc_uk, function_ops.f90:2332-2339, at phi = u + tau d)
 template <typename T> struct F_ls_g {

@@ -161,6 +161,33 @@ def test_eval_J_batch_fast_vs_oracle(q):
     assert np.max(np.abs(J1 - J)) <= 1e-12 * np.max(np.abs(J))
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-11)])
+def test_q6_linesearch_polynomials_vs_chain(precision, tol):
+    """q = 6: J(tau) from the polynomial R sums and the two-round-trip norm
+    against the full |phi|^k chain evaluation on the same fields, at step
+    sizes where D(phi.phi) does not clip (both FP64: the mixed path
+    evaluates J on its FP64 twin)."""
+    from maxlq_amd.lq import MaxLq
+    n, q, B = 64, 6.0, 1.0
+    u = seeded_field(n, 31, "gauss", 16.0, q=q, B=B)
+    d = seeded_field(n, 32, "gauss", 16.0, q=q, B=B)
+    g = MaxLq(n, q=q, B=B, precision=precision)
+    hu, hd = g.field(3), g.field(3)
+    g.upload(hu, u)
+    g.upload(hd, 0.1 * d)
+    taus = 10.0 ** np.linspace(-4, -1, 8)
+    Jp, _, _ = g.time_linesearch(hu, hd, taus, -1)   # polynomials where exact
+    Jc, _, _ = g.time_linesearch(hu, hd, taus, 3)    # every J through the chain
+    rel = np.max(np.abs(Jp - Jc) / np.abs(Jc))
+    print(precision, "q=6 polynomial vs chain J: max rel", rel)
+    assert rel < tol
+    # the solve takes the polynomials for its step sizes on this kind of field
+    r, _, _ = _solve_gpu(u, n, q, B, 2, batch=3, precision=precision)
+    print(precision, "q=6 solve: J requests", r.n_J, "computed", r.n_J_computed,
+          "by polynomials", r.n_J_poly)
+    assert r.n_J_poly > 0
+
+
 def test_mixed_vs_fp64_rcg_32():
     n, q, B, iters = 32, 4.0, 1.0, 3
     u0 = seeded_field(n, 3, "gauss", 16.0, q=q, B=B)
