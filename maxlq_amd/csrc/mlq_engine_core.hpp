@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdio>
 #include <cstdlib>
+#include <iterator>
 #include "mlq_engine_impl.hpp"
 
 namespace mlq {
@@ -51,8 +52,8 @@ template <typename T> Engine<T>::~Engine() {
     if (f.phys) hipFree(f.phys);
     if (f.spec) hipFree(f.spec);
   }
-  for (auto& kv : pool_)
-    for (auto p : kv.second) hipFree(p);
+  for (auto& sl : slabs_)
+    if (sl.base) hipFree(sl.base);
   for (auto& r : prof_recs_) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
   for (auto e : ev_pool_) hipEventDestroy(e);
   hipFree(tw_);
@@ -69,9 +70,7 @@ template <typename T> Engine<T>::~Engine() {
 template <typename T> bool Engine<T>::fast_path_fits() const {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) { (void)hipGetLastError(); return true; }
-  double pooled = 0.0;
-  for (const auto& kv : pool_) pooled += (double)kv.first * (double)kv.second.size();
-  return fast_path_bytes() <= 0.95 * ((double)fr + pooled);
+  return fast_path_bytes() <= 0.95 * ((double)fr + (double)pooled_);
 }
 
 template <typename T> void Engine<T>::resolve_params() {
@@ -81,17 +80,19 @@ template <typename T> void Engine<T>::resolve_params() {
 
 template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
   void* p = nullptr;
-  static const bool log = std::getenv("MLQ_ALLOC_LOG") != nullptr;
-  if (log) std::fprintf(stderr, "[mlq] device allocation of %zu bytes (in use %zu)\n", bytes, used_);
+  const bool log = alloc_log_level() > 0;
+  double t0 = log ? alloc_log_ms() : 0.0;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
-    // the pool caches freed temporaries by size; give them back and retry
+    // give back the slabs that hold no temporary and retry
     (void)hipGetLastError();
-    MLQ_HIP(hipStreamSynchronize(st_));
-    for (auto& kv : pool_)
-      for (void* q : kv.second) dfree(q, kv.first);
-    pool_.clear();
-    pooled_ = 0;
+    const size_t had = release_free_slabs();
+    if (log) {
+      const double t1 = alloc_log_ms();
+      std::fprintf(stderr, "[mlq] at %.0f ms: %zu bytes of free slabs given back for %zu: %.1f ms\n",
+                   t0, had, bytes, t1 - t0);
+      t0 = t1;
+    }
     e = hipMalloc(&p, bytes);
   }
   if (e != hipSuccess) {
@@ -101,29 +102,119 @@ template <typename T> void* Engine<T>::dmalloc(size_t bytes) {
   used_ += bytes;
   peak_ = std::max(peak_, used_);
   note_live();
+  if (log)
+    std::fprintf(stderr, "[mlq] at %.0f ms: device allocation of %zu bytes (in use %zu): %.1f ms\n",
+                 t0, bytes, used_, alloc_log_ms() - t0);
   return p;
 }
 template <typename T> void Engine<T>::dfree(void* p, size_t bytes) {
   if (!p) return;
+  const double t0 = alloc_log_level() > 0 ? alloc_log_ms() : 0.0;
   MLQ_HIP(hipStreamSynchronize(st_));
   MLQ_HIP(hipFree(p));
   used_ -= bytes;
+  if (alloc_log_level() > 0)
+    std::fprintf(stderr, "[mlq] at %.0f ms: device free of %zu bytes: %.1f ms\n", t0, bytes,
+                 alloc_log_ms() - t0);
+}
+
+template <typename T> size_t Engine<T>::release_free_slabs() {
+  size_t had = 0;
+  for (auto& sl : slabs_) {
+    if (!sl.base || sl.fr.size() != 1 || sl.fr.begin()->second != sl.size) continue;
+    pooled_ -= sl.size;
+    had += sl.size;
+    dfree(sl.base, sl.size);
+    sl = Slab{};
+  }
+  return had;
+}
+
+template <typename T> void* Engine<T>::carve(size_t bytes) {
+  constexpr size_t G = 65536;   // blocks start on 64 KiB boundaries
+  bytes = (bytes + G - 1) / G * G;
+  int bs = -1;
+  size_t bo = 0, bl = 0;
+  for (int s = 0; s < (int)slabs_.size(); ++s)
+    for (const auto& r : slabs_[s].fr)
+      if (r.second >= bytes && (bs < 0 || r.second < bl)) { bs = s; bo = r.first; bl = r.second; }
+  if (bs < 0) {
+    // a new slab: at least 9 real fields (the largest temporaries are
+    // 9-component gradients), or just this block when that does not fit
+    const size_t want = std::max(bytes, (sizeof(T) * (size_t)nr_ * 9 + G - 1) / G * G);
+    void* p = nullptr;
+    size_t got = want;
+    if (want > bytes && hipMalloc(&p, want) == hipSuccess) {
+      used_ += want;
+      peak_ = std::max(peak_, used_);
+      if (alloc_log_level() > 0)
+        std::fprintf(stderr, "[mlq] at %.0f ms: slab of %zu bytes (in use %zu)\n", alloc_log_ms(),
+                     want, used_);
+    } else {
+      (void)hipGetLastError();
+      got = bytes;
+      p = dmalloc(bytes);
+    }
+    Slab sl;
+    sl.base = (char*)p;
+    sl.size = got;
+    sl.fr[0] = got;
+    pooled_ += got;
+    bs = -1;
+    for (int s = 0; s < (int)slabs_.size(); ++s)
+      if (!slabs_[s].base) { bs = s; break; }
+    if (bs < 0) { bs = (int)slabs_.size(); slabs_.emplace_back(); }
+    slabs_[bs] = std::move(sl);
+    bo = 0;
+    bl = got;
+  }
+  Slab& sl = slabs_[bs];
+  sl.fr.erase(bo);
+  if (bl > bytes) sl.fr[bo + bytes] = bl - bytes;
+  char* p = sl.base + bo;
+  carved_[p] = {bs, bytes};
+  pooled_ -= bytes;
+  note_live();
+  return p;
+}
+
+template <typename T> void Engine<T>::pool_put(void* p, size_t bytes) {
+  if (alloc_log_level() > 1) std::fprintf(stderr, "[mlq] put %zu %p\n", bytes, p);
+  const auto it = carved_.find((const char*)p);
+  if (it == carved_.end()) {
+    std::fprintf(stderr, "[mlq] temporary %p was not carved from a slab\n", p);
+    std::abort();
+  }
+  Slab& sl = slabs_[it->second.first];
+  size_t off = (size_t)((const char*)p - sl.base), len = it->second.second;
+  carved_.erase(it);
+  pooled_ += len;
+  // merge with the free ranges either side
+  auto nx = sl.fr.lower_bound(off);
+  if (nx != sl.fr.end() && off + len == nx->first) {
+    len += nx->second;
+    nx = sl.fr.erase(nx);
+  }
+  if (nx != sl.fr.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) {
+      pv->second += len;
+      return;
+    }
+  }
+  sl.fr[off] = len;
 }
 
 template <typename T> typename Engine<T>::RT Engine<T>::rtmp(int k) {
   const size_t bytes = sizeof(T) * nr_ * k;
-  auto& v = pool_[bytes];
-  void* p;
-  if (!v.empty()) { p = v.back(); v.pop_back(); pooled_ -= bytes; note_live(); }
-  else p = dmalloc(bytes);
+  void* p = carve(bytes);
+  if (alloc_log_level() > 1) std::fprintf(stderr, "[mlq] get %zu %p\n", bytes, p);
   return RT(this, (T*)p, bytes);
 }
 template <typename T> typename Engine<T>::ST Engine<T>::stmp(int k) {
-  const size_t bytes = sizeof(C) * ns_ * k + 64;  // distinct key from real blocks
-  auto& v = pool_[bytes];
-  void* p;
-  if (!v.empty()) { p = v.back(); v.pop_back(); pooled_ -= bytes; note_live(); }
-  else p = dmalloc(bytes);
+  const size_t bytes = sizeof(C) * ns_ * k;
+  void* p = carve(bytes);
+  if (alloc_log_level() > 1) std::fprintf(stderr, "[mlq] get %zu %p\n", bytes, p);
   return ST(this, (C*)p, bytes);
 }
 

@@ -3,8 +3,11 @@
 // parity path; float: the mixed path, whose reductions and scalars stay FP64).
 #pragma once
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -13,6 +16,21 @@
 #include "mlq_pointwise.hpp"
 
 namespace mlq {
+
+// MLQ_ALLOC_LOG=1: one stderr line per device allocation, pool flush and free,
+// with the host milliseconds each took; 2 adds each temporary taken from and
+// returned to the pool
+static inline double alloc_log_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static inline int alloc_log_level() {
+  static const int level = [] {
+    const char* s = std::getenv("MLQ_ALLOC_LOG");
+    return s ? std::max(1, std::atoi(s)) : 0;
+  }();
+  return level;
+}
 
 template <typename T> class Engine;
 template <typename T> class FastRcg;
@@ -202,13 +220,11 @@ template <typename T> class Engine final : public IEngine {
   // a smooth function of tau (brent resolves tau to TOL = 1e-6, J to 1e-12)
   Engine<double>* fp64_twin();
 
-  // pool of device temporaries, keyed by size
+  // device temporaries: k real fields (rtmp) or k spectra (stmp), carved
+  // out of slabs and given back to them when the Tmp goes
   RT rtmp(int k = 1);
   ST stmp(int k = 1);
-  void pool_put(void* p, size_t bytes) {
-    pool_[bytes].push_back(p);
-    pooled_ += bytes;
-  }
+  void pool_put(void* p, size_t bytes);
 
  private:
   // ---- set-up ----
@@ -344,10 +360,29 @@ template <typename T> class Engine final : public IEngine {
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;
   std::unique_ptr<IEngine> twin_;   // fp64_twin()
-  std::map<size_t, std::vector<void*>> pool_;
+  // Slabs of device memory that the temporaries are carved from: best fit
+  // over the free ranges of all slabs, neighbouring free ranges merged when a
+  // block comes back, a new slab (at least 9 real fields) only when no free
+  // range is large enough.  The temporaries of one phase of an iteration
+  // (9-component gradients, 3-component fields, single components) take
+  // other shapes than those of the next; carving lets one phase reuse the
+  // memory of the last, where a pool of blocks keyed by size runs the device
+  // out of memory, gives its blocks back and allocates them again - and
+  // device memory freed and allocated again is cleared before it is handed
+  // out, ~28 ms per GB (1024^3: 6 s of a 10 s iteration).  Slabs are freed
+  // only when an allocation fails otherwise.
+  struct Slab {
+    char* base = nullptr;   // nullptr: given back
+    size_t size = 0;
+    std::map<size_t, size_t> fr;   // free ranges: offset -> length
+  };
+  std::vector<Slab> slabs_;
+  std::map<const char*, std::pair<int, size_t>> carved_;   // block -> slab, length
+  void* carve(size_t bytes);
+  size_t release_free_slabs();
   size_t used_ = 0, peak_ = 0;
-  // bytes held by the pool (allocated, not in use) and the high-water mark
-  // of the bytes in use (allocated minus pooled)
+  // bytes of the slabs not carved out (allocated, not in use) and the
+  // high-water mark of the bytes in use (allocated minus pooled)
   size_t pooled_ = 0, live_peak_ = 0;
   void note_live() { live_peak_ = std::max(live_peak_, used_ - pooled_); }
   // profiling

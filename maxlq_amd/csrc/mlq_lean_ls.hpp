@@ -154,9 +154,7 @@ template <typename T> struct LeanLs {
 template <typename T> bool Engine<T>::lean_ls_fits(int held) const {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) { (void)hipGetLastError(); return false; }
-  double pooled = 0.0;
-  for (const auto& kv : pool_) pooled += (double)kv.first * (double)kv.second.size();
-  return LeanLs<T>::bytes(*this) + held * real_bytes() <= 0.95 * ((double)fr + pooled);
+  return LeanLs<T>::bytes(*this) + held * real_bytes() <= 0.95 * ((double)fr + (double)pooled_);
 }
 
 }  // namespace mlq
