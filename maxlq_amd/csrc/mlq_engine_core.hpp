@@ -2,7 +2,6 @@
 #pragma once
 #include <cstdio>
 #include <cstdlib>
-#include <iterator>
 #include "mlq_engine_impl.hpp"
 
 namespace mlq {
@@ -52,8 +51,7 @@ template <typename T> Engine<T>::~Engine() {
     if (f.phys) hipFree(f.phys);
     if (f.spec) hipFree(f.spec);
   }
-  for (auto& sl : slabs_)
-    if (sl.base) hipFree(sl.base);
+  slabs_.for_each_slab([](char* base, size_t) { hipFree(base); });
   for (auto& r : prof_recs_) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
   for (auto e : ev_pool_) hipEventDestroy(e);
   hipFree(tw_);
@@ -119,32 +117,21 @@ template <typename T> void Engine<T>::dfree(void* p, size_t bytes) {
 }
 
 template <typename T> size_t Engine<T>::release_free_slabs() {
-  size_t had = 0;
-  for (auto& sl : slabs_) {
-    if (!sl.base || sl.fr.size() != 1 || sl.fr.begin()->second != sl.size) continue;
-    pooled_ -= sl.size;
-    had += sl.size;
-    dfree(sl.base, sl.size);
-    sl = Slab{};
-  }
+  const size_t had = slabs_.release_free([&](char* base, size_t size) { dfree(base, size); });
+  pooled_ -= had;
   return had;
 }
 
 template <typename T> void* Engine<T>::carve(size_t bytes) {
-  constexpr size_t G = 65536;   // blocks start on 64 KiB boundaries
-  bytes = (bytes + G - 1) / G * G;
-  int bs = -1;
-  size_t bo = 0, bl = 0;
-  for (int s = 0; s < (int)slabs_.size(); ++s)
-    for (const auto& r : slabs_[s].fr)
-      if (r.second >= bytes && (bs < 0 || r.second < bl)) { bs = s; bo = r.first; bl = r.second; }
-  if (bs < 0) {
+  bytes = SlabSet::round(bytes);
+  char* p = slabs_.take(bytes);
+  if (!p) {
     // a new slab: at least 9 real fields (the largest temporaries are
     // 9-component gradients), or just this block when that does not fit
-    const size_t want = std::max(bytes, (sizeof(T) * (size_t)nr_ * 9 + G - 1) / G * G);
-    void* p = nullptr;
+    const size_t want = std::max(bytes, SlabSet::round(sizeof(T) * (size_t)nr_ * 9));
+    void* base = nullptr;
     size_t got = want;
-    if (want > bytes && hipMalloc(&p, want) == hipSuccess) {
+    if (want > bytes && hipMalloc(&base, want) == hipSuccess) {
       used_ += want;
       peak_ = std::max(peak_, used_);
       if (alloc_log_level() > 0)
@@ -153,26 +140,12 @@ template <typename T> void* Engine<T>::carve(size_t bytes) {
     } else {
       (void)hipGetLastError();
       got = bytes;
-      p = dmalloc(bytes);
+      base = dmalloc(bytes);
     }
-    Slab sl;
-    sl.base = (char*)p;
-    sl.size = got;
-    sl.fr[0] = got;
+    slabs_.add((char*)base, got);
     pooled_ += got;
-    bs = -1;
-    for (int s = 0; s < (int)slabs_.size(); ++s)
-      if (!slabs_[s].base) { bs = s; break; }
-    if (bs < 0) { bs = (int)slabs_.size(); slabs_.emplace_back(); }
-    slabs_[bs] = std::move(sl);
-    bo = 0;
-    bl = got;
+    p = slabs_.take(bytes);
   }
-  Slab& sl = slabs_[bs];
-  sl.fr.erase(bo);
-  if (bl > bytes) sl.fr[bo + bytes] = bl - bytes;
-  char* p = sl.base + bo;
-  carved_[p] = {bs, bytes};
   pooled_ -= bytes;
   note_live();
   return p;
@@ -180,29 +153,12 @@ template <typename T> void* Engine<T>::carve(size_t bytes) {
 
 template <typename T> void Engine<T>::pool_put(void* p, size_t bytes) {
   if (alloc_log_level() > 1) std::fprintf(stderr, "[mlq] put %zu %p\n", bytes, p);
-  const auto it = carved_.find((const char*)p);
-  if (it == carved_.end()) {
+  const size_t len = slabs_.give(p);
+  if (!len) {
     std::fprintf(stderr, "[mlq] temporary %p was not carved from a slab\n", p);
     std::abort();
   }
-  Slab& sl = slabs_[it->second.first];
-  size_t off = (size_t)((const char*)p - sl.base), len = it->second.second;
-  carved_.erase(it);
   pooled_ += len;
-  // merge with the free ranges either side
-  auto nx = sl.fr.lower_bound(off);
-  if (nx != sl.fr.end() && off + len == nx->first) {
-    len += nx->second;
-    nx = sl.fr.erase(nx);
-  }
-  if (nx != sl.fr.begin()) {
-    auto pv = std::prev(nx);
-    if (pv->first + pv->second == off) {
-      pv->second += len;
-      return;
-    }
-  }
-  sl.fr[off] = len;
 }
 
 template <typename T> typename Engine<T>::RT Engine<T>::rtmp(int k) {

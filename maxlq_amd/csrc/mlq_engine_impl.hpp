@@ -14,6 +14,7 @@
 
 #include "mlq_engine.hpp"
 #include "mlq_pointwise.hpp"
+#include "mlq_slab.hpp"
 
 namespace mlq {
 
@@ -360,24 +361,16 @@ template <typename T> class Engine final : public IEngine {
   double* red_host_ = nullptr;
   std::vector<DevField> fields_;
   std::unique_ptr<IEngine> twin_;   // fp64_twin()
-  // Slabs of device memory that the temporaries are carved from: best fit
-  // over the free ranges of all slabs, neighbouring free ranges merged when a
-  // block comes back, a new slab (at least 9 real fields) only when no free
-  // range is large enough.  The temporaries of one phase of an iteration
-  // (9-component gradients, 3-component fields, single components) take
-  // other shapes than those of the next; carving lets one phase reuse the
-  // memory of the last, where a pool of blocks keyed by size runs the device
-  // out of memory, gives its blocks back and allocates them again - and
-  // device memory freed and allocated again is cleared before it is handed
-  // out, ~28 ms per GB (1024^3: 6 s of a 10 s iteration).  Slabs are freed
-  // only when an allocation fails otherwise.
-  struct Slab {
-    char* base = nullptr;   // nullptr: given back
-    size_t size = 0;
-    std::map<size_t, size_t> fr;   // free ranges: offset -> length
-  };
-  std::vector<Slab> slabs_;
-  std::map<const char*, std::pair<int, size_t>> carved_;   // block -> slab, length
+  // Slabs of device memory that the temporaries are carved from (SlabSet: best
+  // fit, neighbouring free ranges merged when a block comes back); a new slab
+  // of at least 9 real fields only when no free range is large enough.  A
+  // pool of blocks keyed by size runs the device out of memory when the
+  // shapes change from one phase of an iteration to the next, gives its
+  // blocks back and allocates them again - and device memory freed and
+  // allocated again is cleared before it is handed out, ~28 ms per GB
+  // (1024^3: 6 s of a 10 s iteration).  Slabs are freed only when an
+  // allocation fails otherwise.
+  SlabSet slabs_;
   void* carve(size_t bytes);
   size_t release_free_slabs();
   size_t used_ = 0, peak_ = 0;
