@@ -779,7 +779,8 @@ int Engine<T>::rtrip_plan(int nin, int nout, const RtOut* outs, RtPlan& pl) cons
 }
 
 template <typename T> template <int N>
-void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep) {
+void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep,
+                        const T* const* mul) {
   constexpr int NF = xpass_NF<N>();
   constexpr int Tf = FftShape<N>::Tf;
   constexpr int PX = N / 2 + 1, PP = spec_pitch<T>(N);
@@ -789,7 +790,12 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
   // forward x and y passes of the inputs into slots 0..nin-1
   {
     XArgs<T> xa{};
-    for (int b = 0; b < nin; ++b) xa.rsrc[b] = src[b];
+    int nmul = 0;
+    for (int b = 0; b < nin; ++b) {
+      xa.rsrc[b] = src[b];
+      xa.rmul[b] = mul ? mul[b] : nullptr;
+      nmul += xa.rmul[b] != nullptr;
+    }
     xa.spec = scratch_;
     xa.tw = tw_;
     xa.sstride = ns_;
@@ -808,7 +814,7 @@ void Engine<T>::rtrip_n(int nin, const T* const* src, int nout, const RtOut* out
       hipLaunchKernelGGL((k_xfwd<T, N, NF, false>), dim3((unsigned)(pairs / NF)), dim3(NF * Tf), 0,
                          st_, xa);
     MLQ_HIP(hipGetLastError());
-    prof_end(PC_XFWD, nin * (real_bytes() + ab_xcols(keep)));
+    prof_end(PC_XFWD, (nin + nmul) * real_bytes() + nin * ab_xcols(keep));
     SArgs<T> sa{};
     sa.spec = scratch_;
     sa.tw = tw_;
@@ -953,6 +959,19 @@ void Engine<T>::inv(int nb, const InvSrc* src, const SpecOp* ops, T* const* dst,
   }
 }
 
+template <typename T>
+void Engine<T>::mul_dealias(int nb, const T* const* a, const T* const* b, T* const* out,
+                            bool dealias, double order) {
+  const int mc = mcut_of(order);
+  if (!dealias || mask_is_identity(mc)) {
+    for (int i = 0; i < nb; ++i) map(F_mul<T>{a[i], b[i], out[i]}, nr_);
+    return;
+  }
+  std::vector<RtOut> o(nb);
+  for (int i = 0; i < nb; ++i) o[i] = RtOut{i, sp_mask(mc), out[i]};
+  rtrip(nb, a, nb, o.data(), b);
+}
+
 template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double order) {
   const int mc = mcut_of(order);
   if (mask_is_identity(mc)) return;   // identity mask: the round trip only adds round-off
@@ -967,7 +986,8 @@ template <typename T> void Engine<T>::dealias_dev(T* const* f, int nb, double or
 // last pass, after every read of its inputs: an output may alias an input of
 // its own group (dealias in place), never an input of a later group.
 template <typename T>
-void Engine<T>::rtrip(int nin, const T* const* src, int nout, const RtOut* outs) {
+void Engine<T>::rtrip(int nin, const T* const* src, int nout, const RtOut* outs,
+                      const T* const* mul) {
   int keep = -1;
   for (int o = 0; o < nout; ++o) {
     const SpecOp& op = outs[o].op;
@@ -997,12 +1017,15 @@ void Engine<T>::rtrip(int nin, const T* const* src, int nout, const RtOut* outs)
         RtOut r = outs[o];
         r.in = 0;
         const T* s = src[i0];
-        MLQ_DISPATCH_N(rtrip_n, 1, &s, 1, &r, keep);
+        const T* m = mul ? mul[i0] : nullptr;
+        MLQ_DISPATCH_N(rtrip_n, 1, &s, 1, &r, keep, &m);
       }
       i0 += 1;
       continue;
     }
-    if (!g.empty()) MLQ_DISPATCH_N(rtrip_n, i1 - i0, src + i0, (int)g.size(), g.data(), keep);
+    if (!g.empty())
+      MLQ_DISPATCH_N(rtrip_n, i1 - i0, src + i0, (int)g.size(), g.data(), keep,
+                     mul ? mul + i0 : nullptr);
     i0 = i1;
   }
 }

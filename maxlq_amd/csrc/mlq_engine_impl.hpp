@@ -284,14 +284,25 @@ template <typename T> class Engine final : public IEngine {
     inv(1, &s, &op, &dst);
   }
   void dealias_dev(T* const* f, int nb, double order);  // in place, batch
+  // out[i] = a[i] b[i], dealiased (order `order`) when `dealias`: the product
+  // is formed by the forward x pass of the round trip instead of being stored
+  // first.  out[i] may be a[i] or b[i].
+  void mul_dealias(int nb, const T* const* a, const T* const* b, T* const* out, bool dealias,
+                   double order = 2.0);
+  void mul_dealias(const T* a, const T* b, T* out, bool dealias) {
+    mul_dealias(1, &a, &b, &out, dealias);
+  }
   // Round trips dst = REAL(IFFT(op * FFT(src[in]))) of physical fields, nout
   // outputs of nin inputs; the z transforms are fused (k_zrt), so no
   // spectrum is stored.  An output may overwrite its own input only when it
-  // is that input's only output (dealias in place).
+  // is that input's only output (dealias in place).  mul: when given, input
+  // b is the pointwise product src[b] * mul[b] (mul[b] may be null).
   struct RtOut { int in; SpecOp op; T* dst; };
-  void rtrip(int nin, const T* const* src, int nout, const RtOut* outs);
+  void rtrip(int nin, const T* const* src, int nout, const RtOut* outs,
+             const T* const* mul = nullptr);
   template <int N>
-  void rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep);
+  void rtrip_n(int nin, const T* const* src, int nout, const RtOut* outs, int keep,
+               const T* const* mul);
   // slot plan of one rtrip launch group; returns the scratch slots it needs
   struct RtPlan {
     int nz = 0, zin[MAXB], zout[MAXB];

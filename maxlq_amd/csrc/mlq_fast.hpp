@@ -1139,11 +1139,7 @@ template <typename T> double Engine<T>::ls6_norm(const LsGen<T>& f, double t) {
     T* p = g;
     dealias_dev(&p, 1, 1.0);
   }
-  map(F_mul<T>{s1, g, g}, nr_);
-  {
-    T* p = g;
-    dealias_dev(&p, 1, 2.0);
-  }
+  mul_dealias(s1, g, g, true);
   return dot(g, g);
 }
 

@@ -125,6 +125,9 @@ template <typename T> struct XArgs {
   int xmap = 0;
   int xin[MAXB];
   int xik[MAXB];
+  // forward x pass (k_xfwd): a non-null rmul[b] makes input b the pointwise
+  // product rsrc[b] * rmul[b], formed in T as a pointwise pass would store it
+  const T* rmul[MAXB] = {};
 };
 
 template <typename T> struct SArgs {
@@ -335,11 +338,23 @@ __global__ __launch_bounds__(NF* FftShape<N>::Tf) void k_xfwd(XArgs<T> a) {
   const long long lp = gp - (long long)b * pairs_per;
   const long long row0 = 2 * lp;
   const T* s = a.rsrc[b] + row0 * N;
+  const T* mu = a.rmul[b];
   c2<XReg<T>> v[E];
+  if (mu) {   // uniform within the field's waves
+    mu += row0 * N;
 #pragma unroll
-  for (int m = 0; m < E; ++m)
-    v[m] = cmk<XReg<T>>((XReg<T>)gld<nt_ld<PC_XF>()>(s + t + m * Tf),
-                        (XReg<T>)gld<nt_ld<PC_XF>()>(s + N + t + m * Tf));
+    for (int m = 0; m < E; ++m) {
+      const T p0 = gld<nt_ld<PC_XF>()>(s + t + m * Tf) * gld<nt_ld<PC_XF>()>(mu + t + m * Tf);
+      const T p1 =
+          gld<nt_ld<PC_XF>()>(s + N + t + m * Tf) * gld<nt_ld<PC_XF>()>(mu + N + t + m * Tf);
+      v[m] = cmk<XReg<T>>((XReg<T>)p0, (XReg<T>)p1);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = cmk<XReg<T>>((XReg<T>)gld<nt_ld<PC_XF>()>(s + t + m * Tf),
+                          (XReg<T>)gld<nt_ld<PC_XF>()>(s + N + t + m * Tf));
+  }
   LdsRow<T> L{sm + f * LP};
   const int kmax = PR ? a.keep : N / 2;   // PR: pruned launch (a.keep >= 0)
   xfwd_pair<T, N, ZB>(v, t, a.tw, L, a.spec + b * a.sstride + row0 * PX, kmax);

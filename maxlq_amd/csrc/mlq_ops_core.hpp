@@ -45,13 +45,11 @@ template <typename T> const T* Engine<T>::chain_pow(Chain& ch, double k) {
   } else {                                                     // :2305-2317
     if (!ch.haveG2) {
       ch.G2 = rtmp();
-      map(F_mul<T>{ch.g, ch.g, ch.G2}, nr_);
-      if (p_.toDealias) { T* f = ch.G2; dealias_dev(&f, 1, 2.0); }
+      mul_dealias(ch.g, ch.g, ch.G2, p_.toDealias);
       ch.haveG2 = true;
     }
     const T* prev = chain_pow(ch, k - 2.0);
-    map(F_mul<T>{ch.G2, prev, out}, nr_);
-    if (p_.toDealias) { T* f = out; dealias_dev(&f, 1, 2.0); }
+    mul_dealias(ch.G2, prev, out, p_.toDealias);
   }
   T* r = out.get();
   ch.memo.emplace(key, std::move(out));
@@ -148,8 +146,7 @@ template <typename T> void Engine<T>::dLqdt_dev(const T* u, double q, double out
     rtrip(1, &z, 1, &o);
   }
   RT P2 = rtmp();
-  map(F_mul<T>{pr, Bq, P2}, nr_);                                 // :1343/:1349
-  if (D) { T* f = P2; dealias_dev(&f, 1, 2.0); }
+  mul_dealias(pr, Bq, P2, D);                                     // :1343/:1349
   // v.(v.grad)u with the same inner products w_j                   :1345/:1351
   RT V = rtmp();
   map(F_dot3<T>{v, nr_, w0, w1, w2, V}, nr_);
@@ -216,21 +213,24 @@ template <typename T> void Engine<T>::gradL2_dev(const T* u, T* const* Vout) {
       map(F_unit<T>{u, s, nr_, ev}, nr_);                          // :1117
       map(F_udotforce<T>{ev, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);
       if (D && p_.dealiase_if_mult_by_e_u) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-      map(F_mul<T>{A, aux2, aux2}, nr_);
-      if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-      for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, ev + i * nr_, aux3v + i * nr_}, nr_);
-      if (D && p_.dealiase_if_mult_by_e_u) {
-        T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_};
-        dealias_dev(f, 3, 2.0);
+      mul_dealias(A, aux2, aux2, D);
+      {
+        const T* a3[3] = {aux2, aux2, aux2};
+        const T* b3[3] = {ev, ev + nr_, ev + 2 * nr_};
+        T* o3[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_};
+        mul_dealias(3, a3, b3, o3, D && p_.dealiase_if_mult_by_e_u);
       }
     } else {
       map(F_udotforce<T>{u, DeltaU, gradP, nr_, (T)(vc * nu), (T)pc, aux2}, nr_);  // :1131-1133
       if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
       const T* Bq = chain_pow(ch, q - 4.0);                        // :1136
-      map(F_mul<T>{Bq, aux2, aux2}, nr_);
-      if (D) { T* f = aux2; dealias_dev(&f, 1, 2.0); }
-      for (int i = 0; i < 3; ++i) map(F_mul<T>{aux2, u + i * nr_, aux3v + i * nr_}, nr_);
-      if (D) { T* f[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_}; dealias_dev(f, 3, 2.0); }
+      mul_dealias(Bq, aux2, aux2, D);
+      {
+        const T* a3[3] = {aux2, aux2, aux2};
+        const T* b3[3] = {u, u + nr_, u + 2 * nr_};
+        T* o3[3] = {aux3v, aux3v + nr_, aux3v + 2 * nr_};
+        mul_dealias(3, a3, b3, o3, D);
+      }
     }
   }
   // grad of Delta^-1 div(|u|^(q-2)u) and Delta of D(|u|^(q-2)u)       :1150-1158, 1172-1173
@@ -558,8 +558,11 @@ template <typename T> void Engine<T>::constraint_derivative(int u_vec, int out_v
                               1.0 / q);                                     // :993
   const T* A = chain_pow(ch, q - 2.0);                                      // :995
   T* o[3] = {fphys(out_vec, 0), fphys(out_vec, 1), fphys(out_vec, 2)};
-  for (int i = 0; i < 3; ++i) map(F_mul<T>{A, u + i * nr_, o[i]}, nr_);
-  if (p_.toDealias && (q - 2.0 > 2.0e-16)) dealias_dev(o, 3, 2.0);
+  {
+    const T* a3[3] = {A, A, A};
+    const T* b3[3] = {u, u + nr_, u + 2 * nr_};
+    mul_dealias(3, a3, b3, o, p_.toDealias && (q - 2.0 > 2.0e-16));
+  }
   map(F_scale<T>{o[0], (T)std::pow(nrm, 1.0 - q)}, nr_ * 3);               // :1003
 }
 
@@ -573,8 +576,11 @@ void Engine<T>::vector_transport(int u, double q, double B, int eta, int xi, int
   chain_init(ch, upe);
   const T* A = chain_pow(ch, q - 2.0);                                     // :1735
   T* aux[3] = {up[0], up[1], up[2]};
-  for (int i = 0; i < 3; ++i) map(F_mul<T>{A, upe + i * nr_, aux[i]}, nr_);
-  if (p_.toDealias && (q - 2.0 > 2.0e-16)) dealias_dev(aux, 3, 2.0);       // :1739
+  {
+    const T* a3[3] = {A, A, A};
+    const T* b3[3] = {upe, upe + nr_, upe + 2 * nr_};
+    mul_dealias(3, a3, b3, aux, p_.toDealias && (q - 2.0 > 2.0e-16));   // :1739
+  }
   double ir = 0.0;
   for (int i = 0; i < 3; ++i) ir += dot(aux[i], fphys(xi, i)) / (double)nr_;   // :1742
   const T* a2 = chain_pow(ch, q / 2.0);
