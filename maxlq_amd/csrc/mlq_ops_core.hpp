@@ -40,8 +40,15 @@ template <typename T> const T* Engine<T>::chain_pow(Chain& ch, double k) {
   if (k + eps < 0.0) {                                         // :2291-2301
     map(F_pow<T>{ch.g, out, k, 1}, nr_);
   } else if (k + eps < 2.0) {                                  // :2302-2304
-    map(F_pow<T>{ch.g, out, k, 0}, nr_);
-    if (p_.toDealias) { T* f = out; dealias_dev(&f, 1, k); }
+    if (k == 1.0 && p_.toDealias && !mask_is_identity(mcut_of(k))) {
+      // g^1 = g: the round trip reads g itself
+      const T* src = ch.g;
+      const RtOut o{0, sp_mask(mcut_of(k)), out};
+      rtrip(1, &src, 1, &o);
+    } else {
+      map(F_pow<T>{ch.g, out, k, 0}, nr_);
+      if (p_.toDealias) { T* f = out; dealias_dev(&f, 1, k); }
+    }
   } else {                                                     // :2305-2317
     if (!ch.haveG2) {
       ch.G2 = rtmp();

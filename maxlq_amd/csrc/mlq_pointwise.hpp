@@ -99,6 +99,9 @@ template <typename T> struct F_pow {
     if (neg) { o[i] = (fabs((double)v) < 2.0e-16) ? (T)0 : (T)pow((double)v, k); return; }
     if (k == 0.0) { o[i] = (T)1; return; }
     if (k == 1.0) { o[i] = v; return; }
+    // x^(1/2) correctly rounded (what a correctly rounded pow returns; g >= 0
+    // here), at a fraction of pow's cost
+    if (k == 0.5) { o[i] = (T)sqrt((double)v); return; }
     o[i] = (T)pow((double)v, k);
   }
 };
