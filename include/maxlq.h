@@ -413,10 +413,10 @@ int mlq_sizes(const mlq_ctx* ctx, double* real_bytes, double* spec_bytes);
 int mlq_prof_enable(mlq_ctx* ctx, int on);
 int mlq_prof_get(mlq_ctx* ctx, int cls, long long* count, double* total_ms, double* bytes);
 const char* mlq_prof_name(int cls);
-/* device bytes in use by the context (its allocations minus the temporaries
- * cached for reuse) and their high-water mark since mlq_init: the memory the
- * work needed, where mlq_mem_info's peak also counts cached temporaries.  No
- * counterpart in the reference. */
+/* device bytes in use by the context (its allocations minus the free space of
+ * the slabs its temporaries are carved from) and their high-water mark since
+ * mlq_init: the memory the work needed, where mlq_mem_info's peak also counts
+ * the slabs' free space.  No counterpart in the reference. */
 int mlq_mem_live(mlq_ctx* ctx, double* live_bytes, double* peak_live_bytes);
 int mlq_mem_info(mlq_ctx* ctx, double* used_bytes, double* peak_bytes, double* device_free,
                  double* device_total);

@@ -468,7 +468,8 @@ class MaxLq:
         return tuple(x.value for x in v)
 
     def mem_live(self):
-        """(bytes in use, their peak): allocations minus cached temporaries."""
+        """(bytes in use, their peak): allocations minus the free space of the
+        slabs that temporaries are carved from."""
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.mlq_mem_live(self.ctx, C.byref(a), C.byref(b)))
         return a.value, b.value
