@@ -1071,10 +1071,9 @@ namespace mlq {
 template <typename T>
 void Engine<T>::ls_generic(const LsGen<T>& f, double t, double out[4], double s2) {
   const double q = p_.q;
-  RT s1 = rtmp();
-  map(F_quad<T>{f.S0, f.S1, f.S2, s1, (T)t, (T)s2}, nr_);
-  Chain ch;
-  chain_init(ch, nullptr, s1);
+  Chain ch;   // g = sqrt(max(s(t), 0)) in one pass (s itself is not needed)
+  ch.g = rtmp();
+  map(F_quad_sqrtclip<T>{f.S0, f.S1, f.S2, ch.g, (T)t, (T)s2}, nr_);
   const T* A = chain_pow(ch, q - 2.0);
   const T* Bq = chain_pow(ch, q - 4.0);
   const T* Lh = chain_pow(ch, q / 2.0);

@@ -433,6 +433,25 @@ template <typename T> struct F_quad {
     st16(o, iv, r);
   }
 };
+// g = sqrt(max(s, 0)) of s = sc (x0 + t x1 + t^2 x2) as F_quad then
+// F_sqrtclip compute it, without storing s
+template <typename T> struct F_quad_sqrtclip {
+  static constexpr int vec = V16<T>::w;
+  const T* x0; const T* x1; const T* x2; T* o; T t, sc;
+  double abytes() const { return 4.0 * sizeof(T); }
+  __device__ void operator()(long long iv) const {
+    T a[vec], b[vec], c[vec], r[vec];
+    ld16(x0, iv, a);
+    ld16(x1, iv, b);
+    ld16(x2, iv, c);
+#pragma unroll
+    for (int l = 0; l < vec; ++l) {
+      const T v = sc * (a[l] + t * (b[l] + t * c[l]));
+      r[l] = v <= (T)0 ? (T)0 : sqrt(v);
+    }
+    st16(o, iv, r);
+  }
+};
 template <typename T> struct S_quad {
   const c2<T>* x0; const c2<T>* x1; const c2<T>* x2; c2<T>* o; double t, sc;
   __device__ void operator()(long long i) const {
